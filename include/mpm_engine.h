@@ -1,0 +1,234 @@
+/*
+ * mpm_engine.h -- C ABI of the MI355X MLS-MPM step engine.
+ *
+ * Drop-in boundary for the step API of Summer-Particle-Lab's `MPM3` class
+ * (src/three/mpm.js:7-136). Plain C types only: opaque handles, host
+ * pointers and sizes. The Node.js N-API addon (summer-particle-lab_amd/node)
+ * and the Python ctypes wrapper (summer-particle-lab_amd/mpm_amd) both bind
+ * exactly these symbols.
+ *
+ * Conventions
+ *  - Every call returns an int status: MPM_OK (0) or an MPM_ERR_* code;
+ *    mpm_last_error() gives the message of the last failure on a handle.
+ *  - One engine owns one HIP stream on the current device; a handle is not
+ *    thread-safe (callers serialise), distinct handles are independent.
+ *  - Host arrays are caller-owned and read/written only inside the call.
+ *  - Particle tensors F, C, cached_tau are 9 values per particle, row-major,
+ *    particle-major: the layout of MPM3.F / MPM3.C / MPM3.cachedTau
+ *    (src/three/mpm.js:9,12).
+ *  - Particle volume is the physical particle volume V_p. The reference stores
+ *    V_p/64 in `s.volume` and multiplies by 64 in P2G (src/three/mpm.js:68);
+ *    a reference-unit caller passes 64*s.volume.
+ *  - Device state is FP32 structure-of-arrays; FP64 appears on the device only
+ *    in the conservation reductions (mpm_conservation).
+ */
+#ifndef MPM_ENGINE_H
+#define MPM_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 7 and 8 mirror the reference GPU service's status codes for
+ * "MPM CFL limit exceeded" and "GPU state became invalid"
+ * (gpu/mixed_solver.py:381-396). */
+enum {
+  MPM_OK = 0,
+  MPM_ERR_INVALID_ARGUMENT = 1,
+  MPM_ERR_HIP = 2,
+  MPM_ERR_CAPACITY = 3,
+  MPM_ERR_UNSUPPORTED = 4,
+  MPM_ERR_NOT_READY = 5,
+  MPM_ERR_CFL = 7,
+  MPM_ERR_INVALID_STATE = 8
+};
+
+/* Grid wall rule per domain face, applied to nodes within one cell of the
+ * face. SEPARATE is the reference rule (zero the outward normal component
+ * only, src/three/mpm.js:93); STICKY zeroes all components; SLIP zeroes the
+ * normal component in both directions. */
+enum { MPM_WALL_SEPARATE = 0, MPM_WALL_STICKY = 1, MPM_WALL_SLIP = 2 };
+
+/* Array element type of a particle view. */
+enum { MPM_F64 = 0, MPM_F32 = 1 };
+
+/* Constitutive models, numbered as the reference's MODELS table
+ * (src/three/constitutive.js:6-19). Supported: 0 fixed-corotated elastic,
+ * 4 ideal J2, 5 linear-hardening J2, 6 Johnson-Cook, 7 simplified
+ * Johnson-Cook, 10 thermal J2, 11 Norton-Arrhenius creep. */
+enum {
+  MPM_MODEL_FIXED_COROTATED = 0,
+  MPM_MODEL_J2 = 4,
+  MPM_MODEL_J2_HARDENING = 5,
+  MPM_MODEL_JOHNSON_COOK = 6,
+  MPM_MODEL_JOHNSON_COOK_SIMPLE = 7,
+  MPM_MODEL_THERMAL_J2 = 10,
+  MPM_MODEL_CREEP = 11
+};
+
+typedef struct mpm_engine mpm_engine;
+
+/* Engine geometry and global options; replaces MPM3.configure(domain)
+ * (src/three/mpm.js:18) and the domain rules of domainParameters
+ * (src/three/discretization.js:4-15): origin = floor(lower/h)*h - 2h,
+ * nodes = ceil((upper - origin)/h) + 3 per axis, grid edge = max over axes. */
+typedef struct {
+  double lower[3], upper[3]; /* domain box */
+  double cell_size;          /* h */
+  uint32_t capacity;         /* particle capacity */
+  int32_t wall_mode[6];      /* faces -x,+x,-y,+y,-z,+z: MPM_WALL_* */
+  double gravity[3];         /* constant body acceleration */
+  int32_t gravity_on_particles; /* 1: add g*dt to particle v before P2G (the
+                                   reference caller's order,
+                                   src/three/simulation.js:337-344); 0: add
+                                   g*dt to grid velocity after p/m */
+  double mass_eps;           /* grid nodes with mass <= eps keep raw momentum
+                                (reference: 1e-12) */
+  int32_t sort_every;        /* re-bin particles every k steps (>= 1) */
+  int32_t strict_domain;     /* 1: apply the reference's domain limits */
+} mpm_config;
+
+/* One material. Strengths are normalised by E as in the reference records
+ * (src/three/thermal-plasticity.js:1-4, src/three/metal-plasticity.js:1-4).
+ * mu, lambda are absolute Lame parameters; the reference uses
+ * mu = lambda = 0.4 E (src/three/mpm.js:37). */
+typedef struct {
+  int32_t model; /* MPM_MODEL_* */
+  float E, mu, lambda;
+  /* thermal J2 / creep (models 10, 11) */
+  float th_Tref, th_Thot, th_hotRatio, th_A, th_H, th_rate, th_Q, th_n, th_stress, th_kelvin_offset;
+  /* J2 / Johnson-Cook (models 4-7) */
+  float pl_A, pl_B, pl_n, pl_C, pl_rate0, pl_Tref, pl_Tm, pl_m;
+} mpm_material;
+
+/* Host view of particle state (structure of arrays). On upload every pointer
+ * except the optional ones must be non-NULL; on download NULL pointers are
+ * skipped. dtype selects FP64 or FP32 for all floating arrays. */
+typedef struct {
+  uint32_t count;
+  int32_t dtype;
+  void *x, *y, *z, *vx, *vy, *vz; /* [count] */
+  void *F, *C;                    /* [count*9] */
+  void *mass, *volume, *temperature; /* [count] */
+  void *plastic;                  /* accumulated plastic strain ep [count] */
+  void *cached_tau;               /* optional [count*9]: stress cached by the
+                                     last plastic return (MPM3.cachedTau) */
+  uint8_t *material;              /* index into the material table [count] */
+  uint8_t *pinned;                /* 1 = pinned (MPM3 skips G2P) [count] */
+  uint8_t *cache_valid;           /* optional [count] (MPM3.cacheValid) */
+} mpm_particles;
+
+/* FP64 conservation report. Grid totals are those of the most recent P2G. */
+typedef struct {
+  double particle_mass;      /* sum m_p */
+  double grid_mass;          /* sum over nodes (fixed-point total / scale) */
+  int64_t particle_mass_q;   /* sum of per-particle fixed-point masses */
+  int64_t grid_mass_q;       /* sum of node fixed-point masses; equals
+                                particle_mass_q exactly after every P2G */
+  double mass_scale;         /* fixed-point units per unit mass (2^k) */
+  double momentum[3];        /* sum m_p v_p */
+  double grid_momentum[3];   /* sum of node momenta after P2G */
+  double angular[3];         /* sum x_p x m_p v_p + m_p D spin(C_p), D = h^2/4
+                                (src/simulation/mechanical-diagnostics.js:19-31) */
+  double kinetic;            /* sum 1/2 m |v|^2 */
+  double affine;             /* sum 1/2 m D |C|^2 */
+  double min_J;              /* min det F */
+  double max_speed;          /* max |v_p| */
+  int64_t nonfinite;         /* particles with a non-finite x, v or F */
+} mpm_conservation;
+
+/* Per-kernel timing accumulated by HIP events while profiling is on. */
+typedef struct {
+  char name[32];
+  int64_t launches;
+  double total_ms;
+} mpm_kernel_time;
+
+/* Create / destroy; replaces `new MPM3(capacity)` + configure(domain). */
+int mpm_create(const mpm_config* config, mpm_engine** out);
+int mpm_destroy(mpm_engine* e);
+const char* mpm_last_error(const mpm_engine* e);
+
+/* Material table (up to 64 entries); replaces the per-part records the
+ * reference passes to projectPlastic (src/three/mpm.js:127). */
+int mpm_set_materials(mpm_engine* e, const mpm_material* materials, int32_t count);
+
+/* Upload particle state (replaces reading s.x.. and MPM3's own arrays at the
+ * start of MPM3.step). Resets the fixed-point mass scale and re-bins. */
+int mpm_upload(mpm_engine* e, const mpm_particles* p);
+
+/* Download particle state in the caller's original particle order. */
+int mpm_download(mpm_engine* e, mpm_particles* p);
+
+/* Advance nsteps steps of size dt; replaces MPM3.step(s, dt)
+ * (src/three/mpm.js:56-134). Asynchronous on the engine stream; errors of
+ * the device state surface at the next synchronising call. */
+int mpm_step(mpm_engine* e, double dt, int32_t nsteps);
+
+/* CFL-type stable step of the current state; replaces MPM3.stableStep(s)
+ * (src/three/mpm.js:43-55). cap <= 0 uses h/8/240, cfl <= 0 uses 1.4/8 (the
+ * reference constants in reference units). */
+int mpm_stable_dt(mpm_engine* e, double cap, double cfl, double* out);
+
+/* FP64 conservation reductions of the current state (see mpm_conservation);
+ * min_J replaces MPM3.minJ(s) (src/three/mpm.js:135). Synchronising. */
+int mpm_conservation_report(mpm_engine* e, mpm_conservation* out);
+
+/* Per-particle von Mises diagnostic (Cauchy von Mises / E), in the caller's
+ * particle order; replaces MPM3.refreshDiagnostics (src/three/mpm.js:40-42).
+ * stress is [count] FP64. */
+int mpm_refresh_diagnostics(mpm_engine* e, double* stress);
+
+/* Wait for all work of the engine. */
+int mpm_synchronize(mpm_engine* e);
+
+/* Number of particles resident on the device. */
+int mpm_count(const mpm_engine* e, uint32_t* out);
+
+/* ---- measurement and parity hooks (no reference counterpart) ---- */
+
+/* Turn per-kernel HIP-event timing on (1) or off (0); reset clears totals. */
+int mpm_profile(mpm_engine* e, int32_t enable, int32_t reset);
+/* Copy up to max kernel timing records; *n receives the record count. */
+int mpm_kernel_times(mpm_engine* e, mpm_kernel_time* out, int32_t max, int32_t* n);
+
+/* Bin the resident particles and return, in device slot order, the cell key
+ * of every slot before sorting (keys_in), the sorted keys (keys_out) and the
+ * sorted permutation of original particle ids (perm). Any pointer may be
+ * NULL. Keys are (bz*N + by)*N + bx of the base cell
+ * b = clamp(floor((x - O)/h - 1/2), 0, N - 3) on FP32 positions
+ * (src/three/mpm.js:28). */
+int mpm_debug_bin(mpm_engine* e, uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm);
+
+/* Stable LSD radix sort of arbitrary 32-bit keys on the device; values are
+ * the input positions. Used to test the binning sort on edge cases. */
+int mpm_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t key_bits, uint32_t* keys_out, uint32_t* perm_out);
+
+/* One P2G transfer of the resident state with step dt, leaving the dense
+ * grid: mass [N^3] and momentum [3*N^3] as FP64 copies (node-major,
+ * x fastest). Does not advance the state. */
+int mpm_debug_p2g(mpm_engine* e, double dt, double* grid_mass, double* grid_momentum, int32_t* N);
+
+/* The engine's plain constitutive update applied on the device to a batch of
+ * n deformation gradients F [9n] with plastic strain ep [n] and temperature
+ * T [n]: for models >= 2 the plastic return of the step (SVD, radial return,
+ * recomposed F, ep += increment, Hencky stress of the returned state); for
+ * model 0 F is unchanged and tau is the fixed-corotated stress. tau_out is
+ * [6n] (xx, yy, zz, xy, xz, yz). Replaces the reference's per-particle
+ * projectPlastic call (src/three/mpm.js:127, src/three/constitutive.js:52-78),
+ * and is the label generator of the learned constitutive surrogate. */
+int mpm_return_map_batch(const mpm_material* m, uint32_t n, float dt, const float* F, const float* ep,
+                         const float* T, float* F_out, float* ep_out, float* tau_out);
+
+/* Step stress of a batch of F without a cached return (updateStress,
+ * src/three/mpm.js:33-39): fixed-corotated for model 0, Hencky from F's own
+ * decomposition for plastic models. tau_out [6n]. */
+int mpm_stress_batch(const mpm_material* m, uint32_t n, const float* F, float* tau_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_ENGINE_H */

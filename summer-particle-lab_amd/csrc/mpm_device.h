@@ -1,0 +1,301 @@
+// Device-side constitutive math for the MLS-MPM engine (FP32).
+//
+// Each routine restates the algorithm of a reference function; the arithmetic
+// is written for FP32 registers, not transcribed:
+//   stencil weights      src/three/mpm.js:25-32
+//   polar factor         src/three/math.js:16-24 (Newton R <- (R + R^-T)/2)
+//   fixed-corotated tau  src/three/math.js:25-32
+//   principal stretches  src/three/constitutive.js:22-42 (Jacobi on F^T F); the
+//                        rotations use the Golub-Van Loan symmetric Schur
+//                        formula instead of atan2/cos/sin
+//   thermal J2 / creep   src/three/thermal-plasticity.js:12-40
+//   J2 / Johnson-Cook    src/three/metal-plasticity.js:12-38
+//   Hencky return stress src/three/constitutive.js:81-86
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+// Material record as the kernels see it: Lame parameters in absolute units
+// plus the reference's E-normalised strengths. G = mu/E, lr = lambda/mu.
+struct DevMaterial {
+  int32_t model;
+  float E, mu, lambda, G, lr;
+  float th_Tref, th_Thot, th_hotRatio, th_A, th_H, th_rate, th_Q, th_n, th_stress, th_kelvin;
+  float pl_A, pl_B, pl_n, pl_C, pl_rate0, pl_Tref, pl_Tm, pl_m;
+  float pad[2];
+};
+
+struct M3 {
+  float a[9];
+  __device__ __forceinline__ float& operator[](int i) { return a[i]; }
+  __device__ __forceinline__ float operator[](int i) const { return a[i]; }
+};
+
+__device__ __forceinline__ float det3(const M3& a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+// Quadratic B-spline weights of the three nodes b, b+1, b+2 at offset t=q-b.
+__device__ __forceinline__ void bspline(float t, float w[3]) {
+  const float a = 1.5f - t, b = t - 1.0f, c = t - 0.5f;
+  w[0] = 0.5f * a * a;
+  w[1] = 0.75f - b * b;
+  w[2] = 0.5f * c * c;
+}
+
+// Newton polar iteration R <- (R + R^-T)/2 (at most 8 rounds; FP32 stop
+// threshold on the summed absolute change), identity when near singular.
+__device__ __forceinline__ void polar_rotation(const M3& F, M3& R) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = F[k];
+  for (int it = 0; it < 8; ++it) {
+    const float c0 = R[4] * R[8] - R[5] * R[7];
+    const float c1 = R[5] * R[6] - R[3] * R[8];
+    const float c2 = R[3] * R[7] - R[4] * R[6];
+    const float d = R[0] * c0 + R[1] * c1 + R[2] * c2;
+    if (fabsf(d) < 1e-10f) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? 1.f : 0.f;
+      return;
+    }
+    const float id = 1.0f / d;
+    float inv[9];  // inverse transpose = cofactor / det
+    inv[0] = c0 * id;
+    inv[1] = c1 * id;
+    inv[2] = c2 * id;
+    inv[3] = (R[2] * R[7] - R[1] * R[8]) * id;
+    inv[4] = (R[0] * R[8] - R[2] * R[6]) * id;
+    inv[5] = (R[1] * R[6] - R[0] * R[7]) * id;
+    inv[6] = (R[1] * R[5] - R[2] * R[4]) * id;
+    inv[7] = (R[2] * R[3] - R[0] * R[5]) * id;
+    inv[8] = (R[0] * R[4] - R[1] * R[3]) * id;
+    float diff = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const float v = 0.5f * (R[k] + inv[k]);
+      diff += fabsf(v - R[k]);
+      R[k] = v;
+    }
+    if (diff < 3e-7f) break;
+  }
+}
+
+// Fixed-corotated Kirchhoff stress 2 mu (F - R) F^T + lambda (J - 1) J I,
+// returned as the symmetric 6-vector (xx, yy, zz, xy, xz, yz).
+__device__ __forceinline__ void fixed_corotated_tau(const M3& F, float mu, float lambda, float t[6]) {
+  M3 R;
+  polar_rotation(F, R);
+  const float J = det3(F);
+  float D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = F[k] - R[k];
+  const float vol = lambda * (J - 1.f) * J;
+  const float m2 = 2.f * mu;
+  // (F-R) F^T is symmetric for the exact polar factor; symmetrise.
+  float S[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) S[a * 3 + b] = D[a * 3] * F[b * 3] + D[a * 3 + 1] * F[b * 3 + 1] + D[a * 3 + 2] * F[b * 3 + 2];
+  t[0] = m2 * S[0] + vol;
+  t[1] = m2 * S[4] + vol;
+  t[2] = m2 * S[8] + vol;
+  t[3] = mu * (S[1] + S[3]);
+  t[4] = mu * (S[2] + S[6]);
+  t[5] = mu * (S[5] + S[7]);
+}
+
+// Cyclic Jacobi eigen-decomposition of the symmetric A = F^T F. On return
+// V holds the eigenvectors in its columns and lam the eigenvalues.
+__device__ __forceinline__ void jacobi_eig3(float a00, float a11, float a22, float a01, float a02, float a12,
+                                            float V[9], float lam[3]) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) V[k] = (k % 4 == 0) ? 1.f : 0.f;
+  for (int sweep = 0; sweep < 6; ++sweep) {
+    const float off = fabsf(a01) + fabsf(a02) + fabsf(a12);
+    const float scale = fabsf(a00) + fabsf(a11) + fabsf(a22);
+    if (!(off > 1e-8f * scale)) break;
+    // Each rotation zeroes one off-diagonal entry (p,q) with c, s from
+    // t = sign(theta)/(|theta| + sqrt(theta^2+1)), theta = (aqq - app)/(2 apq).
+#define MPM_JACOBI_ROT(app, aqq, apq, akp, akq, P, Q)                               \
+  if (apq != 0.f) {                                                                 \
+    const float theta = (aqq - app) / (2.f * apq);                                  \
+    const float tt = copysignf(1.f, theta) / (fabsf(theta) + sqrtf(theta * theta + 1.f)); \
+    const float c = 1.f / sqrtf(tt * tt + 1.f), s = tt * c;                         \
+    app = app - tt * apq;                                                           \
+    aqq = aqq + tt * apq;                                                           \
+    apq = 0.f;                                                                      \
+    const float x = akp, y = akq;                                                   \
+    akp = c * x - s * y;                                                            \
+    akq = s * x + c * y;                                                            \
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) {                                 \
+      const float vx = V[k * 3 + P], vy = V[k * 3 + Q];                             \
+      V[k * 3 + P] = c * vx - s * vy;                                               \
+      V[k * 3 + Q] = s * vx + c * vy;                                               \
+    }                                                                               \
+  }
+    MPM_JACOBI_ROT(a00, a11, a01, a02, a12, 0, 1)
+    MPM_JACOBI_ROT(a00, a22, a02, a01, a12, 0, 2)
+    MPM_JACOBI_ROT(a11, a22, a12, a01, a02, 1, 2)
+#undef MPM_JACOBI_ROT
+  }
+  lam[0] = a00;
+  lam[1] = a11;
+  lam[2] = a22;
+}
+
+// Principal stretches of F: F = U diag(sigma) V^T with V from the Jacobi
+// eigenvectors of F^T F, sigma = sqrt(max(1e-16, lambda)), U = F V / sigma.
+__device__ __forceinline__ void svd3(const M3& F, float U[9], float V[9], float sigma[3]) {
+  float a[6];
+  a[0] = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+  a[1] = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+  a[2] = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+  a[3] = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+  a[4] = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+  a[5] = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+  float lam[3];
+  jacobi_eig3(a[0], a[1], a[2], a[3], a[4], a[5], V, lam);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) sigma[k] = sqrtf(fmaxf(1e-16f, lam[k]));
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      U[i * 3 + j] = (F[i * 3] * V[j] + F[i * 3 + 1] * V[3 + j] + F[i * 3 + 2] * V[6 + j]) / sigma[j];
+}
+
+// Symmetric U diag(d) U^T as a 6-vector (xx, yy, zz, xy, xz, yz).
+__device__ __forceinline__ void sym_from_principal(const float U[9], const float d[3], float t[6]) {
+  t[0] = U[0] * U[0] * d[0] + U[1] * U[1] * d[1] + U[2] * U[2] * d[2];
+  t[1] = U[3] * U[3] * d[0] + U[4] * U[4] * d[1] + U[5] * U[5] * d[2];
+  t[2] = U[6] * U[6] * d[0] + U[7] * U[7] * d[1] + U[8] * U[8] * d[2];
+  t[3] = U[0] * U[3] * d[0] + U[1] * U[4] * d[1] + U[2] * U[5] * d[2];
+  t[4] = U[0] * U[6] * d[0] + U[1] * U[7] * d[1] + U[2] * U[8] * d[2];
+  t[5] = U[3] * U[6] * d[0] + U[4] * U[7] * d[1] + U[5] * U[8] * d[2];
+}
+
+// F = U diag(s) V^T.
+__device__ __forceinline__ void compose(const float U[9], const float s[3], const float V[9], M3& F) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      F[i * 3 + j] = U[i * 3] * s[0] * V[j * 3] + U[i * 3 + 1] * s[1] * V[j * 3 + 1] + U[i * 3 + 2] * s[2] * V[j * 3 + 2];
+}
+
+__device__ __forceinline__ float thermal_strength(float ep, float T, const DevMaterial& m) {
+  const float theta = fminf(1.f, fmaxf(0.f, (T - m.th_Tref) / (m.th_Thot - m.th_Tref)));
+  return (m.th_A + m.th_H * ep) * (1.f - (1.f - m.th_hotRatio) * theta);
+}
+
+__device__ __forceinline__ float creep_log_rate(float q, float T, const DevMaterial& m) {
+  if (m.th_rate == 0.f || q <= 0.f) return -INFINITY;
+  const float act = fmaxf(-60.f, fminf(60.f, m.th_Q / 8.314462618f *
+                                                (1.f / (m.th_Tref + m.th_kelvin) - 1.f / fmaxf(1.f, T + m.th_kelvin))));
+  return logf(m.th_rate) + act + m.th_n * logf(q / m.th_stress);
+}
+
+__device__ __forceinline__ float flow_stress(int model, float ep, float rate, float T, const DevMaterial& m) {
+  if (model == 4) return m.pl_A;
+  if (model == 5) return m.pl_A + m.pl_B * ep;
+  const float rateFactor = 1.f + m.pl_C * logf(fmaxf(1.f, rate / m.pl_rate0));
+  const float theta = fminf(1.f, fmaxf(0.f, (T - m.pl_Tref) / (m.pl_Tm - m.pl_Tref)));
+  const float hard = m.pl_A + m.pl_B * (ep > 0.f ? powf(ep, m.pl_n) : 0.f);
+  return hard * rateFactor * (model == 6 ? 1.f - powf(theta, m.pl_m) : 1.f);
+}
+
+// Radial return on logarithmic principal strains e (in place) for models 4-7,
+// 10, 11. Returns the plastic strain increment. G = mu/E (E-normalised units).
+__device__ __forceinline__ float return_map(float e[3], int model, float ep, float dt, float T,
+                                            const DevMaterial& m) {
+  const float tr = e[0] + e[1] + e[2];
+  const float mean = tr * (1.f / 3.f);
+  float dev[3] = {e[0] - mean, e[1] - mean, e[2] - mean};
+  const float length = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const float J = fmaxf(1e-6f, expf(tr));
+  const float q = 2.4494897427831781f * m.G * length / J;
+  const float slope = 3.f * m.G / J;
+  float inc = 0.f;
+  if (model == 10) {
+    const float factor = thermal_strength(0.f, T, m) / m.th_A;
+    inc = fmaxf(0.f, (q - thermal_strength(ep, T, m)) / (slope + m.th_H * factor));
+  } else if (model == 11) {
+    if (dt > 0.f && m.th_rate > 0.f && q > 1e-20f) {
+      float lo = 0.f, hi = q / slope;
+      for (int k = 0; k < 32; ++k) {
+        const float mid = 0.5f * (lo + hi);
+        if (logf(mid / dt) < creep_log_rate(fmaxf(0.f, q - slope * mid), T, m))
+          lo = mid;
+        else
+          hi = mid;
+      }
+      inc = 0.5f * (lo + hi);
+    }
+  } else {
+    const float y0 = flow_stress(model, ep, 0.f, T, m);
+    if (q > y0) {
+      if (model <= 5) {
+        inc = (q - y0) / (slope + (model == 5 ? m.pl_B : 0.f));
+      } else {
+        float lo = 0.f, hi = q / slope;
+        const float idt = 1.f / fmaxf(dt, 1e-12f);
+        for (int it = 0; it < 24; ++it) {
+          const float mid = 0.5f * (lo + hi);
+          if (q - slope * mid > flow_stress(model, ep + mid, mid * idt, T, m))
+            lo = mid;
+          else
+            hi = mid;
+        }
+        inc = 0.5f * (lo + hi);
+      }
+    }
+  }
+  const float scale = fmaxf(0.f, 1.f - 1.2247448713915890f * inc / fmaxf(length, 1e-20f));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[k] = mean + scale * dev[k];
+  return inc;
+}
+
+// Plastic projection of F (models 4-7, 10, 11): SVD, return on log stretches,
+// recompose F, and the Hencky stress on the returned principal directions,
+// 2 mu e + lambda tr(e), as the next step's stress (6-vector).
+__device__ __forceinline__ float plastic_project(M3& F, float ep, float dt, float T, const DevMaterial& m,
+                                                 float tau[6]) {
+  float U[9], V[9], s[3];
+  svd3(F, U, V, s);
+  float e[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[k] = logf(fmaxf(s[k], 1e-6f));
+  const float inc = return_map(e, m.model, ep, dt, T, m);
+  float sp[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) sp[k] = expf(e[k]);
+  compose(U, sp, V, F);
+  const float tr = e[0] + e[1] + e[2];
+  float d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = 2.f * m.mu * e[k] + m.lambda * tr;
+  sym_from_principal(U, d, tau);
+  return inc;
+}
+
+// Hencky stress 2 mu log(sigma) + lambda tr from F's own decomposition (the
+// first-step stress of a plastic model before any return has been cached).
+__device__ __forceinline__ void hencky_tau(const M3& F, const DevMaterial& m, float tau[6]) {
+  float U[9], V[9], s[3];
+  svd3(F, U, V, s);
+  float e[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[k] = logf(fmaxf(s[k], 1e-6f));
+  const float tr = e[0] + e[1] + e[2];
+  float d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = 2.f * m.mu * e[k] + m.lambda * tr;
+  sym_from_principal(U, d, tau);
+}
+
+}  // namespace mpm

@@ -1,0 +1,742 @@
+// C ABI of the MLS-MPM engine (include/mpm_engine.h): engine object, device
+// buffers, upload/download, the step driver and the measurement hooks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mpm_engine.h"
+#include "mpm_internal.h"
+
+using namespace mpm;
+
+struct PendingTiming {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+struct mpm_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mpm_config cfg{};
+  GridDims g{};
+  int lo_idx[3], hi_idx[3];
+  DevState d{};
+  uint32_t n = 0;
+  std::vector<mpm_material> materials;
+  double mass_scale = 1.0;
+  int64_t particle_mass_q = 0;
+  int steps_since_sort = 0;
+  std::string err;
+  double* red_host = nullptr;  // pinned
+  // profiling
+  bool profiling = false;
+  std::vector<PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<int64_t, double>> times;
+  std::vector<std::string> order;
+};
+
+namespace {
+
+int fail(mpm_engine* e, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  return code;
+}
+
+#define HIP_OK(e, call)                                                                         \
+  do {                                                                                          \
+    hipError_t _st = (call);                                                                    \
+    if (_st != hipSuccess) return fail((e), MPM_ERR_HIP, "%s: %s", #call, hipGetErrorString(_st)); \
+  } while (0)
+
+hipEvent_t take_event(mpm_engine* e) {
+  if (!e->event_pool.empty()) {
+    hipEvent_t ev = e->event_pool.back();
+    e->event_pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev;
+  hipEventCreate(&ev);
+  return ev;
+}
+
+// Brackets one launch with events when profiling is on.
+struct Timed {
+  mpm_engine* e;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(mpm_engine* e_, const char* n) : e(e_), name(n) {
+    if (e->profiling) {
+      a = take_event(e);
+      b = take_event(e);
+      hipEventRecord(a, e->stream);
+    }
+  }
+  ~Timed() {
+    if (e->profiling) {
+      hipEventRecord(b, e->stream);
+      e->pending.push_back({name, a, b});
+    }
+  }
+};
+
+void collect_times(mpm_engine* e) {
+  for (auto& p : e->pending) {
+    hipEventSynchronize(p.stop);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.start, p.stop);
+    auto it = e->times.find(p.name);
+    if (it == e->times.end()) {
+      e->times[p.name] = {1, ms};
+      e->order.push_back(p.name);
+    } else {
+      it->second.first += 1;
+      it->second.second += ms;
+    }
+    e->event_pool.push_back(p.start);
+    e->event_pool.push_back(p.stop);
+  }
+  e->pending.clear();
+}
+
+uint32_t key_bits(int N) {
+  const uint64_t cells = (uint64_t)N * N * N;
+  uint32_t b = 1;
+  while (b < 32 && (1ull << b) < cells) ++b;
+  return b;
+}
+
+}  // namespace
+
+namespace mpm {
+DevMaterial to_dev_material(const mpm_material& m) {
+  DevMaterial d{};
+  d.model = m.model;
+  d.E = m.E;
+  d.mu = m.mu;
+  d.lambda = m.lambda;
+  d.G = m.E > 0 ? m.mu / m.E : 0.f;
+  d.lr = m.mu != 0 ? m.lambda / m.mu : 0.f;
+  d.th_Tref = m.th_Tref;
+  d.th_Thot = m.th_Thot;
+  d.th_hotRatio = m.th_hotRatio;
+  d.th_A = m.th_A;
+  d.th_H = m.th_H;
+  d.th_rate = m.th_rate;
+  d.th_Q = m.th_Q;
+  d.th_n = m.th_n;
+  d.th_stress = m.th_stress;
+  d.th_kelvin = m.th_kelvin_offset;
+  d.pl_A = m.pl_A;
+  d.pl_B = m.pl_B;
+  d.pl_n = m.pl_n;
+  d.pl_C = m.pl_C;
+  d.pl_rate0 = m.pl_rate0;
+  d.pl_Tref = m.pl_Tref;
+  d.pl_Tm = m.pl_Tm;
+  d.pl_m = m.pl_m;
+  return d;
+}
+}  // namespace mpm
+
+namespace {
+
+bool supported_model(int m) { return m == 0 || (m >= 4 && m <= 7) || m == 10 || m == 11; }
+
+template <typename T>
+inline double rd(const void* p, size_t i) {
+  return (double)static_cast<const T*>(p)[i];
+}
+inline double rdv(const void* p, int dtype, size_t i) {
+  return dtype == MPM_F64 ? rd<double>(p, i) : rd<float>(p, i);
+}
+inline void wrv(void* p, int dtype, size_t i, double v) {
+  if (dtype == MPM_F64)
+    static_cast<double*>(p)[i] = v;
+  else
+    static_cast<float*>(p)[i] = (float)v;
+}
+
+// Bin, sort and permute the resident particles into cell order.
+int rebin(mpm_engine* e) {
+  if (e->n == 0) return MPM_OK;
+  {
+    Timed t(e, "bin_keys");
+    launch_bin_keys(e->stream, e->d, e->n, e->g);
+  }
+  {
+    Timed t(e, "radix_sort");
+    radix_sort_pairs(e->stream, e->d.keys, e->d.vals, e->d.keys_alt, e->d.vals_alt, e->d.hist, e->d.partials, e->n,
+                     key_bits(e->g.N));
+  }
+  {
+    Timed t(e, "permute");
+    launch_permute(e->stream, e->d, e->n, e->d.vals);
+  }
+  HIP_OK(e, hipGetLastError());
+  e->steps_since_sort = 0;
+  return MPM_OK;
+}
+
+StepParams params(const mpm_engine* e, double dt) {
+  StepParams p;
+  p.dt = (float)dt;
+  p.mass_scale = (float)e->mass_scale;
+  p.inv_mass_scale = (float)(1.0 / e->mass_scale);
+  return p;
+}
+
+// Runs a device kernel over host arrays of n items: copies the inputs in,
+// launches, copies the outputs back, frees everything.
+template <typename Launch>
+int batch_call(uint32_t n, std::vector<std::pair<const float*, size_t>> ins,
+                      std::vector<std::pair<float*, size_t>> outs, Launch launch) {
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return MPM_ERR_HIP;
+  std::vector<float*> din(ins.size(), nullptr), dout(outs.size(), nullptr);
+  hipError_t st = hipSuccess;
+  for (size_t k = 0; k < ins.size() && st == hipSuccess; ++k) {
+    st = hipMalloc((void**)&din[k], sizeof(float) * std::max<size_t>(ins[k].second * n, 1));
+    if (st == hipSuccess && n)
+      st = hipMemcpyAsync(din[k], ins[k].first, sizeof(float) * ins[k].second * n, hipMemcpyHostToDevice, s);
+  }
+  for (size_t k = 0; k < outs.size() && st == hipSuccess; ++k)
+    st = hipMalloc((void**)&dout[k], sizeof(float) * std::max<size_t>(outs[k].second * n, 1));
+  if (st == hipSuccess) {
+    launch(s, din, dout);
+    st = hipGetLastError();
+  }
+  for (size_t k = 0; k < outs.size() && st == hipSuccess; ++k)
+    if (n) st = hipMemcpyAsync(outs[k].first, dout[k], sizeof(float) * outs[k].second * n, hipMemcpyDeviceToHost, s);
+  if (st == hipSuccess) st = hipStreamSynchronize(s);
+  for (float* p : din)
+    if (p) (void)hipFree(p);
+  for (float* p : dout)
+    if (p) (void)hipFree(p);
+  (void)hipStreamDestroy(s);
+  return st == hipSuccess ? MPM_OK : MPM_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpm_last_error(const mpm_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int mpm_create(const mpm_config* c, mpm_engine** out) {
+  if (!c || !out) return MPM_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  const double h = c->cell_size;
+  if (!(h > 0) || !std::isfinite(h)) return MPM_ERR_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a) {
+    if (!std::isfinite(c->lower[a]) || !std::isfinite(c->upper[a]) || c->upper[a] - c->lower[a] < h)
+      return MPM_ERR_INVALID_ARGUMENT;
+  }
+  for (int f = 0; f < 6; ++f)
+    if (c->wall_mode[f] < 0 || c->wall_mode[f] > 2) return MPM_ERR_INVALID_ARGUMENT;
+  if (c->strict_domain) {  // src/three/discretization.js:8-10
+    if (h < 4 || h > 16) return MPM_ERR_INVALID_ARGUMENT;
+    for (int a = 0; a < 3; ++a) {
+      const double cells = std::ceil(c->upper[a] / h - 1e-10) - std::floor(c->lower[a] / h + 1e-10);
+      if (c->lower[a] < -2400 || c->upper[a] > 2400 || cells > 60) return MPM_ERR_INVALID_ARGUMENT;
+    }
+  }
+  auto* e = new mpm_engine();
+  e->cfg = *c;
+  if (e->cfg.sort_every < 1) e->cfg.sort_every = 1;
+  // Geometry of src/three/discretization.js:13 with a cubic node block of
+  // edge max(nodes) (src/three/mpm.js:18).
+  int N = 0;
+  double origin[3];
+  for (int a = 0; a < 3; ++a) {
+    origin[a] = std::floor(c->lower[a] / h) * h - 2 * h;
+    N = std::max(N, (int)(std::ceil((c->upper[a] - origin[a]) / h) + 3));
+  }
+  const uint64_t Nn = (uint64_t)N * N * N;
+  if (Nn > (1ull << 31)) {
+    delete e;
+    return MPM_ERR_CAPACITY;
+  }
+  GridDims& g = e->g;
+  g.N = N;
+  for (int a = 0; a < 3; ++a) {
+    g.O[a] = (float)origin[a];
+    g.lower[a] = (float)c->lower[a];
+    g.upper[a] = (float)c->upper[a];
+    g.gravity[a] = (float)c->gravity[a];
+    // Wall bands as node-index thresholds, evaluated in FP64 exactly as the
+    // reference compares node positions O + i h with lower + h / upper - h.
+    int lo = -1, hi = N;
+    for (int i = 0; i < N; ++i) {
+      const double q = origin[a] + (double)i * h;
+      if (q < c->lower[a] + h) lo = i;
+      if (q > c->upper[a] - h && hi == N) hi = i;
+    }
+    e->lo_idx[a] = lo;
+    e->hi_idx[a] = hi;
+  }
+  for (int f = 0; f < 6; ++f) g.wall[f] = c->wall_mode[f];
+  g.H = (float)h;
+  g.invH = (float)(1.0 / h);
+  g.gravity_on_particles = c->gravity_on_particles ? 1 : 0;
+  g.mass_eps = (float)c->mass_eps;
+
+  hipError_t st = hipGetDevice(&e->device);
+  if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  const uint32_t cap = std::max<uint32_t>(c->capacity, 1);
+  DevState& d = e->d;
+  d.cap = cap;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (st == hipSuccess) st = hipMalloc(p, std::max<size_t>(bytes, 256));
+  };
+  alloc((void**)&d.planes, sizeof(float) * NPLANES * (size_t)cap);
+  alloc((void**)&d.planes_alt, sizeof(float) * NPLANES * (size_t)cap);
+  alloc((void**)&d.ids, sizeof(uint32_t) * cap);
+  alloc((void**)&d.ids_alt, sizeof(uint32_t) * cap);
+  alloc((void**)&d.meta, sizeof(uint32_t) * cap);
+  alloc((void**)&d.meta_alt, sizeof(uint32_t) * cap);
+  alloc((void**)&d.acc_m, sizeof(unsigned long long) * Nn);
+  alloc((void**)&d.acc_p, sizeof(float) * 3 * Nn);
+  alloc((void**)&d.vel, sizeof(float4) * Nn);
+  alloc((void**)&d.fixed, Nn);
+  alloc((void**)&d.pieces, (Nn + 63) / 64);
+  alloc((void**)&d.keys, sizeof(uint32_t) * cap);
+  alloc((void**)&d.keys_alt, sizeof(uint32_t) * cap);
+  alloc((void**)&d.vals, sizeof(uint32_t) * cap);
+  alloc((void**)&d.vals_alt, sizeof(uint32_t) * cap);
+  alloc((void**)&d.hist, sizeof(uint32_t) * radix_hist_words(cap));
+  alloc((void**)&d.partials, sizeof(uint32_t) * radix_partial_words(cap));
+  alloc((void**)&d.red, sizeof(double) * 8);
+  alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
+  if (st == hipSuccess) st = hipHostMalloc((void**)&e->red_host, sizeof(double) * 8, hipHostMallocDefault);
+  if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, (Nn + 63) / 64, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 8, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  if (st != hipSuccess) {
+    mpm_destroy(e);
+    return MPM_ERR_HIP;
+  }
+  // Default material: fixed-corotated, E = 1, mu = lambda = 0.4.
+  mpm_material m0{};
+  m0.model = 0;
+  m0.E = 1.f;
+  m0.mu = m0.lambda = .4f;
+  mpm_set_materials(e, &m0, 1);
+  *out = e;
+  return MPM_OK;
+}
+
+int mpm_destroy(mpm_engine* e) {
+  if (!e) return MPM_OK;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  DevState& d = e->d;
+  void* ptrs[] = {d.planes, d.planes_alt, d.ids,  d.ids_alt,  d.meta,     d.meta_alt, d.acc_m,
+                  d.acc_p,  d.vel,        d.fixed, d.pieces,  d.keys,     d.keys_alt, d.vals,
+                  d.vals_alt, d.hist,     d.partials, d.red,  d.materials};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (e->red_host) hipHostFree(e->red_host);
+  for (auto& p : e->pending) {
+    hipEventDestroy(p.start);
+    hipEventDestroy(p.stop);
+  }
+  for (auto ev : e->event_pool) hipEventDestroy(ev);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return MPM_OK;
+}
+
+int mpm_set_materials(mpm_engine* e, const mpm_material* mats, int32_t count) {
+  if (!e || !mats || count < 1 || count > 64) return fail(e, MPM_ERR_INVALID_ARGUMENT, "1..64 materials required");
+  std::vector<DevMaterial> dm(count);
+  for (int i = 0; i < count; ++i) {
+    if (!supported_model(mats[i].model))
+      return fail(e, MPM_ERR_UNSUPPORTED, "material %d: model %d is not supported", i, mats[i].model);
+    if (!(mats[i].E > 0) || !(mats[i].mu >= 0) || !std::isfinite(mats[i].lambda))
+      return fail(e, MPM_ERR_INVALID_ARGUMENT, "material %d: E must be > 0 and mu >= 0", i);
+    dm[i] = to_dev_material(mats[i]);
+  }
+  e->materials.assign(mats, mats + count);
+  HIP_OK(e, hipMemcpyAsync(e->d.materials, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice,
+                           e->stream));
+  HIP_OK(e, hipStreamSynchronize(e->stream));
+  return MPM_OK;
+}
+
+int mpm_upload(mpm_engine* e, const mpm_particles* p) {
+  if (!e || !p) return MPM_ERR_INVALID_ARGUMENT;
+  const uint32_t n = p->count;
+  if (n > e->d.cap) return fail(e, MPM_ERR_CAPACITY, "count %u exceeds capacity %u", n, e->d.cap);
+  if (p->dtype != MPM_F64 && p->dtype != MPM_F32) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dtype");
+  if (n && (!p->x || !p->y || !p->z || !p->vx || !p->vy || !p->vz || !p->F || !p->C || !p->mass || !p->volume ||
+            !p->temperature || !p->plastic || !p->material || !p->pinned))
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "missing particle array");
+  const int dt = p->dtype;
+  const uint32_t cap = e->d.cap;
+  std::vector<float> planes((size_t)NPLANES * cap, 0.f);
+  std::vector<uint32_t> ids(n), meta(n);
+  double total = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (p->material[i] >= e->materials.size())
+      return fail(e, MPM_ERR_INVALID_ARGUMENT, "particle %u: material %u out of range", i, p->material[i]);
+    const double m = rdv(p->mass, dt, i), vol = rdv(p->volume, dt, i);
+    if (!(m > 0) || !(vol > 0)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "particle %u: mass and volume must be > 0", i);
+    total += (double)(float)m;
+    auto set = [&](int k, double v) { planes[(size_t)k * cap + i] = (float)v; };
+    set(P_X, rdv(p->x, dt, i));
+    set(P_Y, rdv(p->y, dt, i));
+    set(P_Z, rdv(p->z, dt, i));
+    set(P_VX, rdv(p->vx, dt, i));
+    set(P_VY, rdv(p->vy, dt, i));
+    set(P_VZ, rdv(p->vz, dt, i));
+    for (int k = 0; k < 9; ++k) {
+      set(P_C0 + k, rdv(p->C, dt, (size_t)i * 9 + k));
+      set(P_F0 + k, rdv(p->F, dt, (size_t)i * 9 + k));
+    }
+    uint32_t flags = p->pinned[i] ? FLAG_PINNED : 0u;
+    if (p->cached_tau && p->cache_valid && p->cache_valid[i]) {
+      const size_t b = (size_t)i * 9;
+      set(P_T0 + 0, rdv(p->cached_tau, dt, b + 0));
+      set(P_T0 + 1, rdv(p->cached_tau, dt, b + 4));
+      set(P_T0 + 2, rdv(p->cached_tau, dt, b + 8));
+      set(P_T0 + 3, 0.5 * (rdv(p->cached_tau, dt, b + 1) + rdv(p->cached_tau, dt, b + 3)));
+      set(P_T0 + 4, 0.5 * (rdv(p->cached_tau, dt, b + 2) + rdv(p->cached_tau, dt, b + 6)));
+      set(P_T0 + 5, 0.5 * (rdv(p->cached_tau, dt, b + 5) + rdv(p->cached_tau, dt, b + 7)));
+      flags |= FLAG_CACHE;
+    }
+    set(P_MASS, m);
+    set(P_VOL, vol);
+    set(P_TEMP, rdv(p->temperature, dt, i));
+    set(P_EP, rdv(p->plastic, dt, i));
+    ids[i] = i;
+    meta[i] = (uint32_t)p->material[i] | (flags << 8);
+  }
+  // Fixed-point mass unit 2^k with the total mass at most 2^61, so every node
+  // sum and the grand total fit a signed 64-bit integer.
+  int k = 0;
+  if (total > 0) k = (int)std::floor(61.0 - std::log2(total));
+  k = std::max(-100, std::min(100, k));
+  e->mass_scale = std::ldexp(1.0, k);
+  int64_t q = 0;
+  const float sf = (float)e->mass_scale;
+  for (uint32_t i = 0; i < n; ++i) q += (int64_t)std::llrint((double)(planes[(size_t)P_MASS * cap + i] * sf));
+  e->particle_mass_q = q;
+  e->n = n;
+  HIP_OK(e, hipMemcpyAsync(e->d.planes, planes.data(), sizeof(float) * planes.size(), hipMemcpyHostToDevice,
+                           e->stream));
+  if (n) {
+    HIP_OK(e, hipMemcpyAsync(e->d.ids, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(e, hipMemcpyAsync(e->d.meta, meta.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, e->stream));
+  }
+  int rc = rebin(e);
+  if (rc) return rc;
+  HIP_OK(e, hipStreamSynchronize(e->stream));
+  return MPM_OK;
+}
+
+int mpm_download(mpm_engine* e, mpm_particles* p) {
+  if (!e || !p) return MPM_ERR_INVALID_ARGUMENT;
+  if (p->count != e->n) return fail(e, MPM_ERR_INVALID_ARGUMENT, "count %u != resident %u", p->count, e->n);
+  const uint32_t n = e->n, cap = e->d.cap;
+  const int dt = p->dtype;
+  if (dt != MPM_F64 && dt != MPM_F32) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dtype");
+  std::vector<float> planes((size_t)NPLANES * cap);
+  std::vector<uint32_t> ids(n), meta(n);
+  HIP_OK(e, hipMemcpyAsync(planes.data(), e->d.planes, sizeof(float) * planes.size(), hipMemcpyDeviceToHost,
+                           e->stream));
+  if (n) {
+    HIP_OK(e, hipMemcpyAsync(ids.data(), e->d.ids, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(e, hipMemcpyAsync(meta.data(), e->d.meta, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream));
+  }
+  HIP_OK(e, hipStreamSynchronize(e->stream));
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t i = ids[s];
+    auto get = [&](int k) { return (double)planes[(size_t)k * cap + s]; };
+    if (p->x) wrv(p->x, dt, i, get(P_X));
+    if (p->y) wrv(p->y, dt, i, get(P_Y));
+    if (p->z) wrv(p->z, dt, i, get(P_Z));
+    if (p->vx) wrv(p->vx, dt, i, get(P_VX));
+    if (p->vy) wrv(p->vy, dt, i, get(P_VY));
+    if (p->vz) wrv(p->vz, dt, i, get(P_VZ));
+    for (int k = 0; k < 9; ++k) {
+      if (p->F) wrv(p->F, dt, (size_t)i * 9 + k, get(P_F0 + k));
+      if (p->C) wrv(p->C, dt, (size_t)i * 9 + k, get(P_C0 + k));
+    }
+    if (p->mass) wrv(p->mass, dt, i, get(P_MASS));
+    if (p->volume) wrv(p->volume, dt, i, get(P_VOL));
+    if (p->temperature) wrv(p->temperature, dt, i, get(P_TEMP));
+    if (p->plastic) wrv(p->plastic, dt, i, get(P_EP));
+    if (p->cached_tau) {
+      const double t[6] = {get(P_T0), get(P_T0 + 1), get(P_T0 + 2), get(P_T0 + 3), get(P_T0 + 4), get(P_T0 + 5)};
+      const double full[9] = {t[0], t[3], t[4], t[3], t[1], t[5], t[4], t[5], t[2]};
+      for (int k = 0; k < 9; ++k) wrv(p->cached_tau, dt, (size_t)i * 9 + k, full[k]);
+    }
+    if (p->material) p->material[i] = (uint8_t)(meta[s] & 255u);
+    if (p->pinned) p->pinned[i] = (meta[s] >> 8) & FLAG_PINNED ? 1 : 0;
+    if (p->cache_valid) p->cache_valid[i] = (meta[s] >> 8) & FLAG_CACHE ? 1 : 0;
+  }
+  return MPM_OK;
+}
+
+int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  if (!(dt > 0) || !std::isfinite(dt) || nsteps < 0) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt or nsteps");
+  const StepParams sp = params(e, dt);
+  for (int s = 0; s < nsteps; ++s) {
+    if (e->steps_since_sort >= e->cfg.sort_every) {
+      int rc = rebin(e);
+      if (rc) return rc;
+    }
+    launch_clear_red(e->stream, e->d);
+    {
+      Timed t(e, "p2g");
+      launch_p2g(e->stream, e->d, e->n, e->g, sp);
+    }
+    {
+      Timed t(e, "grid");
+      launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx);
+    }
+    {
+      Timed t(e, "g2p");
+      launch_g2p(e->stream, e->d, e->n, e->g, sp);
+    }
+    e->steps_since_sort++;
+  }
+  HIP_OK(e, hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_synchronize(mpm_engine* e) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  HIP_OK(e, hipStreamSynchronize(e->stream));
+  return MPM_OK;
+}
+
+int mpm_count(const mpm_engine* e, uint32_t* out) {
+  if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+  *out = e->n;
+  return MPM_OK;
+}
+
+int mpm_stable_dt(mpm_engine* e, double cap, double cfl, double* out) {
+  if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+  const int nb = particle_reduce_blocks();
+  float* dbuf = nullptr;
+  HIP_OK(e, hipMallocAsync((void**)&dbuf, sizeof(float) * 2 * nb, e->stream));
+  launch_stable(e->stream, e->d, e->n, dbuf);
+  std::vector<float> h(2 * nb);
+  hipError_t st = hipMemcpyAsync(h.data(), dbuf, sizeof(float) * 2 * nb, hipMemcpyDeviceToHost, e->stream);
+  hipFreeAsync(dbuf, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "stable_dt: %s", hipGetErrorString(st));
+  double mx = 1, grad = 1;
+  for (int b = 0; b < nb; ++b) {
+    mx = std::max(mx, (double)h[2 * b]);
+    grad = std::max(grad, (double)h[2 * b + 1]);
+  }
+  const double H = e->cfg.cell_size;
+  *out = std::min(std::min(cap > 0 ? cap : H / 8 / 240, (cfl > 0 ? cfl * H : 1.4 * H / 8) / mx), .08 / grad);
+  return MPM_OK;
+}
+
+int mpm_conservation_report(mpm_engine* e, mpm_conservation* out) {
+  if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+  const int K = particle_reduce_k(), nb = particle_reduce_blocks();
+  double* dbuf = nullptr;
+  HIP_OK(e, hipMallocAsync((void**)&dbuf, sizeof(double) * K * nb, e->stream));
+  const float D = (float)(e->cfg.cell_size * e->cfg.cell_size / 4);
+  launch_particle_reduce(e->stream, e->d, e->n, (float)e->mass_scale, D, dbuf);
+  std::vector<double> h((size_t)K * nb);
+  hipError_t st = hipMemcpyAsync(h.data(), dbuf, sizeof(double) * K * nb, hipMemcpyDeviceToHost, e->stream);
+  if (st == hipSuccess) st = hipMemcpyAsync(e->red_host, e->d.red, sizeof(double) * 8, hipMemcpyDeviceToHost, e->stream);
+  hipFreeAsync(dbuf, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "conservation: %s", hipGetErrorString(st));
+  std::memset(out, 0, sizeof *out);
+  double acc[16] = {0};
+  double minJ = 1.0, maxv = 0;
+  for (int b = 0; b < nb; ++b) {
+    for (int k = 0; k < 11; ++k) acc[k] += h[(size_t)b * K + k];
+    minJ = std::min(minJ, h[(size_t)b * K + 11]);
+    maxv = std::max(maxv, h[(size_t)b * K + 12]);
+  }
+  out->particle_mass = acc[0];
+  out->particle_mass_q = e->particle_mass_q;
+  int64_t gq;
+  std::memcpy(&gq, &e->red_host[0], sizeof gq);
+  out->grid_mass_q = gq;
+  out->mass_scale = e->mass_scale;
+  out->grid_mass = (double)gq / e->mass_scale;
+  for (int k = 0; k < 3; ++k) {
+    out->momentum[k] = acc[2 + k];
+    out->angular[k] = acc[5 + k];
+    out->grid_momentum[k] = e->red_host[1 + k];
+  }
+  out->kinetic = acc[8];
+  out->affine = acc[9];
+  out->nonfinite = (int64_t)acc[10];
+  out->min_J = e->n ? minJ : 1.0;
+  out->max_speed = maxv;
+  return MPM_OK;
+}
+
+int mpm_refresh_diagnostics(mpm_engine* e, double* stress) {
+  if (!e || !stress) return MPM_ERR_INVALID_ARGUMENT;
+  const uint32_t n = e->n;
+  if (!n) return MPM_OK;
+  float* dbuf = nullptr;
+  HIP_OK(e, hipMallocAsync((void**)&dbuf, sizeof(float) * n, e->stream));
+  launch_von_mises(e->stream, e->d, n, dbuf);
+  std::vector<float> h(n);
+  std::vector<uint32_t> ids(n);
+  hipError_t st = hipMemcpyAsync(h.data(), dbuf, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream);
+  if (st == hipSuccess) st = hipMemcpyAsync(ids.data(), e->d.ids, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream);
+  hipFreeAsync(dbuf, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "diagnostics: %s", hipGetErrorString(st));
+  for (uint32_t s = 0; s < n; ++s) stress[ids[s]] = h[s];
+  return MPM_OK;
+}
+
+int mpm_profile(mpm_engine* e, int32_t enable, int32_t reset) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  collect_times(e);
+  if (reset) {
+    e->times.clear();
+    e->order.clear();
+  }
+  e->profiling = enable != 0;
+  return MPM_OK;
+}
+
+int mpm_kernel_times(mpm_engine* e, mpm_kernel_time* out, int32_t max, int32_t* n) {
+  if (!e || !n) return MPM_ERR_INVALID_ARGUMENT;
+  collect_times(e);
+  int k = 0;
+  for (const auto& name : e->order) {
+    if (out && k < max) {
+      std::memset(&out[k], 0, sizeof out[k]);
+      std::snprintf(out[k].name, sizeof out[k].name, "%s", name.c_str());
+      out[k].launches = e->times[name].first;
+      out[k].total_ms = e->times[name].second;
+    }
+    ++k;
+  }
+  *n = k;
+  return MPM_OK;
+}
+
+int mpm_debug_bin(mpm_engine* e, uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  const uint32_t n = e->n;
+  if (!n) return MPM_OK;
+  launch_bin_keys(e->stream, e->d, n, e->g);
+  if (keys_in) HIP_OK(e, hipMemcpyAsync(keys_in, e->d.keys, 4ull * n, hipMemcpyDeviceToHost, e->stream));
+  std::vector<uint32_t> ids(n);
+  HIP_OK(e, hipMemcpyAsync(ids.data(), e->d.ids, 4ull * n, hipMemcpyDeviceToHost, e->stream));
+  radix_sort_pairs(e->stream, e->d.keys, e->d.vals, e->d.keys_alt, e->d.vals_alt, e->d.hist, e->d.partials, n,
+                   key_bits(e->g.N));
+  std::vector<uint32_t> order(n);
+  if (keys_out) HIP_OK(e, hipMemcpyAsync(keys_out, e->d.keys, 4ull * n, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(e, hipMemcpyAsync(order.data(), e->d.vals, 4ull * n, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(e, hipGetLastError());
+  HIP_OK(e, hipStreamSynchronize(e->stream));
+  if (perm)
+    for (uint32_t s = 0; s < n; ++s) perm[s] = ids[order[s]];
+  return MPM_OK;
+}
+
+int mpm_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t bits, uint32_t* keys_out, uint32_t* perm_out) {
+  if (n && (!keys || !keys_out || !perm_out)) return MPM_ERR_INVALID_ARGUMENT;
+  if (bits > 32) return MPM_ERR_INVALID_ARGUMENT;
+  if (n == 0) return MPM_OK;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return MPM_ERR_HIP;
+  uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *hist = nullptr, *part = nullptr;
+  hipError_t st = hipMalloc((void**)&k0, 4ull * n);
+  if (st == hipSuccess) st = hipMalloc((void**)&k1, 4ull * n);
+  if (st == hipSuccess) st = hipMalloc((void**)&v0, 4ull * n);
+  if (st == hipSuccess) st = hipMalloc((void**)&v1, 4ull * n);
+  if (st == hipSuccess) st = hipMalloc((void**)&hist, 4 * radix_hist_words(n));
+  if (st == hipSuccess) st = hipMalloc((void**)&part, 4 * radix_partial_words(n));
+  std::vector<uint32_t> iota(n);
+  for (uint32_t i = 0; i < n; ++i) iota[i] = i;
+  if (st == hipSuccess) st = hipMemcpyAsync(k0, keys, 4ull * n, hipMemcpyHostToDevice, s);
+  if (st == hipSuccess) st = hipMemcpyAsync(v0, iota.data(), 4ull * n, hipMemcpyHostToDevice, s);
+  if (st == hipSuccess) {
+    radix_sort_pairs(s, k0, v0, k1, v1, hist, part, n, bits);
+    st = hipGetLastError();
+  }
+  if (st == hipSuccess) st = hipMemcpyAsync(keys_out, k0, 4ull * n, hipMemcpyDeviceToHost, s);
+  if (st == hipSuccess) st = hipMemcpyAsync(perm_out, v0, 4ull * n, hipMemcpyDeviceToHost, s);
+  if (st == hipSuccess) st = hipStreamSynchronize(s);
+  for (void* p : {(void*)k0, (void*)k1, (void*)v0, (void*)v1, (void*)hist, (void*)part})
+    if (p) hipFree(p);
+  hipStreamDestroy(s);
+  return st == hipSuccess ? MPM_OK : MPM_ERR_HIP;
+}
+
+int mpm_return_map_batch(const mpm_material* m, uint32_t n, float dt, const float* F, const float* ep,
+                         const float* T, float* F_out, float* ep_out, float* tau_out) {
+  if (!m || (n && (!F || !ep || !T || !F_out || !ep_out || !tau_out))) return MPM_ERR_INVALID_ARGUMENT;
+  if (!supported_model(m->model)) return MPM_ERR_UNSUPPORTED;
+  const DevMaterial dm = to_dev_material(*m);
+  return batch_call(n, {{F, 9}, {ep, 1}, {T, 1}}, {{F_out, 9}, {ep_out, 1}, {tau_out, 6}},
+                    [&](hipStream_t s, std::vector<float*>& in, std::vector<float*>& out) {
+                      launch_return_batch(s, dm, n, dt, in[0], in[1], in[2], out[0], out[1], out[2]);
+                    });
+}
+
+int mpm_stress_batch(const mpm_material* m, uint32_t n, const float* F, float* tau_out) {
+  if (!m || (n && (!F || !tau_out))) return MPM_ERR_INVALID_ARGUMENT;
+  if (!supported_model(m->model)) return MPM_ERR_UNSUPPORTED;
+  const DevMaterial dm = to_dev_material(*m);
+  return batch_call(n, {{F, 9}}, {{tau_out, 6}}, [&](hipStream_t s, std::vector<float*>& in, std::vector<float*>& out) {
+    launch_stress_batch(s, dm, n, in[0], out[0]);
+  });
+}
+
+int mpm_debug_p2g(mpm_engine* e, double dt, double* grid_mass, double* grid_momentum, int32_t* Nout) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  const int N = e->g.N;
+  if (Nout) *Nout = N;
+  if (!grid_mass && !grid_momentum) return MPM_OK;
+  const size_t Nn = (size_t)N * N * N;
+  const StepParams sp = params(e, dt);
+  launch_p2g(e->stream, e->d, e->n, e->g, sp);
+  std::vector<unsigned long long> m(Nn);
+  std::vector<float> p(3 * Nn);
+  HIP_OK(e, hipMemcpyAsync(m.data(), e->d.acc_m, 8 * Nn, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(e, hipMemcpyAsync(p.data(), e->d.acc_p, 12 * Nn, hipMemcpyDeviceToHost, e->stream));
+  // Leave the accumulators clear for the next step.
+  HIP_OK(e, hipMemsetAsync(e->d.acc_m, 0, 8 * Nn, e->stream));
+  HIP_OK(e, hipMemsetAsync(e->d.acc_p, 0, 12 * Nn, e->stream));
+  HIP_OK(e, hipMemsetAsync(e->d.fixed, 0, Nn, e->stream));
+  HIP_OK(e, hipMemsetAsync(e->d.pieces, 0, (Nn + 63) / 64, e->stream));
+  HIP_OK(e, hipGetLastError());
+  HIP_OK(e, hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < Nn; ++i) {
+    if (grid_mass) grid_mass[i] = (double)(int64_t)m[i] / e->mass_scale;
+    if (grid_momentum)
+      for (int a = 0; a < 3; ++a) grid_momentum[i * 3 + a] = p[a * Nn + i];
+  }
+  return MPM_OK;
+}
+
+}  // extern "C"

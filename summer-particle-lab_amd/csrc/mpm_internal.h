@@ -1,0 +1,99 @@
+// Internal declarations shared by the engine's HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "mpm_device.h"
+#include "mpm_engine.h"
+
+namespace mpm {
+
+// Particle state planes (FP32, one plane of `cap` floats each, slot order).
+enum Plane : int {
+  P_X = 0, P_Y, P_Z,
+  P_VX, P_VY, P_VZ,
+  P_C0,                 // C00..C22: P_C0 + 0..8
+  P_F0 = P_C0 + 9,      // F00..F22: P_F0 + 0..8
+  P_T0 = P_F0 + 9,      // cached stress xx, yy, zz, xy, xz, yz: P_T0 + 0..5
+  P_MASS = P_T0 + 6,
+  P_VOL,
+  P_TEMP,
+  P_EP,
+  NPLANES
+};
+
+// Integer planes: original particle id, and material | flags << 8.
+enum : uint32_t { FLAG_PINNED = 1u, FLAG_CACHE = 2u };
+
+struct GridDims {
+  int N;              // nodes per axis (cubic grid, N^3 nodes)
+  float O[3];         // origin
+  float H, invH;
+  float lower[3], upper[3];
+  int wall[6];
+  float gravity[3];
+  int gravity_on_particles;
+  float mass_eps;
+};
+
+struct StepParams {
+  float dt;
+  float mass_scale;      // fixed-point units per unit mass (2^k)
+  float inv_mass_scale;
+};
+
+// Device pointers of one engine.
+struct DevState {
+  float* planes;      // NPLANES * cap
+  float* planes_alt;  // permutation target
+  uint32_t* ids;      // original id per slot
+  uint32_t* ids_alt;
+  uint32_t* meta;     // material | flags << 8
+  uint32_t* meta_alt;
+  uint32_t cap;
+  // grid
+  unsigned long long* acc_m;  // fixed-point node mass [N^3]
+  float* acc_p;               // node momentum 3 planes [3*N^3]
+  float4* vel;                // node velocity (xyz) and mass (w) [N^3]
+  uint8_t* fixed;             // [N^3]
+  uint8_t* pieces;            // active 64-node pieces [ceil(N^3/64)]
+  // sort scratch
+  uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *partials;
+  // reductions: [0] grid mass (int64), [1..3] grid momentum (double)
+  double* red;
+  DevMaterial* materials;
+};
+
+// Kernel launchers (mpm_kernels.hip)
+void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
+void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
+void launch_p2g(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p);
+void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
+                 const int hi_idx[3]);
+void launch_g2p(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p);
+void launch_clear_red(hipStream_t s, const DevState& d);
+void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
+// Particle-side FP64 sums: particle_reduce_blocks() records of
+// particle_reduce_k() doubles each (see k_particle_reduce).
+int particle_reduce_k();
+int particle_reduce_blocks();
+void launch_particle_reduce(hipStream_t s, const DevState& d, uint32_t n, float mass_scale, float D, double* out);
+// Per block {max wave speed, max sum |C|}; particle_reduce_blocks() records.
+void launch_stable(hipStream_t s, const DevState& d, uint32_t n, float* out);
+void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out);
+void launch_return_batch(hipStream_t s, const DevMaterial& m, uint32_t n, float dt, const float* F, const float* ep,
+                         const float* T, float* F_out, float* ep_out, float* tau_out);
+void launch_stress_batch(hipStream_t s, const DevMaterial& m, uint32_t n, const float* F, float* tau_out);
+DevMaterial to_dev_material(const mpm_material& m);
+
+// Radix sort (mpm_sort.hip): stable LSD sort of (keys, vals) pairs. On return
+// the sorted pairs are in (keys, vals) (the buffers are swapped as needed).
+size_t radix_hist_words(uint32_t n);
+size_t radix_partial_words(uint32_t n);
+void radix_sort_pairs(hipStream_t s, uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt,
+                      uint32_t* hist, uint32_t* partials, uint32_t n, uint32_t key_bits);
+
+}  // namespace mpm

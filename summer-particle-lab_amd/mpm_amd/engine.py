@@ -1,0 +1,248 @@
+"""Python host wrapper over the engine's C ABI.
+
+``Engine`` owns one device engine (one HIP stream). It mirrors the method
+surface of the reference ``MPM3`` class (src/three/mpm.js:7-136):
+``configure`` -> constructor config, ``step``, ``stable_step``,
+``refresh_diagnostics``, ``min_j``; state moves through ``upload`` /
+``download`` of structure-of-arrays particle data.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lib as L
+
+
+@dataclass
+class Material:
+    """One constitutive record; strengths normalised by E (reference convention,
+    src/three/thermal-plasticity.js:1-4, src/three/metal-plasticity.js:1-4)."""
+    model: int = 0
+    E: float = 1.0
+    mu: float = 0.4
+    lam: float = 0.4
+    thermal: dict = field(default_factory=dict)
+    plastic: dict = field(default_factory=dict)
+
+    @staticmethod
+    def from_E_nu(model, E, nu, **kw):
+        mu = E / (2 * (1 + nu))
+        lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+        return Material(model=model, E=E, mu=mu, lam=lam, **kw)
+
+    def to_c(self):
+        th = dict(Tref=20.0, Thot=180.0, hotRatio=.08, A=.04, H=.04, rate=.0005, Q=35000.0, n=3.0, stress=.04,
+                  kelvin_offset=273.15)
+        th.update(self.thermal)
+        pl = dict(A=.025, B=.08, n=.35, C=.025, rate0=1.0, Tref=20.0, Tm=1000.0, m=1.0)
+        pl.update(self.plastic)
+        m = L.MpmMaterial()
+        m.model = self.model
+        m.E, m.mu, m.lambda_ = self.E, self.mu, self.lam
+        for k, v in th.items():
+            setattr(m, "th_" + k, v)
+        for k, v in pl.items():
+            setattr(m, "pl_" + k, v)
+        return m
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Engine:
+    """One MLS-MPM engine on the current HIP device."""
+
+    def __init__(self, lower, upper, cell_size, capacity, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
+                 gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False):
+        self._lib = L.load()
+        cfg = L.MpmConfig()
+        cfg.lower[:] = list(map(float, lower))
+        cfg.upper[:] = list(map(float, upper))
+        cfg.cell_size = float(cell_size)
+        cfg.capacity = int(capacity)
+        cfg.wall_mode[:] = list(map(int, wall_mode))
+        cfg.gravity[:] = list(map(float, gravity))
+        cfg.gravity_on_particles = int(bool(gravity_on_particles))
+        cfg.mass_eps = float(mass_eps)
+        cfg.sort_every = int(sort_every)
+        cfg.strict_domain = int(bool(strict_domain))
+        h = C.c_void_p()
+        L.check(self._lib.mpm_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.config = cfg
+        self.count = 0
+
+    def close(self):
+        if self.h:
+            self._lib.mpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        L.check(rc, self.h)
+
+    def set_materials(self, materials):
+        arr = (L.MpmMaterial * len(materials))(*[m.to_c() for m in materials])
+        self._check(self._lib.mpm_set_materials(self.h, arr, len(materials)))
+
+    def upload(self, x, v, mass, volume, *, F=None, Cmat=None, temperature=None, plastic=None, material=None,
+               pinned=None, cached_tau=None, cache_valid=None, dtype=np.float64):
+        x = np.asarray(x, np.float64)
+        n = len(x)
+        f = np.float64 if dtype == np.float64 else np.float32
+        cols = lambda a: [np.ascontiguousarray(a[:, k], f) for k in range(3)]
+        xs = cols(x)
+        vs = cols(np.broadcast_to(np.asarray(v, np.float64), (n, 3)))
+        F = np.tile(np.eye(3).reshape(9), n) if F is None else np.asarray(F).reshape(n * 9)
+        Cm = np.zeros(9 * n) if Cmat is None else np.asarray(Cmat).reshape(n * 9)
+        arr = lambda a, fill: np.ascontiguousarray(np.broadcast_to(np.asarray(fill if a is None else a, f), (n,)))
+        p = L.MpmParticles()
+        keep = dict(
+            x=xs[0], y=xs[1], z=xs[2], vx=vs[0], vy=vs[1], vz=vs[2],
+            F=np.ascontiguousarray(F, f), C=np.ascontiguousarray(Cm, f),
+            mass=arr(mass, 0), volume=arr(volume, 0), temperature=arr(temperature, 20.0),
+            plastic=arr(plastic, 0.0),
+            material=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if material is None else material, np.uint8),
+                                                          (n,))),
+            pinned=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if pinned is None else pinned, np.uint8),
+                                                        (n,))),
+        )
+        if cached_tau is not None:
+            keep["cached_tau"] = np.ascontiguousarray(np.asarray(cached_tau).reshape(n * 9), f)
+            keep["cache_valid"] = np.ascontiguousarray(np.broadcast_to(np.asarray(cache_valid, np.uint8), (n,)))
+        p.count = n
+        p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
+        for k, a in keep.items():
+            setattr(p, k, _ptr(a))
+        self._check(self._lib.mpm_upload(self.h, C.byref(p)))
+        self.count = n
+
+    def download(self, dtype=np.float64):
+        n = self.count
+        f = dtype
+        out = dict(x=np.zeros(n, f), y=np.zeros(n, f), z=np.zeros(n, f), vx=np.zeros(n, f), vy=np.zeros(n, f),
+                   vz=np.zeros(n, f), F=np.zeros(9 * n, f), C=np.zeros(9 * n, f), mass=np.zeros(n, f),
+                   volume=np.zeros(n, f), temperature=np.zeros(n, f), plastic=np.zeros(n, f),
+                   cached_tau=np.zeros(9 * n, f), material=np.zeros(n, np.uint8), pinned=np.zeros(n, np.uint8),
+                   cache_valid=np.zeros(n, np.uint8))
+        p = L.MpmParticles()
+        p.count = n
+        p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
+        for k, a in out.items():
+            setattr(p, k, _ptr(a))
+        self._check(self._lib.mpm_download(self.h, C.byref(p)))
+        out["positions"] = np.stack([out["x"], out["y"], out["z"]], 1)
+        out["velocities"] = np.stack([out["vx"], out["vy"], out["vz"]], 1)
+        return out
+
+    def step(self, dt, nsteps=1):
+        self._check(self._lib.mpm_step(self.h, float(dt), int(nsteps)))
+
+    def synchronize(self):
+        self._check(self._lib.mpm_synchronize(self.h))
+
+    def stable_step(self, cap=0.0, cfl=0.0):
+        out = C.c_double()
+        self._check(self._lib.mpm_stable_dt(self.h, cap, cfl, C.byref(out)))
+        return out.value
+
+    def conservation(self):
+        r = L.MpmConservation()
+        self._check(self._lib.mpm_conservation_report(self.h, C.byref(r)))
+        return dict(particle_mass=r.particle_mass, grid_mass=r.grid_mass, particle_mass_q=r.particle_mass_q,
+                    grid_mass_q=r.grid_mass_q, mass_scale=r.mass_scale, momentum=np.array(r.momentum[:]),
+                    grid_momentum=np.array(r.grid_momentum[:]), angular=np.array(r.angular[:]),
+                    kinetic=r.kinetic, affine=r.affine, min_J=r.min_J, max_speed=r.max_speed,
+                    nonfinite=r.nonfinite)
+
+    def min_j(self):
+        return self.conservation()["min_J"]
+
+    def refresh_diagnostics(self):
+        out = np.zeros(self.count)
+        self._check(self._lib.mpm_refresh_diagnostics(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def profile(self, enable=True, reset=True):
+        self._check(self._lib.mpm_profile(self.h, int(enable), int(reset)))
+
+    def kernel_times(self):
+        n = C.c_int32()
+        self._check(self._lib.mpm_kernel_times(self.h, None, 0, C.byref(n)))
+        arr = (L.MpmKernelTime * max(n.value, 1))()
+        self._check(self._lib.mpm_kernel_times(self.h, arr, n.value, C.byref(n)))
+        return {a.name.decode(): (a.launches, a.total_ms) for a in arr[:n.value]}
+
+    def debug_bin(self):
+        n = self.count
+        kin, kout, perm = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        P = C.POINTER(C.c_uint32)
+        self._check(self._lib.mpm_debug_bin(self.h, kin.ctypes.data_as(P), kout.ctypes.data_as(P),
+                                            perm.ctypes.data_as(P)))
+        return kin, kout, perm
+
+    def debug_p2g(self, dt):
+        N = C.c_int32()
+        self._check(self._lib.mpm_debug_p2g(self.h, dt, None, None, C.byref(N)))
+        nn = N.value ** 3
+        m, p = np.zeros(nn), np.zeros(3 * nn)
+        P = C.POINTER(C.c_double)
+        self._check(self._lib.mpm_debug_p2g(self.h, dt, m.ctypes.data_as(P), p.ctypes.data_as(P), C.byref(N)))
+        return m, p.reshape(nn, 3), N.value
+
+
+def sort_pairs(keys, key_bits=32):
+    """Stable device radix sort of uint32 keys; returns (sorted keys, permutation)."""
+    lib = L.load()
+    keys = np.ascontiguousarray(keys, np.uint32)
+    n = len(keys)
+    ko, perm = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    P = C.POINTER(C.c_uint32)
+    L.check(lib.mpm_sort_pairs(keys.ctypes.data_as(P), n, key_bits, ko.ctypes.data_as(P), perm.ctypes.data_as(P)))
+    return ko, perm
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def return_map_batch(material, F, ep, T, dt):
+    """Device plain return map on a batch (mpm_return_map_batch): returns
+    (F_out [n,9], ep_out [n], tau_out [n,6]) in FP32."""
+    lib = L.load()
+    F = np.ascontiguousarray(np.asarray(F, np.float32).reshape(-1, 9))
+    n = len(F)
+    ep = np.ascontiguousarray(np.broadcast_to(np.asarray(ep, np.float32), (n,)))
+    T = np.ascontiguousarray(np.broadcast_to(np.asarray(T, np.float32), (n,)))
+    Fo, epo, to = np.zeros((n, 9), np.float32), np.zeros(n, np.float32), np.zeros((n, 6), np.float32)
+    m = material.to_c()
+    L.check(lib.mpm_return_map_batch(C.byref(m), n, float(dt), _f32p(F), _f32p(ep), _f32p(T), _f32p(Fo), _f32p(epo),
+                                     _f32p(to)))
+    return Fo, epo, to
+
+
+def stress_batch(material, F):
+    """Device step stress of a batch of F (mpm_stress_batch), [n,6] FP32."""
+    lib = L.load()
+    F = np.ascontiguousarray(np.asarray(F, np.float32).reshape(-1, 9))
+    n = len(F)
+    to = np.zeros((n, 6), np.float32)
+    m = material.to_c()
+    L.check(lib.mpm_stress_batch(C.byref(m), n, _f32p(F), _f32p(to)))
+    return to
+
+
+def sym6_to_full(t):
+    """(xx, yy, zz, xy, xz, yz) -> row-major 3x3."""
+    t = np.asarray(t)
+    return np.stack([t[..., 0], t[..., 3], t[..., 4], t[..., 3], t[..., 1], t[..., 5], t[..., 4], t[..., 5],
+                     t[..., 2]], -1)

@@ -1,0 +1,127 @@
+"""ctypes binding of the engine's C ABI (include/mpm_engine.h).
+
+Loads ``lib/libmpm_amd.so`` from this package's parent directory. There is no
+fallback: if the library is missing the import of the engine fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmpm_amd.so")
+
+MPM_OK = 0
+MPM_ERR_INVALID_ARGUMENT = 1
+MPM_ERR_HIP = 2
+MPM_ERR_CAPACITY = 3
+MPM_ERR_UNSUPPORTED = 4
+MPM_ERR_NOT_READY = 5
+MPM_ERR_CFL = 7
+MPM_ERR_INVALID_STATE = 8
+
+MPM_WALL_SEPARATE, MPM_WALL_STICKY, MPM_WALL_SLIP = 0, 1, 2
+MPM_F64, MPM_F32 = 0, 1
+
+
+class MpmConfig(C.Structure):
+    _fields_ = [
+        ("lower", C.c_double * 3), ("upper", C.c_double * 3), ("cell_size", C.c_double),
+        ("capacity", C.c_uint32), ("wall_mode", C.c_int32 * 6), ("gravity", C.c_double * 3),
+        ("gravity_on_particles", C.c_int32), ("mass_eps", C.c_double), ("sort_every", C.c_int32),
+        ("strict_domain", C.c_int32),
+    ]
+
+
+_MAT_FLOATS = ["E", "mu", "lambda_",
+               "th_Tref", "th_Thot", "th_hotRatio", "th_A", "th_H", "th_rate", "th_Q", "th_n", "th_stress",
+               "th_kelvin_offset",
+               "pl_A", "pl_B", "pl_n", "pl_C", "pl_rate0", "pl_Tref", "pl_Tm", "pl_m"]
+
+
+class MpmMaterial(C.Structure):
+    _fields_ = [("model", C.c_int32)] + [(k, C.c_float) for k in _MAT_FLOATS]
+
+
+class MpmParticles(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32), ("dtype", C.c_int32),
+        ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+        ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+        ("F", C.c_void_p), ("C", C.c_void_p),
+        ("mass", C.c_void_p), ("volume", C.c_void_p), ("temperature", C.c_void_p),
+        ("plastic", C.c_void_p), ("cached_tau", C.c_void_p),
+        ("material", C.c_void_p), ("pinned", C.c_void_p), ("cache_valid", C.c_void_p),
+    ]
+
+
+class MpmConservation(C.Structure):
+    _fields_ = [
+        ("particle_mass", C.c_double), ("grid_mass", C.c_double),
+        ("particle_mass_q", C.c_int64), ("grid_mass_q", C.c_int64), ("mass_scale", C.c_double),
+        ("momentum", C.c_double * 3), ("grid_momentum", C.c_double * 3), ("angular", C.c_double * 3),
+        ("kinetic", C.c_double), ("affine", C.c_double), ("min_J", C.c_double), ("max_speed", C.c_double),
+        ("nonfinite", C.c_int64),
+    ]
+
+
+class MpmKernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
+
+
+# Every symbol declared in include/mpm_engine.h, with (restype, argtypes).
+_P = C.c_void_p
+SIGNATURES = {
+    "mpm_create": (C.c_int, [C.POINTER(MpmConfig), C.POINTER(_P)]),
+    "mpm_destroy": (C.c_int, [_P]),
+    "mpm_last_error": (C.c_char_p, [_P]),
+    "mpm_set_materials": (C.c_int, [_P, C.POINTER(MpmMaterial), C.c_int32]),
+    "mpm_upload": (C.c_int, [_P, C.POINTER(MpmParticles)]),
+    "mpm_download": (C.c_int, [_P, C.POINTER(MpmParticles)]),
+    "mpm_step": (C.c_int, [_P, C.c_double, C.c_int32]),
+    "mpm_stable_dt": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    "mpm_conservation_report": (C.c_int, [_P, C.POINTER(MpmConservation)]),
+    "mpm_refresh_diagnostics": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mpm_synchronize": (C.c_int, [_P]),
+    "mpm_count": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "mpm_profile": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "mpm_kernel_times": (C.c_int, [_P, C.POINTER(MpmKernelTime), C.c_int32, C.POINTER(C.c_int32)]),
+    "mpm_debug_bin": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mpm_sort_pairs": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
+    "mpm_debug_p2g": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.POINTER(C.c_int32)]),
+    "mpm_return_map_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.c_float] + [C.POINTER(C.c_float)] * 6),
+    "mpm_stress_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+class MpmError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"MPM error {code}: {message}")
+        self.code = code
+
+
+def load():
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"MI355X engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, handle=None):
+    if rc != MPM_OK:
+        msg = load().mpm_last_error(handle).decode() if handle else ""
+        raise MpmError(rc, msg)

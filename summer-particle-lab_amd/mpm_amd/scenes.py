@@ -1,0 +1,224 @@
+"""Scene seeding and the benchmark workloads.
+
+Seeding restates the reference's lattice and jitter rules, vectorised:
+  random(seed, index, attempt, axis)   src/three/sampling.js:8-12
+  jitterPoint                          src/three/sampling.js:13-29
+  geometry lattice (unrotated)         src/three/discretization.js:33-48
+  domain grid                          src/three/discretization.js:4-15
+The workloads are the MLS-MPM configurations the engine is measured on:
+  elastic_cube    unit domain, 32^3 cells, 0.3 cube, fixed-corotated
+  thermal_boxes   128^3 cells, ~2M particles in 6 seeded boxes, thermal J2
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import Material
+
+_M32 = 0xFFFFFFFF
+
+
+def random(seed, index, attempt, axis):
+    """Seeded uint32 hash mapped to [-1, 1); vectorised over numpy inputs."""
+    seed = np.asarray(seed, np.uint64)
+    index = np.asarray(index, np.int64)
+    attempt = np.asarray(attempt, np.int64)
+    axis = np.asarray(axis, np.int64)
+    a = ((index + 1).astype(np.uint64) & _M32) * np.uint64(0x9E3779B9)
+    b = ((attempt * 3 + axis + 1).astype(np.uint64) & _M32) * np.uint64(0x85EBCA6B)
+    h = (seed + (a & _M32) + (b & _M32)) & _M32
+    h = ((h ^ (h >> np.uint64(16))) * np.uint64(0x7FEB352D)) & _M32
+    h = ((h ^ (h >> np.uint64(15))) * np.uint64(0x846CA68B)) & _M32
+    h = (h ^ (h >> np.uint64(16))) & _M32
+    return h.astype(np.float64) / 4294967296.0 * 2 - 1
+
+
+def uniform(seed, index, attempt, axis, lo, hi):
+    """Uniform draw in [lo, hi) from the seeded hash."""
+    return lo + (hi - lo) * (random(seed, index, attempt, axis) + 1) * 0.5
+
+
+def normal(seed, index, attempt, axis):
+    """Standard normal draw (Box-Muller on two hash draws)."""
+    u1 = (random(seed, index, attempt, 2 * axis) + 1) * 0.5
+    u2 = (random(seed, index, attempt, 2 * axis + 1) + 1) * 0.5
+    u1 = np.maximum(u1, 1e-300)
+    return np.sqrt(-2 * np.log(u1)) * np.cos(2 * np.pi * u2)
+
+
+def domain(lower, upper, h):
+    """Background grid: origin = floor(lower/h) h - 2h, nodes = ceil((upper -
+    origin)/h) + 3 per axis (src/three/discretization.js:13)."""
+    lower, upper = np.asarray(lower, float), np.asarray(upper, float)
+    origin = np.floor(lower / h) * h - 2 * h
+    nodes = (np.ceil((upper - origin) / h) + 3).astype(np.int64)
+    return origin, nodes
+
+
+SHAPES = {"box": 0, "ellipsoid": 1, "cylinder": 2}
+
+
+def lattice(center, size, shape, spacing):
+    """Cell-centred lattice points (k + 1/2) spacing inside an unrotated shape,
+    z outermost, x fastest (src/three/discretization.js:33-48)."""
+    center, size = np.asarray(center, float), np.asarray(size, float)
+    lo, hi = center - size * .5, center + size * .5
+    first = np.ceil(lo / spacing - .5 - 1e-10).astype(np.int64)
+    last = np.ceil(hi / spacing - .5 - 1e-10).astype(np.int64) - 1
+    zs, ys, xs = [np.arange(first[a], last[a] + 1) for a in (2, 1, 0)]
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    p = np.stack([(X.ravel() + .5) * spacing, (Y.ravel() + .5) * spacing, (Z.ravel() + .5) * spacing], 1)
+    q = 2 * (p - center) / size
+    inside = np.all(np.abs(q) <= 1 + 1e-10, axis=1)
+    if shape == "ellipsoid":
+        inside &= (q ** 2).sum(1) <= 1 + 1e-10
+    elif shape == "cylinder":
+        inside &= q[:, 0] ** 2 + q[:, 2] ** 2 <= 1 + 1e-10
+    return p[inside]
+
+
+def jitter(points, lo, hi, shape, seed, strength, spacing, fixed=None):
+    """Bounded seeded jitter of points in local coordinates with reflection at
+    the faces and up to 8 retries (src/three/sampling.js:13-29). Point k uses
+    hash index k."""
+    p = np.asarray(points, float)
+    n = len(p)
+    if strength == 0 or n == 0:
+        return p.copy()
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    fixed = np.zeros(n, bool) if fixed is None else np.asarray(fixed, bool)
+    amplitude = spacing * strength
+    out = p.copy()
+    done = np.zeros(n, bool)
+    idx = np.arange(n)
+    for attempt in range(8):
+        d = np.stack([random(seed, idx, attempt, a) for a in range(3)], 1)
+        d[fixed, 0] = 0
+        scale = amplitude / np.maximum(1.0, np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]))
+        c = p + d * scale[:, None]
+        c = np.where(c < lo, 2 * lo - c, c)
+        c = np.where(c > hi, 2 * hi - c, c)
+        q = (c - (lo + hi) / 2) / ((hi - lo) / 2)
+        if shape == "box":
+            ok = np.ones(n, bool)
+        elif shape == "ellipsoid":
+            ok = q[:, 0] ** 2 + q[:, 1] ** 2 + q[:, 2] ** 2 <= 1 + 1e-12
+        else:
+            ok = q[:, 0] ** 2 + q[:, 2] ** 2 <= 1 + 1e-12
+        take = ok & ~done
+        out[take] = c[take]
+        done |= ok
+    return out
+
+
+def geometry(center, size, shape, spacing, seed=42, strength=0.0):
+    """Seed one geometry: lattice, then jitter in local coordinates
+    (src/three/simulation.js:66-71 with rotation 0). Returns (points, V_p)."""
+    center, size = np.asarray(center, float), np.asarray(size, float)
+    pts = lattice(center, size, shape, spacing)
+    local = pts - center
+    if strength > 0:
+        local = jitter(local, -size * .5, size * .5, shape, seed, strength, spacing)
+    vol = float(np.prod(size)) * (np.pi / 6 if shape == "ellipsoid" else np.pi / 4 if shape == "cylinder" else 1.0)
+    return local + center, vol / max(len(pts), 1)
+
+
+@dataclass
+class Scene:
+    """A workload: particle arrays plus engine configuration."""
+    name: str
+    lower: tuple
+    upper: tuple
+    h: float
+    x: np.ndarray
+    v: np.ndarray
+    mass: np.ndarray
+    volume: np.ndarray
+    temperature: np.ndarray
+    material: np.ndarray
+    materials: list
+    dt: float
+    steps: int
+    gravity: tuple = (0.0, -9.81, 0.0)
+    wall_mode: tuple = (0, 0, 0, 0, 0, 0)
+    gravity_on_particles: bool = False
+    pinned: np.ndarray = None
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return len(self.x)
+
+    def engine(self, capacity=None, sort_every=1):
+        from .engine import Engine
+        e = Engine(self.lower, self.upper, self.h, capacity or self.n, wall_mode=self.wall_mode,
+                   gravity=self.gravity, gravity_on_particles=self.gravity_on_particles, sort_every=sort_every)
+        e.set_materials(self.materials)
+        e.upload(self.x, self.v, self.mass, self.volume, temperature=self.temperature, material=self.material,
+                 pinned=self.pinned)
+        return e
+
+
+def elastic_cube(nu=0.3, E=1e5, rho=1000.0, sticky_floor=True, seed=42, strength=0.2):
+    """Elastic cube in the unit domain: h = 1/32, 2 particles per axis per cell
+    (spacing 1/64), side 0.3 centred in x/z with its base at y = 0.04, seeded
+    jitter 0.2; fixed-corotated E = 1e5 Pa, nu = 0.3, rho = 1000, gravity
+    -9.81, sticky floor, dt = 1e-4, 200 steps. nu = 0.25 with separating walls
+    and gravity on the particles is the reference's own convention
+    (mu = lambda = 0.4 E, src/three/mpm.js:37; src/three/simulation.js:337-344)."""
+    h = 1 / 32
+    pts, vp = geometry((0.5, 0.19, 0.5), (0.3, 0.3, 0.3), "box", h / 2, seed, strength)
+    n = len(pts)
+    mat = Material.from_E_nu(0, E, nu)
+    walls = (0, 0, 1, 0, 0, 0) if sticky_floor else (0,) * 6
+    return Scene("elastic_cube", (0, 0, 0), (1, 1, 1), h, pts, np.zeros((n, 3)), np.full(n, rho * vp),
+                 np.full(n, vp), np.full(n, 20.0), np.zeros(n, np.uint8), [mat], 1e-4, 200, wall_mode=walls,
+                 gravity_on_particles=not sticky_floor,
+                 info=dict(E=E, nu=nu, rho=rho, spacing=h / 2, particle_volume=vp))
+
+
+# Thermal-J2 steel in scaled units: E = 2e11 Pa, nu = 0.3, yield 2.5e8 Pa
+# (A = 1.25e-3 E) with linear softening to 10% between 300 K and 1200 K,
+# perfectly plastic (H = 0). The density is scaled to 2.5e7 kg/m^3 so that the
+# P-wave CFL number c_p dt / h stays 0.27 at h = 1/128, dt = 2e-5.
+STEEL_THERMAL = dict(Tref=300.0, Thot=1200.0, hotRatio=0.1, A=1.25e-3, H=0.0, kelvin_offset=0.0)
+STEEL_RHO_SCALED = 2.5e7
+
+
+def thermal_boxes(cells=128, nboxes=6, ppa=4, seed=6, side=(0.05, 0.2), T=(300.0, 900.0), vsigma=0.5,
+                  dt=2e-5, steps=500, rho=STEEL_RHO_SCALED):
+    """Thermal-J2 workload: 6 seeded non-overlapping boxes (sides uniform in
+    [0.05, 0.2]) in the unit domain at 128^3 cells, 4 particles per axis per
+    cell (~2M particles), per-box velocity N(0, 0.5 m/s) per component,
+    per-particle temperature uniform in [300, 900] K, gravity -9.81,
+    separating walls, dt = 2e-5, 500 steps."""
+    h = 1.0 / cells
+    spacing = h / ppa
+    boxes = []
+    attempt = 0
+    while len(boxes) < nboxes and attempt < 1000:
+        k = len(boxes)
+        s = uniform(seed, k, attempt, 0, side[0], side[1])
+        c = np.array([uniform(seed, k, attempt, 1 + a, 0.1 + s / 2, 0.9 - s / 2) for a in range(3)])
+        # Separated along at least one axis by a gap of two cells.
+        ok = all(np.any(np.abs(c - c2) > (s + s2) / 2 + 2 * h) for c2, s2 in boxes)
+        if ok:
+            boxes.append((c, float(s)))
+        attempt += 1
+    xs, vs, ms, vols = [], [], [], []
+    for k, (c, s) in enumerate(boxes):
+        pts, vp = geometry(c, (s, s, s), "box", spacing, seed + k, 0.2)
+        vel = np.array([vsigma * normal(seed, 10_000 + k, 0, a) for a in range(3)], float)
+        xs.append(pts)
+        vs.append(np.broadcast_to(vel, pts.shape))
+        ms.append(np.full(len(pts), rho * vp))
+        vols.append(np.full(len(pts), vp))
+    x = np.concatenate(xs)
+    n = len(x)
+    temp = uniform(seed + 7, np.arange(n), 0, 0, T[0], T[1])
+    mat = Material.from_E_nu(10, 2e11, 0.3, thermal=dict(STEEL_THERMAL))
+    return Scene("thermal_boxes", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
+                 np.concatenate(vols), temp, np.zeros(n, np.uint8), [mat], dt, steps,
+                 info=dict(boxes=[(c.tolist(), s) for c, s in boxes], rho=rho, spacing=spacing))

@@ -1,0 +1,81 @@
+"""The C-ABI library loads, exports every symbol include/mpm_engine.h declares,
+and the ctypes mirrors of its structs match the C layout. No device calls."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from mpm_amd import lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mpm_engine.h")
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(mpm_[a-z0-9_]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    syms = header_symbols()
+    assert len(syms) >= 19
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert syms == set(L.SIGNATURES), syms ^ set(L.SIGNATURES)
+
+
+def test_struct_layout_matches_c(tmp_path):
+    structs = {"mpm_config": L.MpmConfig, "mpm_material": L.MpmMaterial, "mpm_particles": L.MpmParticles,
+               "mpm_conservation": L.MpmConservation, "mpm_kernel_time": L.MpmKernelTime}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_engine.h"', "int main(void){"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            cf = "lambda" if fname == "lambda_" else fname
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line)
+    for cname, cls in structs.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_invalid_arguments_rejected_before_any_device_call():
+    lib = L.load()
+    h = C.c_void_p()
+    assert lib.mpm_create(None, C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    cfg = L.MpmConfig()
+    cfg.lower[:] = [0, 0, 0]
+    cfg.upper[:] = [1, 1, 1]
+    cfg.cell_size = 0.0
+    assert lib.mpm_create(C.byref(cfg), C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    cfg.cell_size = 8.0  # domain smaller than one cell
+    assert lib.mpm_create(C.byref(cfg), C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    cfg.cell_size = 1 / 32
+    cfg.wall_mode[0] = 7
+    assert lib.mpm_create(C.byref(cfg), C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    cfg.wall_mode[0] = 0
+    cfg.strict_domain = 1  # the reference caps h to [4, 16]
+    assert lib.mpm_create(C.byref(cfg), C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_step(None, 1e-3, 1) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_destroy(None) == L.MPM_OK
+    m = L.MpmMaterial()
+    m.model = 3  # sand: not supported by the engine
+    assert lib.mpm_return_map_batch(C.byref(m), 0, 0.0, None, None, None, None, None, None) == L.MPM_ERR_UNSUPPORTED
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(L, "_lib", None)
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        L.load()
