@@ -497,26 +497,39 @@ int mpm_download(mpm_engine* e, mpm_particles* p) {
 int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
   if (!e) return MPM_ERR_INVALID_ARGUMENT;
   if (!(dt > 0) || !std::isfinite(dt) || nsteps < 0) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt or nsteps");
+  if (nsteps == 0 || e->n == 0) return MPM_OK;
   const StepParams sp = params(e, dt);
+  if (e->steps_since_sort >= e->cfg.sort_every) {
+    int rc = rebin(e);
+    if (rc) return rc;
+  }
+  {
+    Timed t(e, "p2g");
+    launch_fused(e->stream, e->d, e->n, e->g, sp, 0);
+  }
   for (int s = 0; s < nsteps; ++s) {
-    if (e->steps_since_sort >= e->cfg.sort_every) {
-      int rc = rebin(e);
-      if (rc) return rc;
-    }
-    launch_clear_red(e->stream, e->d);
-    {
-      Timed t(e, "p2g");
-      launch_p2g(e->stream, e->d, e->n, e->g, sp);
-    }
     {
       Timed t(e, "grid");
       launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx);
     }
-    {
-      Timed t(e, "g2p");
-      launch_g2p(e->stream, e->d, e->n, e->g, sp);
-    }
     e->steps_since_sort++;
+    const bool last = s == nsteps - 1;
+    const bool resort = !last && e->steps_since_sort >= e->cfg.sort_every;
+    if (last || resort) {
+      {
+        Timed t(e, "g2p");
+        launch_fused(e->stream, e->d, e->n, e->g, sp, 2);
+      }
+      if (resort) {
+        int rc = rebin(e);
+        if (rc) return rc;
+        Timed t(e, "p2g");
+        launch_fused(e->stream, e->d, e->n, e->g, sp, 0);
+      }
+    } else {
+      Timed t(e, "g2p2g");
+      launch_fused(e->stream, e->d, e->n, e->g, sp, 1);
+    }
   }
   HIP_OK(e, hipGetLastError());
   return MPM_OK;
@@ -719,7 +732,7 @@ int mpm_debug_p2g(mpm_engine* e, double dt, double* grid_mass, double* grid_mome
   if (!grid_mass && !grid_momentum) return MPM_OK;
   const size_t Nn = (size_t)N * N * N;
   const StepParams sp = params(e, dt);
-  launch_p2g(e->stream, e->d, e->n, e->g, sp);
+  launch_fused(e->stream, e->d, e->n, e->g, sp, 0);
   std::vector<unsigned long long> m(Nn);
   std::vector<float> p(3 * Nn);
   HIP_OK(e, hipMemcpyAsync(m.data(), e->d.acc_m, 8 * Nn, hipMemcpyDeviceToHost, e->stream));

@@ -1,0 +1,441 @@
+// Fused particle kernel of the MLS-MPM step for gfx950.
+//
+// One launch per step does G2P of step s and P2G of step s+1 for a chunk of
+// 256 cell-sorted particles per workgroup (one particle per lane):
+//   1. gather v and C from the grid velocity (src/three/mpm.js:107-117); the
+//      27-node neighbourhoods of the chunk are first staged in LDS as one box
+//      of nodes, so the lanes read LDS instead of issuing 27 global loads;
+//   2. F <- (I + dt C) F, x += dt v (src/three/mpm.js:118-122), the plastic
+//      return for models >= 2 (src/three/mpm.js:126-131) and the stress of the
+//      next step (src/three/mpm.js:33-39);
+//   3. scatter of mass and APIC momentum into the next grid
+//      (src/three/mpm.js:66-85). Lanes whose particles share a cell are first
+//      summed inside the wave by a butterfly over aligned lane groups of equal
+//      cell key; one lane per group adds the group sum to an LDS box of nodes;
+//      the box is added to the global grid with one atomic per node and
+//      channel.
+// Node mass is 64-bit fixed point: a particle's 27 parts sum exactly to its
+// own fixed-point mass (the centre node takes the remainder), so the grid
+// total equals the particle total bit for bit in any summation order.
+//
+// MODE 0: P2G from the stored state (first step of a call, after a re-bin).
+// MODE 1: G2P of this step and P2G of the next (steady state); only x, F
+//         and ep are written back, v and C stay in registers.
+// MODE 2: G2P only, writing the full state (last step of a call).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpm_internal.h"
+
+namespace mpm {
+namespace {
+
+constexpr int FB = 256;            // threads per workgroup = particles per chunk
+constexpr int PATCH_ACC = 1024;    // LDS node box for the scatter (20 B per node)
+constexpr int PATCH_VEL = 1280;    // LDS node box for the gather (16 B per node)
+constexpr int LDS_BYTES = 20480;
+
+__device__ __forceinline__ float ld(const float* P, uint32_t cap, int k, uint32_t i) {
+  return P[(size_t)k * cap + i];
+}
+__device__ __forceinline__ void st(float* P, uint32_t cap, int k, uint32_t i, float v) {
+  P[(size_t)k * cap + i] = v;
+}
+
+__device__ __forceinline__ int base_axis(float x, float O, float invH, int N, float& t) {
+  const float q = __fmul_rn(__fsub_rn(x, O), invH);
+  int b = (int)floorf(__fsub_rn(q, 0.5f));
+  b = b < 0 ? 0 : (b > N - 3 ? N - 3 : b);
+  t = __fsub_rn(q, (float)b);
+  return b;
+}
+
+// Block-wide box of base cells: lo[a] = min b[a], hi[a] = max b[a] over the
+// lanes with `use` set. Returns false when no lane is in use.
+__device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], int hi[3], int* s_box) {
+  int mn[3], mx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = use ? b[a] : 0x7fffffff;
+    mx[a] = use ? b[a] : -1;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
+      mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      s_box[wave * 6 + a] = mn[a];
+      s_box[wave * 6 + 3 + a] = mx[a];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s_box[a];
+    hi[a] = s_box[3 + a];
+#pragma unroll
+    for (int w = 1; w < FB / 64; ++w) {
+      lo[a] = min(lo[a], s_box[w * 6 + a]);
+      hi[a] = max(hi[a], s_box[w * 6 + 3 + a]);
+    }
+  }
+  return hi[0] >= 0;
+}
+
+// Material record, read through the scalar path when the wave shares it.
+__device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restrict__ mats, uint32_t id) {
+  const uint32_t first = __builtin_amdgcn_readfirstlane(id);
+  if (__all(id == first)) return mats[first];
+  return mats[id];
+}
+
+// Node visiting order of the scatter: the 26 outer nodes, then the centre,
+// which receives the remainder of the fixed-point mass.
+struct NodeOff {
+  int i, j, k;
+};
+__device__ constexpr NodeOff kOrder[27] = {
+    {0, 0, 0}, {1, 0, 0}, {2, 0, 0}, {0, 1, 0}, {1, 1, 0}, {2, 1, 0}, {0, 2, 0}, {1, 2, 0}, {2, 2, 0},
+    {0, 0, 1}, {1, 0, 1}, {2, 0, 1}, {0, 1, 1}, {2, 1, 1}, {0, 2, 1}, {1, 2, 1}, {2, 2, 1},
+    {0, 0, 2}, {1, 0, 2}, {2, 0, 2}, {0, 1, 2}, {1, 1, 2}, {2, 1, 2}, {0, 2, 2}, {1, 2, 2}, {2, 2, 2},
+    {1, 1, 1}};
+
+template <int MODE>
+__global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
+                                              uint32_t n, const DevMaterial* __restrict__ mats, GridDims g,
+                                              StepParams sp, const float4* __restrict__ vel,
+                                              unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
+                                              uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
+                                              double* __restrict__ red) {
+  __shared__ __align__(16) unsigned char s_raw[LDS_BYTES];
+  __shared__ int s_box[6 * (FB / 64)];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t i = blockIdx.x * FB + tid;
+  const bool valid = i < n;
+  const uint32_t ic = valid ? i : 0u;
+  const int N = g.N;
+  const size_t Nn = (size_t)N * N * N;
+  const float H = g.H, invH = g.invH, dt = sp.dt;
+
+  if (MODE != 2 && blockIdx.x == 0 && tid < 8) red[tid] = 0.0;  // grid sums of this step's P2G
+
+  const uint32_t mt = meta[ic];
+  const uint32_t flags = mt >> 8;
+  const bool pinned = (flags & FLAG_PINNED) != 0;
+  const DevMaterial mat = load_material(mats, mt & 255u);
+
+  float x[3] = {ld(P, cap, P_X, ic), ld(P, cap, P_Y, ic), ld(P, cap, P_Z, ic)};
+  M3 F;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F[k] = ld(P, cap, P_F0 + k, ic);
+  float v[3] = {0.f, 0.f, 0.f}, Cm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float tau[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float ep = 0.f;
+  uint32_t new_flags = flags;
+
+  if (MODE == 0 || pinned) {
+    // State as stored: stress of this step from F (or the cached return).
+    v[0] = ld(P, cap, P_VX, ic);
+    v[1] = ld(P, cap, P_VY, ic);
+    v[2] = ld(P, cap, P_VZ, ic);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Cm[k] = ld(P, cap, P_C0 + k, ic);
+    if (mat.model < 2) {
+      fixed_corotated_tau(F, mat.mu, mat.lambda, tau);
+    } else if (flags & FLAG_CACHE) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) tau[k] = ld(P, cap, P_T0 + k, ic);
+    } else {
+      hencky_tau(F, mat, tau);
+    }
+  }
+
+  if (MODE != 0) {
+    // ---------------------------------------------------------------- G2P
+    float t[3], w[3][3];
+    int b[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      b[a] = base_axis(x[a], g.O[a], invH, N, t[a]);
+      bspline(t[a], w[a]);
+    }
+    int lo[3], hi[3];
+    const bool gather = valid && !pinned;
+    block_box(b, gather, lo, hi, s_box);
+    const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
+    const bool staged = hi[0] >= 0 && ex * ey * ez <= PATCH_VEL;
+    float4* s_vel = reinterpret_cast<float4*>(s_raw);
+    if (staged) {
+      const int cnt = ex * ey * ez;
+      for (int p = tid; p < cnt; p += FB) {
+        const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
+        s_vel[p] = vel[((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx)];
+      }
+    }
+    __syncthreads();
+    if (gather) {
+      float V[3] = {0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0}, Z[3] = {0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          float4 n0, n1, n2;
+          if (staged) {
+            const int r = ((b[2] - lo[2] + k) * ey + (b[1] - lo[1] + j)) * ex + (b[0] - lo[0]);
+            n0 = s_vel[r];
+            n1 = s_vel[r + 1];
+            n2 = s_vel[r + 2];
+          } else {
+            const size_t r = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
+            n0 = vel[r];
+            n1 = vel[r + 1];
+            n2 = vel[r + 2];
+          }
+          const float wjk = w[1][j] * w[2][k];
+          const float r0x = w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x;
+          const float r0y = w[0][0] * n0.y + w[0][1] * n1.y + w[0][2] * n2.y;
+          const float r0z = w[0][0] * n0.z + w[0][1] * n1.z + w[0][2] * n2.z;
+          const float r1x = w[0][1] * n1.x + 2.f * w[0][2] * n2.x;
+          const float r1y = w[0][1] * n1.y + 2.f * w[0][2] * n2.y;
+          const float r1z = w[0][1] * n1.z + 2.f * w[0][2] * n2.z;
+          const float ax = wjk * r0x, ay = wjk * r0y, az = wjk * r0z;
+          V[0] += ax;
+          V[1] += ay;
+          V[2] += az;
+          X[0] += wjk * r1x;
+          X[1] += wjk * r1y;
+          X[2] += wjk * r1z;
+          if (j) {
+            Y[0] += (float)j * ax;
+            Y[1] += (float)j * ay;
+            Y[2] += (float)j * az;
+          }
+          if (k) {
+            Z[0] += (float)k * ax;
+            Z[1] += (float)k * ay;
+            Z[2] += (float)k * az;
+          }
+        }
+      }
+      const float s4 = 4.f * invH;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cm[c * 3 + 0] = s4 * (X[c] - V[c] * t[0]);
+        Cm[c * 3 + 1] = s4 * (Y[c] - V[c] * t[1]);
+        Cm[c * 3 + 2] = s4 * (Z[c] - V[c] * t[2]);
+        v[c] = V[c];
+      }
+      M3 Fn;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          Fn[a * 3 + c] =
+              F[a * 3 + c] + dt * (Cm[a * 3] * F[c] + Cm[a * 3 + 1] * F[3 + c] + Cm[a * 3 + 2] * F[6 + c]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) x[a] += dt * V[a];
+      if (mat.model >= 2) {
+        ep = ld(P, cap, P_EP, ic);
+        ep += plastic_project(Fn, ep, dt, ld(P, cap, P_TEMP, ic), mat, tau);
+        new_flags |= FLAG_CACHE;
+      } else if (MODE == 1) {
+        fixed_corotated_tau(Fn, mat.mu, mat.lambda, tau);
+      }
+      F = Fn;
+    }
+    __syncthreads();  // the LDS box is reused by the scatter
+
+    if (valid && !pinned) {
+      st(P, cap, P_X, i, x[0]);
+      st(P, cap, P_Y, i, x[1]);
+      st(P, cap, P_Z, i, x[2]);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) st(P, cap, P_F0 + k, i, F[k]);
+      if (mat.model >= 2) st(P, cap, P_EP, i, ep);
+      if (new_flags != flags) meta[i] = (mt & 255u) | (new_flags << 8);
+      if (MODE == 2) {
+        st(P, cap, P_VX, i, v[0]);
+        st(P, cap, P_VY, i, v[1]);
+        st(P, cap, P_VZ, i, v[2]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) st(P, cap, P_C0 + k, i, Cm[k]);
+        if (mat.model >= 2) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) st(P, cap, P_T0 + k, i, tau[k]);
+        }
+      }
+    }
+    if (MODE == 2) return;
+  }
+
+  // -------------------------------------------------------------------- P2G
+  const float m = ld(P, cap, P_MASS, ic);
+  const float vol = ld(P, cap, P_VOL, ic);
+  if (g.gravity_on_particles && !pinned) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] += g.gravity[a] * dt;
+  }
+  const float k4 = dt * vol * 4.f * invH * invH;
+  const float T9[9] = {tau[0], tau[3], tau[4], tau[3], tau[1], tau[5], tau[4], tau[5], tau[2]};
+  float A[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) A[k] = m * Cm[k] - k4 * T9[k];
+  float t[3], w[3][3];
+  int b[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b[a] = base_axis(x[a], g.O[a], invH, N, t[a]);
+    bspline(t[a], w[a]);
+  }
+  // Momentum of node (i,j,k): w (m v + A ((i,j,k) - t) h).
+  float q0[3], c0[3], c1[3], c2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c0[a] = A[a * 3 + 0] * H;
+    c1[a] = A[a * 3 + 1] * H;
+    c2[a] = A[a * 3 + 2] * H;
+    q0[a] = m * v[a] - (c0[a] * t[0] + c1[a] * t[1] + c2[a] * t[2]);
+  }
+  float wyz[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wyz[j][k] = w[1][j] * w[2][k];
+  const float Qf = valid ? m * sp.mass_scale : 0.f;
+  const long long Q = (long long)__float2ll_rn(Qf);
+
+  // Aligned lane groups of equal cell key (butterfly): U[l] holds when the
+  // group of 2^(l+1) lanes containing this lane shares one key.
+  const uint32_t key = valid ? ((uint32_t)b[2] * (uint32_t)N + (uint32_t)b[1]) * (uint32_t)N + (uint32_t)b[0]
+                             : 0xffffffffu - lane;
+  bool U[6];
+  bool u = true;
+  int levels = 0;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    const uint32_t kp = __shfl_xor(key, 1 << l, 64);
+    const int up = __shfl_xor((int)u, 1 << l, 64);
+    u = u && up && kp == key;
+    U[l] = u;
+    if (__any(u)) levels = l + 1;
+  }
+  int gsize = 1;
+#pragma unroll
+  for (int l = 0; l < 6; ++l)
+    if (U[l]) gsize = 2 << l;
+  const bool owner = valid && ((lane & (uint32_t)(gsize - 1)) == 0);
+
+  int lo[3], hi[3];
+  block_box(b, valid, lo, hi, s_box);
+  const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
+  const int cnt = ex * ey * ez;
+  const bool staged = hi[0] >= 0 && cnt <= PATCH_ACC;
+  long long* s_m = reinterpret_cast<long long*>(s_raw);
+  float* s_p = reinterpret_cast<float*>(s_raw + 8 * PATCH_ACC);
+  if (staged) {
+    for (int p = tid; p < cnt; p += FB) {
+      s_m[p] = 0;
+      s_p[p] = 0.f;
+      s_p[PATCH_ACC + p] = 0.f;
+      s_p[2 * PATCH_ACC + p] = 0.f;
+    }
+  }
+  __syncthreads();
+  const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
+  long long qsum = 0;
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    const int ii = kOrder[o].i, j = kOrder[o].j, k = kOrder[o].k;
+    const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
+    long long qm;
+    if (o < 26) {
+      qm = (long long)__float2ll_rn(wt * Qf);
+      qsum += qm;
+    } else {
+      qm = Q - qsum;
+    }
+    float px = wt * (q0[0] + (float)ii * c0[0] + (float)j * c1[0] + (float)k * c2[0]);
+    float py = wt * (q0[1] + (float)ii * c0[1] + (float)j * c1[1] + (float)k * c2[1]);
+    float pz = wt * (q0[2] + (float)ii * c0[2] + (float)j * c1[2] + (float)k * c2[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      if (l < levels) {  // wave-uniform
+        const long long om = __shfl_xor(qm, 1 << l, 64);
+        const float ox = __shfl_xor(px, 1 << l, 64);
+        const float oy = __shfl_xor(py, 1 << l, 64);
+        const float oz = __shfl_xor(pz, 1 << l, 64);
+        if (U[l]) {
+          qm += om;
+          px += ox;
+          py += oy;
+          pz += oz;
+        }
+      }
+    }
+    if (owner) {
+      if (staged) {
+        const int ln = lbase + (k * ey + j) * ex + ii;
+        atomicAdd((unsigned long long*)&s_m[ln], (unsigned long long)qm);
+        atomicAdd(&s_p[ln], px);
+        atomicAdd(&s_p[PATCH_ACC + ln], py);
+        atomicAdd(&s_p[2 * PATCH_ACC + ln], pz);
+      } else {
+        const size_t node = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + (b[0] + ii);
+        atomicAdd(&acc_m[node], (unsigned long long)qm);
+        atomicAdd(&acc_p[node], px);
+        atomicAdd(&acc_p[Nn + node], py);
+        atomicAdd(&acc_p[2 * Nn + node], pz);
+        if (ii == 0) {
+          pieces[node >> 6] = 1;
+          pieces[(node + 2) >> 6] = 1;
+        }
+      }
+    }
+    if (pinned && valid && wt > 1e-8f) fixed[((size_t)(b[2] + k) * N + (b[1] + j)) * N + (b[0] + ii)] = 1;
+  }
+  if (staged) {
+    __syncthreads();
+    for (int p = tid; p < cnt; p += FB) {
+      const long long mq = s_m[p];
+      const float px = s_p[p], py = s_p[PATCH_ACC + p], pz = s_p[2 * PATCH_ACC + p];
+      if (mq != 0 || px != 0.f || py != 0.f || pz != 0.f) {
+        const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
+        const size_t node = ((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx);
+        atomicAdd(&acc_m[node], (unsigned long long)mq);
+        atomicAdd(&acc_p[node], px);
+        atomicAdd(&acc_p[Nn + node], py);
+        atomicAdd(&acc_p[2 * Nn + node], pz);
+        pieces[node >> 6] = 1;
+      }
+    }
+  }
+}
+
+inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
+
+}  // namespace
+
+void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode) {
+  if (!n) return;
+  const dim3 grid(fblocks(n)), block(FB);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_fused<0>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
+                       d.acc_p, d.fixed, d.pieces, d.red);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_fused<1>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
+                       d.acc_p, d.fixed, d.pieces, d.red);
+  else
+    hipLaunchKernelGGL(k_fused<2>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
+                       d.acc_p, d.fixed, d.pieces, d.red);
+}
+
+}  // namespace mpm

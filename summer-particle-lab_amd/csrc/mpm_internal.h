@@ -70,10 +70,11 @@ struct DevState {
 // Kernel launchers (mpm_kernels.hip)
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
 void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
-void launch_p2g(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p);
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
                  const int hi_idx[3]);
-void launch_g2p(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p);
+// Fused particle kernel (mpm_fused.hip): mode 0 P2G from the stored state,
+// 1 G2P of this step + P2G of the next, 2 G2P writing the full state.
+void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode);
 void launch_clear_red(hipStream_t s, const DevState& d);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 // Particle-side FP64 sums: particle_reduce_blocks() records of

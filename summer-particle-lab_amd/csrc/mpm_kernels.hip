@@ -1,16 +1,8 @@
 // MLS-MPM step kernels for gfx950 (FP32 state, FP64 conservation sums).
 //
-// Step structure of MPM3.step (src/three/mpm.js:56-134):
-//   k_p2g   stress for this step (src/three/mpm.js:33-39, 61) and the 27-node
-//           scatter of mass and APIC momentum (src/three/mpm.js:66-85)
+// Grid update, binning, permutation, reductions and batch constitutive
+// kernels. The particle transfers are in mpm_fused.hip.
 //   k_grid  v = p/m, gravity, wall rules (src/three/mpm.js:89-102)
-//   k_g2p   gather of v and C, F <- (I + dt C) F, x += dt v
-//           (src/three/mpm.js:107-123) and the plastic return
-//           (src/three/mpm.js:126-131)
-// Node mass is accumulated in 64-bit fixed point: each particle's mass
-// m*2^k is split into 27 integer parts whose sum is exactly m*2^k (the centre
-// node takes the remainder), so the grid total equals the particle total
-// bit for bit after every P2G, independent of the atomic order.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -85,81 +77,6 @@ __device__ __forceinline__ void step_stress(const M3& F, const DevMaterial& m, u
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_p2g(const float* __restrict__ P, const uint32_t* __restrict__ meta,
-                                               uint32_t cap, uint32_t n, const DevMaterial* __restrict__ mats,
-                                               GridDims g, StepParams sp, unsigned long long* __restrict__ acc_m,
-                                               float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
-                                               uint8_t* __restrict__ pieces) {
-  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t mt = meta[i];
-  const DevMaterial mat = mats[mt & 255u];
-  const uint32_t flags = mt >> 8;
-  const bool pinned = flags & FLAG_PINNED;
-  M3 F;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
-  float tau[6];
-  step_stress(F, mat, flags, P, cap, i, tau);
-  const float m = plane(P, cap, P_MASS, i);
-  const float vol = plane(P, cap, P_VOL, i);
-  float v[3] = {plane(P, cap, P_VX, i), plane(P, cap, P_VY, i), plane(P, cap, P_VZ, i)};
-  if (g.gravity_on_particles && !pinned) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v[a] += g.gravity[a] * sp.dt;
-  }
-  const float k4 = sp.dt * vol * 4.f * g.invH * g.invH;
-  float A[9];
-  const float T9[9] = {tau[0], tau[3], tau[4], tau[3], tau[1], tau[5], tau[4], tau[5], tau[2]};
-#pragma unroll
-  for (int k = 0; k < 9; ++k) A[k] = m * plane(P, cap, P_C0 + k, i) - k4 * T9[k];
-  float t[3], w[3][3];
-  int b[3];
-  b[0] = base_of(plane(P, cap, P_X, i), g.O[0], g.invH, g.N, t[0]);
-  b[1] = base_of(plane(P, cap, P_Y, i), g.O[1], g.invH, g.N, t[1]);
-  b[2] = base_of(plane(P, cap, P_Z, i), g.O[2], g.invH, g.N, t[2]);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) bspline(t[a], w[a]);
-  const float Qf = m * sp.mass_scale;
-  const long long Q = (long long)__float2ll_rn(Qf);
-  long long qsum = 0;
-  const size_t Nn = (size_t)g.N * g.N * g.N;
-  const int N = g.N;
-  const float H = g.H;
-  const float mv[3] = {m * v[0], m * v[1], m * v[2]};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const size_t row = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
-      pieces[row >> 6] = 1;
-      pieces[(row + 2) >> 6] = 1;
-      const float wjk = w[1][j] * w[2][k];
-      const float dy = ((float)j - t[1]) * H, dz = ((float)k - t[2]) * H;
-#pragma unroll
-      for (int ii = 0; ii < 3; ++ii) {
-        const float wt = w[0][ii] * wjk;
-        const float dx = ((float)ii - t[0]) * H;
-        const size_t node = row + ii;
-        long long q;
-        if (ii == 1 && j == 1 && k == 1) {
-          q = 0;  // centre node: filled with the remainder below
-        } else {
-          q = (long long)__float2ll_rn(wt * Qf);
-          qsum += q;
-          atomicAdd(&acc_m[node], (unsigned long long)q);
-        }
-        atomicAdd(&acc_p[node], wt * (mv[0] + A[0] * dx + A[1] * dy + A[2] * dz));
-        atomicAdd(&acc_p[Nn + node], wt * (mv[1] + A[3] * dx + A[4] * dy + A[5] * dz));
-        atomicAdd(&acc_p[2 * Nn + node], wt * (mv[2] + A[6] * dx + A[7] * dy + A[8] * dz));
-        if (pinned && wt > 1e-8f) fixed[node] = 1;
-      }
-    }
-  }
-  const size_t centre = ((size_t)(b[2] + 1) * N + (b[1] + 1)) * N + b[0] + 1;
-  atomicAdd(&acc_m[centre], (unsigned long long)(Q - qsum));
-}
-
 // One wave per 64-node piece of the dense grid; inactive pieces exit at once.
 __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
                                                 uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
@@ -230,92 +147,6 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
     atomicAdd(&red[3], p2);
     pieces[piece] = 0;
   }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_g2p(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
-                                               uint32_t n, const DevMaterial* __restrict__ mats, GridDims g,
-                                               StepParams sp, const float4* __restrict__ vel) {
-  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t mt = meta[i];
-  const uint32_t flags = mt >> 8;
-  if (flags & FLAG_PINNED) return;  // src/three/mpm.js:108
-  const DevMaterial mat = mats[mt & 255u];
-  float x[3] = {plane(P, cap, P_X, i), plane(P, cap, P_Y, i), plane(P, cap, P_Z, i)};
-  float t[3], w[3][3];
-  int b[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    b[a] = base_of(x[a], g.O[a], g.invH, g.N, t[a]);
-    bspline(t[a], w[a]);
-  }
-  const int N = g.N;
-  // Sums of w v and of w v times the node offset (i, j, k); C follows as
-  // (4/h) (sum w v (i,j,k)^T - (sum w v) t^T).
-  float V[3] = {0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0}, Z[3] = {0, 0, 0};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const size_t row = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
-      const float4 n0 = vel[row], n1 = vel[row + 1], n2 = vel[row + 2];
-      const float wjk = w[1][j] * w[2][k];
-      float r0[3], r1[3];
-      r0[0] = w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x;
-      r0[1] = w[0][0] * n0.y + w[0][1] * n1.y + w[0][2] * n2.y;
-      r0[2] = w[0][0] * n0.z + w[0][1] * n1.z + w[0][2] * n2.z;
-      r1[0] = w[0][1] * n1.x + 2.f * w[0][2] * n2.x;
-      r1[1] = w[0][1] * n1.y + 2.f * w[0][2] * n2.y;
-      r1[2] = w[0][1] * n1.z + 2.f * w[0][2] * n2.z;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float a0 = wjk * r0[c];
-        V[c] += a0;
-        X[c] += wjk * r1[c];
-        Y[c] += (float)j * a0;
-        Z[c] += (float)k * a0;
-      }
-    }
-  }
-  const float s4 = 4.f * g.invH;
-  float Cm[9];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    Cm[c * 3 + 0] = s4 * (X[c] - V[c] * t[0]);
-    Cm[c * 3 + 1] = s4 * (Y[c] - V[c] * t[1]);
-    Cm[c * 3 + 2] = s4 * (Z[c] - V[c] * t[2]);
-  }
-  M3 F;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
-  M3 Fn;
-  const float dt = sp.dt;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      Fn[a * 3 + c] =
-          F[a * 3 + c] + dt * (Cm[a * 3] * F[c] + Cm[a * 3 + 1] * F[3 + c] + Cm[a * 3 + 2] * F[6 + c]);
-  if (mat.model >= 2) {
-    float tau[6];
-    const float ep = plane(P, cap, P_EP, i);
-    const float inc = plastic_project(Fn, ep, dt, plane(P, cap, P_TEMP, i), mat, tau);
-    set_plane(P, cap, P_EP, i, ep + inc);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) set_plane(P, cap, P_T0 + k, i, tau[k]);
-    if (!(flags & FLAG_CACHE)) meta[i] = mt | (FLAG_CACHE << 8);
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    set_plane(P, cap, P_F0 + k, i, Fn[k]);
-    set_plane(P, cap, P_C0 + k, i, Cm[k]);
-  }
-  set_plane(P, cap, P_VX, i, V[0]);
-  set_plane(P, cap, P_VY, i, V[1]);
-  set_plane(P, cap, P_VZ, i, V[2]);
-  set_plane(P, cap, P_X, i, x[0] + dt * V[0]);
-  set_plane(P, cap, P_Y, i, x[1] + dt * V[1]);
-  set_plane(P, cap, P_Z, i, x[2] + dt * V[2]);
 }
 
 __global__ void k_clear_red(double* red) {
@@ -530,24 +361,12 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
   std::swap(d.meta, d.meta_alt);
 }
 
-void launch_p2g(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_p2g, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p,
-                     d.acc_m, d.acc_p, d.fixed, d.pieces);
-}
-
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
                  const int hi_idx[3]) {
   const size_t Nn = (size_t)g.N * g.N * g.N;
   const size_t npieces = (Nn + 63) / 64;
   hipLaunchKernelGGL(k_grid, dim3(blocks(npieces, BLOCK / 64)), dim3(BLOCK), 0, s, d.acc_m, d.acc_p, d.fixed,
                      d.pieces, d.vel, g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1], hi_idx[2], d.red);
-}
-
-void launch_g2p(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_g2p, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p,
-                     d.vel);
 }
 
 void launch_clear_red(hipStream_t s, const DevState& d) {

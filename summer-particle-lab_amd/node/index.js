@@ -1,0 +1,179 @@
+'use strict';
+// Drop-in `MPM3` for Summer-Particle-Lab's 3D MPM step API
+// (src/three/mpm.js:7-136), backed by the MI355X engine through the N-API
+// addon (mpm_napi.node) and the C ABI (include/mpm_engine.h).
+//
+// Same method surface: configure(domain), reset(), resetParticle(p),
+// remove(p, last), stableStep(s), step(s, dt), refreshDiagnostics(s), minJ(s),
+// and the same public arrays F, C, Jp, dilation, stress, plastic, cachedTau,
+// cacheValid (Float64Array/Uint8Array, 9 values per particle for tensors).
+// `step(s, dt)` reads the fields the reference step reads from `s` (count,
+// x..vz, mass, volume, model, pinned, isSolid, settings.stiffness, lab.part,
+// temperature, plasticity[], thermalMaterials[]) and writes x..vz and its own
+// arrays back, so callers and tests that mutate F/C between steps keep
+// working. Reference units are kept: E = m/(64 V) 16000 stiffness,
+// mu = lambda = 0.4 E, particle volume 64 V (src/three/mpm.js:34,37,68),
+// separating walls (src/three/mpm.js:93), no gravity inside the step.
+//
+// Option `resident: true` keeps particle state on the device between steps
+// (upload once, call `download(s)` to read it back); the default copies state
+// in and out on every step like the reference's in-place arrays.
+const native = require('./mpm_napi.node');
+
+const DEFAULT_DOMAIN = Object.freeze({lower: [-64, -48, -64], upper: [64, 72, 64], cellSize: 8, particlesPerAxis: 2});
+const THERMAL_KEYS = ['Tref', 'Thot', 'hotRatio', 'A', 'H', 'rate', 'Q', 'n', 'stress'];
+const PLASTIC_KEYS = ['A', 'B', 'n', 'C', 'rate0', 'Tref', 'Tm', 'm'];
+const SUPPORTED = new Set([0, 4, 5, 6, 7, 10, 11]);
+
+function identity() { return [1, 0, 0, 0, 1, 0, 0, 0, 1]; }
+function det(a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+class MPM3 {
+  constructor(capacity, options = {}) {
+    this.capacity = capacity;
+    this.options = Object.assign({sortEvery: 8, resident: false}, options);
+    this.F = new Float64Array(capacity * 9); this.C = new Float64Array(capacity * 9);
+    this.Jp = new Float64Array(capacity); this.dilation = new Float64Array(capacity);
+    this.stress = new Float64Array(capacity); this.plastic = new Float64Array(capacity);
+    this.tau = new Float64Array(capacity * 9); this.cachedTau = new Float64Array(capacity * 9);
+    this.cacheValid = new Uint8Array(capacity);
+    this.engine = null;
+    this.configure(this.options.domain || DEFAULT_DOMAIN);
+    this.reset();
+  }
+
+  configure(domain) {
+    this.domain = domain; this.H = domain.cellSize;
+    if (this.engine) native.destroy(this.engine);
+    this.engine = native.create({lower: domain.lower, upper: domain.upper, cellSize: domain.cellSize,
+      capacity: this.capacity, wallMode: [0, 0, 0, 0, 0, 0], gravity: [0, 0, 0], gravityOnParticles: 0,
+      massEps: 1e-12, sortEvery: this.options.sortEvery, strictDomain: 0});
+    this.onDevice = 0;
+  }
+
+  reset() { this.C.fill(0); this.F.fill(0); for (let p = 0; p < this.F.length / 9; p++) this.resetParticle(p); }
+
+  resetParticle(p) {
+    this.F.set(identity(), p * 9); this.C.fill(0, p * 9, p * 9 + 9); this.Jp[p] = 1; this.dilation[p] = 0;
+    this.stress[p] = 0; this.plastic[p] = 0; this.cacheValid[p] = 0; this.onDevice = 0;
+  }
+
+  remove(p, last) {
+    for (const key of ['F', 'C', 'cachedTau']) this[key].copyWithin(p * 9, last * 9, last * 9 + 9);
+    for (const key of ['Jp', 'dilation', 'stress', 'plastic', 'cacheValid']) this[key][p] = this[key][last];
+    this.onDevice = 0;
+  }
+
+  // Particles the step acts on and the material table of (model, part, E).
+  _gather(s) {
+    const ids = [];
+    const keys = new Map(), materials = [];
+    const material = new Uint8Array(s.count);
+    for (let p = 0; p < s.count; p++) {
+      if (s.isSolid && !s.isSolid(p)) continue;
+      const model = s.model ? s.model[p] : 0;
+      if (!SUPPORTED.has(model)) throw new Error(`model ${model} is not supported by the MI355X engine`);
+      const part = s.lab && s.lab.part ? s.lab.part[p] : 0;
+      const E = s.mass[p] / (64 * s.volume[p]) * 16000 * ((s.settings && s.settings.stiffness) || 1);
+      const key = `${model}|${part}|${E}`;
+      let m = keys.get(key);
+      if (m === undefined) {
+        m = materials.length;
+        if (m >= 64) throw new Error('more than 64 distinct materials');
+        keys.set(key, m);
+        const th = (s.thermalMaterials && s.thermalMaterials[part]) || {};
+        const pl = (s.plasticity && s.plasticity[part]) || {};
+        const thermal = {}, plastic = {};
+        for (const k of THERMAL_KEYS) if (Number.isFinite(th[k])) thermal[k] = th[k];
+        for (const k of PLASTIC_KEYS) if (Number.isFinite(pl[k])) plastic[k] = pl[k];
+        materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic});
+      }
+      material[ids.length] = m;
+      ids.push(p);
+    }
+    return {ids, materials, material: material.subarray(0, ids.length)};
+  }
+
+  _view(n) {
+    const v = {count: n};
+    for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature', 'plastic']) v[k] = new Float64Array(n);
+    v.F = new Float64Array(n * 9); v.C = new Float64Array(n * 9); v.cachedTau = new Float64Array(n * 9);
+    v.pinned = new Uint8Array(n); v.cacheValid = new Uint8Array(n); v.material = new Uint8Array(n);
+    return v;
+  }
+
+  // Copy the step's inputs from `s` and this object to the device.
+  upload(s) {
+    const {ids, materials, material} = this._gather(s);
+    const n = ids.length, v = this._view(n);
+    this._scale = new Float64Array(n);
+    for (let k = 0; k < n; k++) {
+      const p = ids[k];
+      v.x[k] = s.x[p]; v.y[k] = s.y[p]; v.z[k] = s.z[p]; v.vx[k] = s.vx[p]; v.vy[k] = s.vy[p]; v.vz[k] = s.vz[p];
+      v.mass[k] = s.mass[p]; v.volume[k] = 64 * s.volume[p];
+      v.temperature[k] = s.temperature ? s.temperature[p] : 20; v.plastic[k] = this.plastic[p];
+      v.pinned[k] = s.pinned ? s.pinned[p] : 0; v.material[k] = material[k];
+      v.cacheValid[k] = this.cacheValid[p];
+      // Cached return stress: unit-modulus in the reference, absolute here.
+      const E = materials[material[k]].E;
+      this._scale[k] = 0.4 * E;
+      for (let j = 0; j < 9; j++) {
+        v.F[k * 9 + j] = this.F[p * 9 + j]; v.C[k * 9 + j] = this.C[p * 9 + j];
+        v.cachedTau[k * 9 + j] = this.cachedTau[p * 9 + j] * this._scale[k];
+      }
+    }
+    native.setMaterials(this.engine, materials.length ? materials : [{model: 0, E: 1, mu: .4, lambda: .4}]);
+    native.upload(this.engine, v);
+    this._ids = ids; this._view_ = v; this.onDevice = 1;
+  }
+
+  // Copy device state back into `s` and this object.
+  download(s) {
+    const ids = this._ids, v = this._view_;
+    if (!ids) return;
+    native.download(this.engine, v);
+    for (let k = 0; k < ids.length; k++) {
+      const p = ids[k];
+      s.x[p] = v.x[k]; s.y[p] = v.y[k]; s.z[p] = v.z[k]; s.vx[p] = v.vx[k]; s.vy[p] = v.vy[k]; s.vz[p] = v.vz[k];
+      this.plastic[p] = v.plastic[k]; this.cacheValid[p] = v.cacheValid[k];
+      for (let j = 0; j < 9; j++) {
+        this.F[p * 9 + j] = v.F[k * 9 + j]; this.C[p * 9 + j] = v.C[k * 9 + j];
+        if (v.cacheValid[k]) this.cachedTau[p * 9 + j] = v.cachedTau[k * 9 + j] / this._scale[k];
+      }
+    }
+  }
+
+  stableStep(s) {
+    if (!this.options.resident || !this.onDevice) this.upload(s);
+    // Defaults of the C ABI are the reference constants h/8/240 and 1.4/8.
+    return native.stableDt(this.engine, 0, 0);
+  }
+
+  step(s, dt) {
+    if (!this.options.resident || !this.onDevice) this.upload(s);
+    native.step(this.engine, dt, 1);
+    if (!this.options.resident) this.download(s);
+  }
+
+  refreshDiagnostics(s) {
+    if (!this.options.resident || !this.onDevice) this.upload(s);
+    const out = new Float64Array(this._ids.length);
+    native.refreshDiagnostics(this.engine, out);
+    this.stress.fill(0, 0, s.count);
+    for (let k = 0; k < this._ids.length; k++) this.stress[this._ids[k]] = out[k];
+  }
+
+  minJ(s) {
+    let value = 1;
+    for (let i = 0; i < s.count; i++) if (!s.isSolid || s.isSolid(i)) value = Math.min(value, det(this.F.subarray(i * 9, i * 9 + 9)));
+    return value;
+  }
+
+  conservation() { return native.conservation(this.engine); }
+
+  destroy() { if (this.engine) native.destroy(this.engine); this.engine = null; }
+}
+
+module.exports = {MPM3, DEFAULT_DOMAIN, native};

@@ -1,0 +1,422 @@
+/*
+ * Node.js N-API addon over the engine's C ABI (include/mpm_engine.h).
+ *
+ * Exposes the C ABI one to one as functions on an external engine handle;
+ * node/index.js builds the reference's `MPM3` class (src/three/mpm.js:7-136)
+ * on top of it. Typed arrays cross the boundary without copies on the JS side:
+ * Float64Array (or Float32Array) views are passed straight to
+ * mpm_upload/mpm_download.
+ */
+#define NAPI_VERSION 4
+#include <node_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mpm_engine.h"
+
+#define CHECK_NAPI(env, call)                                   \
+  do {                                                          \
+    if ((call) != napi_ok) {                                    \
+      napi_throw_error((env), NULL, "N-API call failed: " #call); \
+      return NULL;                                              \
+    }                                                           \
+  } while (0)
+
+static napi_value throw_mpm(napi_env env, int rc, mpm_engine* e) {
+  char code[16], msg[600];
+  snprintf(code, sizeof code, "MPM%d", rc);
+  snprintf(msg, sizeof msg, "mpm error %d: %s", rc, e ? mpm_last_error(e) : "");
+  napi_throw_error(env, code, msg);
+  return NULL;
+}
+
+static void finalize_engine(napi_env env, void* data, void* hint) {
+  (void)env;
+  (void)hint;
+  mpm_engine** slot = (mpm_engine**)data;
+  if (*slot) mpm_destroy(*slot);
+  free(slot);
+}
+
+static mpm_engine* get_engine(napi_env env, napi_value v) {
+  void* data = NULL;
+  if (napi_get_value_external(env, v, &data) != napi_ok || !data) {
+    napi_throw_type_error(env, NULL, "expected an engine handle");
+    return NULL;
+  }
+  mpm_engine* e = *(mpm_engine**)data;
+  if (!e) napi_throw_error(env, NULL, "engine was destroyed");
+  return e;
+}
+
+static double get_num(napi_env env, napi_value obj, const char* key, double dflt) {
+  bool has = false;
+  napi_value v;
+  double out = dflt;
+  if (napi_has_named_property(env, obj, key, &has) == napi_ok && has &&
+      napi_get_named_property(env, obj, key, &v) == napi_ok) {
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t == napi_number) napi_get_value_double(env, v, &out);
+    if (t == napi_boolean) {
+      bool b;
+      napi_get_value_bool(env, v, &b);
+      out = b ? 1 : 0;
+    }
+  }
+  return out;
+}
+
+static void get_vec(napi_env env, napi_value obj, const char* key, double* out, int n) {
+  bool has = false;
+  napi_value arr, el;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return;
+  napi_get_named_property(env, obj, key, &arr);
+  for (int i = 0; i < n; ++i) {
+    if (napi_get_element(env, arr, (uint32_t)i, &el) == napi_ok) {
+      double d;
+      if (napi_get_value_double(env, el, &d) == napi_ok) out[i] = d;
+    }
+  }
+}
+
+/* create({lower, upper, cellSize, capacity, wallMode, gravity,
+ *         gravityOnParticles, massEps, sortEvery, strictDomain}) */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1) {
+    napi_throw_type_error(env, NULL, "create(config) needs a config object");
+    return NULL;
+  }
+  mpm_config c;
+  memset(&c, 0, sizeof c);
+  get_vec(env, argv[0], "lower", c.lower, 3);
+  get_vec(env, argv[0], "upper", c.upper, 3);
+  c.cell_size = get_num(env, argv[0], "cellSize", 0);
+  c.capacity = (uint32_t)get_num(env, argv[0], "capacity", 0);
+  double walls[6] = {0, 0, 0, 0, 0, 0};
+  get_vec(env, argv[0], "wallMode", walls, 6);
+  for (int f = 0; f < 6; ++f) c.wall_mode[f] = (int32_t)walls[f];
+  get_vec(env, argv[0], "gravity", c.gravity, 3);
+  c.gravity_on_particles = (int32_t)get_num(env, argv[0], "gravityOnParticles", 0);
+  c.mass_eps = get_num(env, argv[0], "massEps", 1e-12);
+  c.sort_every = (int32_t)get_num(env, argv[0], "sortEvery", 8);
+  c.strict_domain = (int32_t)get_num(env, argv[0], "strictDomain", 0);
+  mpm_engine* e = NULL;
+  int rc = mpm_create(&c, &e);
+  if (rc) return throw_mpm(env, rc, NULL);
+  mpm_engine** slot = (mpm_engine**)malloc(sizeof *slot);
+  *slot = e;
+  napi_value ext;
+  CHECK_NAPI(env, napi_create_external(env, slot, finalize_engine, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  void* data = NULL;
+  if (argc < 1 || napi_get_value_external(env, argv[0], &data) != napi_ok || !data) return NULL;
+  mpm_engine** slot = (mpm_engine**)data;
+  if (*slot) mpm_destroy(*slot);
+  *slot = NULL;
+  return NULL;
+}
+
+/* setMaterials(h, [{model, E, mu, lambda, thermal:{...}, plastic:{...}}]) */
+static napi_value js_set_materials(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  uint32_t n = 0;
+  CHECK_NAPI(env, napi_get_array_length(env, argv[1], &n));
+  if (n < 1 || n > 64) {
+    napi_throw_range_error(env, NULL, "1..64 materials required");
+    return NULL;
+  }
+  mpm_material mats[64];
+  memset(mats, 0, sizeof mats);
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value m, th, pl;
+    bool has;
+    CHECK_NAPI(env, napi_get_element(env, argv[1], i, &m));
+    mpm_material* o = &mats[i];
+    o->model = (int32_t)get_num(env, m, "model", 0);
+    o->E = (float)get_num(env, m, "E", 1);
+    o->mu = (float)get_num(env, m, "mu", .4);
+    o->lambda = (float)get_num(env, m, "lambda", .4);
+    /* Reference defaults: src/three/thermal-plasticity.js:3, metal-plasticity.js:3 */
+    o->th_Tref = 20; o->th_Thot = 180; o->th_hotRatio = .08f; o->th_A = .04f; o->th_H = .04f;
+    o->th_rate = .0005f; o->th_Q = 35000; o->th_n = 3; o->th_stress = .04f; o->th_kelvin_offset = 273.15f;
+    o->pl_A = .025f; o->pl_B = .08f; o->pl_n = .35f; o->pl_C = .025f; o->pl_rate0 = 1; o->pl_Tref = 20;
+    o->pl_Tm = 1000; o->pl_m = 1;
+    if (napi_has_named_property(env, m, "thermal", &has) == napi_ok && has &&
+        napi_get_named_property(env, m, "thermal", &th) == napi_ok) {
+      o->th_Tref = (float)get_num(env, th, "Tref", o->th_Tref);
+      o->th_Thot = (float)get_num(env, th, "Thot", o->th_Thot);
+      o->th_hotRatio = (float)get_num(env, th, "hotRatio", o->th_hotRatio);
+      o->th_A = (float)get_num(env, th, "A", o->th_A);
+      o->th_H = (float)get_num(env, th, "H", o->th_H);
+      o->th_rate = (float)get_num(env, th, "rate", o->th_rate);
+      o->th_Q = (float)get_num(env, th, "Q", o->th_Q);
+      o->th_n = (float)get_num(env, th, "n", o->th_n);
+      o->th_stress = (float)get_num(env, th, "stress", o->th_stress);
+      o->th_kelvin_offset = (float)get_num(env, th, "kelvinOffset", o->th_kelvin_offset);
+    }
+    if (napi_has_named_property(env, m, "plastic", &has) == napi_ok && has &&
+        napi_get_named_property(env, m, "plastic", &pl) == napi_ok) {
+      o->pl_A = (float)get_num(env, pl, "A", o->pl_A);
+      o->pl_B = (float)get_num(env, pl, "B", o->pl_B);
+      o->pl_n = (float)get_num(env, pl, "n", o->pl_n);
+      o->pl_C = (float)get_num(env, pl, "C", o->pl_C);
+      o->pl_rate0 = (float)get_num(env, pl, "rate0", o->pl_rate0);
+      o->pl_Tref = (float)get_num(env, pl, "Tref", o->pl_Tref);
+      o->pl_Tm = (float)get_num(env, pl, "Tm", o->pl_Tm);
+      o->pl_m = (float)get_num(env, pl, "m", o->pl_m);
+    }
+  }
+  int rc = mpm_set_materials(e, mats, (int32_t)n);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+/* Fills an mpm_particles view from {count, x, y, z, vx, vy, vz, F, C, mass,
+ * volume, temperature, plastic, cachedTau, material, pinned, cacheValid}:
+ * floating arrays all Float64Array or all Float32Array, flags Uint8Array. */
+static int particle_view(napi_env env, napi_value obj, mpm_particles* p) {
+  memset(p, 0, sizeof *p);
+  p->count = (uint32_t)get_num(env, obj, "count", 0);
+  p->dtype = -1;
+  struct {
+    const char* key;
+    void** dst;
+    int flag;
+  } fields[] = {{"x", &p->x, 0}, {"y", &p->y, 0}, {"z", &p->z, 0}, {"vx", &p->vx, 0}, {"vy", &p->vy, 0},
+                {"vz", &p->vz, 0}, {"F", &p->F, 0}, {"C", &p->C, 0}, {"mass", &p->mass, 0},
+                {"volume", &p->volume, 0}, {"temperature", &p->temperature, 0}, {"plastic", &p->plastic, 0},
+                {"cachedTau", &p->cached_tau, 0}, {"material", (void**)&p->material, 1},
+                {"pinned", (void**)&p->pinned, 1}, {"cacheValid", (void**)&p->cache_valid, 1}};
+  for (size_t k = 0; k < sizeof fields / sizeof fields[0]; ++k) {
+    bool has = false;
+    napi_value v;
+    if (napi_has_named_property(env, obj, fields[k].key, &has) != napi_ok || !has) continue;
+    napi_get_named_property(env, obj, fields[k].key, &v);
+    bool is_ta = false;
+    napi_is_typedarray(env, v, &is_ta);
+    if (!is_ta) continue;
+    napi_typedarray_type type;
+    size_t length, offset;
+    void* data;
+    napi_value ab;
+    napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &offset);
+    const size_t need = (fields[k].key[0] == 'F' || fields[k].key[0] == 'C' || !strcmp(fields[k].key, "cachedTau"))
+                            ? (size_t)p->count * 9
+                            : (size_t)p->count;
+    if (length < need) {
+      napi_throw_range_error(env, NULL, fields[k].key);
+      return -1;
+    }
+    if (fields[k].flag) {
+      if (type != napi_uint8_array) {
+        napi_throw_type_error(env, NULL, "flag arrays must be Uint8Array");
+        return -1;
+      }
+    } else {
+      const int dt = type == napi_float64_array ? MPM_F64 : type == napi_float32_array ? MPM_F32 : -2;
+      if (dt < 0 || (p->dtype >= 0 && p->dtype != dt)) {
+        napi_throw_type_error(env, NULL, "floating arrays must all be Float64Array or all Float32Array");
+        return -1;
+      }
+      p->dtype = dt;
+    }
+    *fields[k].dst = data;
+  }
+  if (p->dtype < 0) p->dtype = MPM_F64;
+  return 0;
+}
+
+static napi_value js_upload(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm_particles p;
+  if (particle_view(env, argv[1], &p)) return NULL;
+  int rc = mpm_upload(e, &p);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+static napi_value js_download(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm_particles p;
+  if (particle_view(env, argv[1], &p)) return NULL;
+  int rc = mpm_download(e, &p);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+static napi_value js_step(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  double dt = 0;
+  int32_t n = 1;
+  napi_get_value_double(env, argv[1], &dt);
+  if (argc >= 3) napi_get_value_int32(env, argv[2], &n);
+  int rc = mpm_step(e, dt, n);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+static napi_value js_synchronize(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  int rc = mpm_synchronize(e);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+static napi_value js_stable_dt(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  double cap = 0, cfl = 0, out = 0;
+  if (argc >= 2) napi_get_value_double(env, argv[1], &cap);
+  if (argc >= 3) napi_get_value_double(env, argv[2], &cfl);
+  int rc = mpm_stable_dt(e, cap, cfl, &out);
+  if (rc) return throw_mpm(env, rc, e);
+  napi_value r;
+  CHECK_NAPI(env, napi_create_double(env, out, &r));
+  return r;
+}
+
+static void set_num(napi_env env, napi_value obj, const char* key, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, key, x);
+}
+
+static void set_vec(napi_env env, napi_value obj, const char* key, const double* v, int n) {
+  napi_value arr, x;
+  napi_create_array_with_length(env, (size_t)n, &arr);
+  for (int i = 0; i < n; ++i) {
+    napi_create_double(env, v[i], &x);
+    napi_set_element(env, arr, (uint32_t)i, x);
+  }
+  napi_set_named_property(env, obj, key, arr);
+}
+
+static napi_value js_conservation(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm_conservation c;
+  int rc = mpm_conservation_report(e, &c);
+  if (rc) return throw_mpm(env, rc, e);
+  napi_value o, b;
+  CHECK_NAPI(env, napi_create_object(env, &o));
+  set_num(env, o, "particleMass", c.particle_mass);
+  set_num(env, o, "gridMass", c.grid_mass);
+  /* fixed-point totals as BigInt-free strings keep all 64 bits */
+  char buf[32];
+  snprintf(buf, sizeof buf, "%lld", (long long)c.particle_mass_q);
+  napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &b);
+  napi_set_named_property(env, o, "particleMassQ", b);
+  snprintf(buf, sizeof buf, "%lld", (long long)c.grid_mass_q);
+  napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &b);
+  napi_set_named_property(env, o, "gridMassQ", b);
+  set_num(env, o, "massScale", c.mass_scale);
+  set_vec(env, o, "momentum", c.momentum, 3);
+  set_vec(env, o, "gridMomentum", c.grid_momentum, 3);
+  set_vec(env, o, "angular", c.angular, 3);
+  set_num(env, o, "kinetic", c.kinetic);
+  set_num(env, o, "affine", c.affine);
+  set_num(env, o, "minJ", c.min_J);
+  set_num(env, o, "maxSpeed", c.max_speed);
+  set_num(env, o, "nonfinite", (double)c.nonfinite);
+  return o;
+}
+
+static napi_value js_refresh_diagnostics(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  napi_typedarray_type type;
+  size_t length, offset;
+  void* data;
+  napi_value ab;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[1], &is_ta);
+  if (!is_ta || napi_get_typedarray_info(env, argv[1], &type, &length, &data, &ab, &offset) != napi_ok ||
+      type != napi_float64_array) {
+    napi_throw_type_error(env, NULL, "refreshDiagnostics needs a Float64Array");
+    return NULL;
+  }
+  uint32_t n = 0;
+  mpm_count(e, &n);
+  if (length < n) {
+    napi_throw_range_error(env, NULL, "stress array too short");
+    return NULL;
+  }
+  int rc = mpm_refresh_diagnostics(e, (double*)data);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+static napi_value js_last_error(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], r;
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  CHECK_NAPI(env, napi_create_string_utf8(env, mpm_last_error(e), NAPI_AUTO_LENGTH, &r));
+  return r;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  struct {
+    const char* name;
+    napi_callback fn;
+  } fns[] = {{"create", js_create},
+             {"destroy", js_destroy},
+             {"setMaterials", js_set_materials},
+             {"upload", js_upload},
+             {"download", js_download},
+             {"step", js_step},
+             {"synchronize", js_synchronize},
+             {"stableDt", js_stable_dt},
+             {"conservation", js_conservation},
+             {"refreshDiagnostics", js_refresh_diagnostics},
+             {"lastError", js_last_error}};
+  for (size_t i = 0; i < sizeof fns / sizeof fns[0]; ++i) {
+    napi_value f;
+    CHECK_NAPI(env, napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f));
+    CHECK_NAPI(env, napi_set_named_property(env, exports, fns[i].name, f));
+  }
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

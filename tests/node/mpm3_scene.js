@@ -1,0 +1,31 @@
+'use strict';
+// Drives the drop-in MPM3 (summer-particle-lab_amd/node) on a scene given as
+// JSON {points, velocity, models, F, dt, steps, thermal?} in reference units
+// and prints the resulting state as JSON. The scene object carries exactly
+// the fields MPM3.step reads in the reference (src/three/mpm.js:56-134).
+const path = require('path');
+const fs = require('fs');
+const {MPM3} = require(path.join(__dirname, '..', '..', 'summer-particle-lab_amd', 'node', 'index.js'));
+
+const input = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));
+const n = input.points.length;
+const s = {count: n, settings: {stiffness: 1}, lab: {part: new Uint8Array(n)},
+  thermalMaterials: [input.thermal || {}], plasticity: [input.plastic || {}], isSolid: () => true};
+for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature']) s[k] = new Float64Array(n);
+s.model = new Uint8Array(n); s.pinned = new Uint8Array(n);
+for (let i = 0; i < n; i++) {
+  [s.x[i], s.y[i], s.z[i]] = input.points[i];
+  [s.vx[i], s.vy[i], s.vz[i]] = input.velocity[i] || input.velocity;
+  s.mass[i] = 1.1; s.volume[i] = 1; s.temperature[i] = input.temperature || 20; s.model[i] = input.models[i];
+}
+const mpm = new MPM3(n, {sortEvery: 1});
+if (input.F) for (let i = 0; i < n; i++) mpm.F.set(input.F, i * 9);
+const dtStable = mpm.stableStep(s);
+for (let k = 0; k < input.steps; k++) mpm.step(s, input.dt);
+mpm.refreshDiagnostics(s);
+const out = {dtStable, x: Array.from(s.x), y: Array.from(s.y), z: Array.from(s.z), vx: Array.from(s.vx),
+  vy: Array.from(s.vy), vz: Array.from(s.vz), F: Array.from(mpm.F.subarray(0, n * 9)),
+  plastic: Array.from(mpm.plastic.subarray(0, n)), stress: Array.from(mpm.stress.subarray(0, n)),
+  minJ: mpm.minJ(s), conservation: mpm.conservation()};
+mpm.destroy();
+console.log(JSON.stringify(out));

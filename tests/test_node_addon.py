@@ -1,0 +1,81 @@
+"""The Node.js drop-in: the N-API addon loads under node, exports the C ABI
+surface and rejects bad input before touching a device (CPU); and the JS
+MPM3 class steps the reference's own 4-block parity scene on the GPU
+(gpu/reference.mjs:1-9, models 0 and 10 instead of the unsupported 1-3) to
+the reference's CPU-GPU bar: positions atol .003 rtol .001 after 12 steps
+(gpu/test_solver.py:42)."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import binding as o
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE_DIR = os.path.join(ROOT, "summer-particle-lab_amd", "node")
+NODE = shutil.which("node")
+
+
+def node(code, *args):
+    return subprocess.run([NODE, "-e", code, *args], capture_output=True, text=True, timeout=120)
+
+
+def test_addon_loads_and_validates_without_device():
+    assert NODE, "node is required for the drop-in addon"
+    r = node(f"""
+const m = require({json.dumps(os.path.join(NODE_DIR, 'mpm_napi.node'))});
+const names = Object.keys(m).sort();
+let code = null;
+try {{ m.create({{lower: [0, 0, 0], upper: [1, 1, 1], cellSize: 0, capacity: 10}}); }} catch (e) {{ code = e.code; }}
+let typeErr = null;
+try {{ m.step({{}}, 1e-3, 1); }} catch (e) {{ typeErr = e.message; }}
+console.log(JSON.stringify({{names, code, typeErr}}));
+""")
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out["names"] == sorted(["create", "destroy", "setMaterials", "upload", "download", "step", "synchronize",
+                                   "stableDt", "conservation", "refreshDiagnostics", "lastError"])
+    assert out["code"] == "MPM1"
+    assert "engine handle" in out["typeErr"]
+
+
+def four_blocks():
+    """Four 8^3 blocks at spacing 4 (Simulation3.box lattice), one model each."""
+    pts, models = [], []
+    for k, model in enumerate((0, 10, 0, 10)):
+        x0, z0 = (-24 if k % 2 == 0 else 24), (-24 if k < 2 else 24)
+        for z in range(3):
+            for y in range(3):
+                for x in range(3):
+                    pts.append([x0 - 4 + 4 * x, 0 + 4 * y, z0 - 4 + 4 * z])
+                    models.append(model)
+    return np.array(pts, float), np.array(models)
+
+
+@pytest.mark.gpu
+def test_drop_in_mpm3_matches_oracle_on_reference_scene(gpu, tmp_path):
+    pts, models = four_blocks()
+    F = [.98, .1, 0, 0, .95, 0, 0, 0, 1.02]
+    dt, steps = 1 / 240, 12
+    inp = dict(points=pts.tolist(), velocity=[0, -3, 0], models=models.tolist(), F=F, dt=dt, steps=steps)
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout)
+    orc = o.Scene(pts, (0, -3, 0), 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8, model=models)
+    orc.F[:] = np.tile(F, len(pts))
+    assert abs(got["dtStable"] - orc.stable_step()) <= 1e-6 * orc.stable_step()
+    for _ in range(steps):
+        orc.step(dt)
+    x = np.stack([got["x"], got["y"], got["z"]], 1)
+    np.testing.assert_allclose(x, orc.positions(), atol=.003, rtol=.001)
+    np.testing.assert_allclose(np.array(got["F"]), orc.F, atol=4e-5, rtol=4e-4)
+    np.testing.assert_allclose(np.array(got["plastic"]), orc.plastic, atol=2e-5)
+    assert got["conservation"]["gridMassQ"] == got["conservation"]["particleMassQ"]
+    orc.refresh_diagnostics()
+    np.testing.assert_allclose(np.array(got["stress"]), orc.stress, atol=3e-5, rtol=5e-3)
