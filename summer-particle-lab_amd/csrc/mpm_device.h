@@ -61,7 +61,7 @@ __device__ __forceinline__ void polar_rotation(const M3& F, M3& R) {
       for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? 1.f : 0.f;
       return;
     }
-    const float id = 1.0f / d;
+    const float id = __builtin_amdgcn_rcpf(d);
     float inv[9];  // inverse transpose = cofactor / det
     inv[0] = c0 * id;
     inv[1] = c1 * id;
@@ -122,9 +122,9 @@ __device__ __forceinline__ void jacobi_eig3(float a00, float a11, float a22, flo
     // t = sign(theta)/(|theta| + sqrt(theta^2+1)), theta = (aqq - app)/(2 apq).
 #define MPM_JACOBI_ROT(app, aqq, apq, akp, akq, P, Q)                               \
   if (apq != 0.f) {                                                                 \
-    const float theta = (aqq - app) / (2.f * apq);                                  \
-    const float tt = copysignf(1.f, theta) / (fabsf(theta) + sqrtf(theta * theta + 1.f)); \
-    const float c = 1.f / sqrtf(tt * tt + 1.f), s = tt * c;                         \
+    const float theta = (aqq - app) * __builtin_amdgcn_rcpf(2.f * apq);             \
+    const float tt = copysignf(__builtin_amdgcn_rcpf(fabsf(theta) + __builtin_amdgcn_sqrtf(theta * theta + 1.f)), theta); \
+    const float c = __builtin_amdgcn_rsqf(tt * tt + 1.f), s = tt * c;               \
     app = app - tt * apq;                                                           \
     aqq = aqq + tt * apq;                                                           \
     apq = 0.f;                                                                      \
@@ -161,11 +161,13 @@ __device__ __forceinline__ void svd3(const M3& F, float U[9], float V[9], float 
   jacobi_eig3(a[0], a[1], a[2], a[3], a[4], a[5], V, lam);
 #pragma unroll
   for (int k = 0; k < 3; ++k) sigma[k] = sqrtf(fmaxf(1e-16f, lam[k]));
+  const float inv[3] = {__builtin_amdgcn_rcpf(sigma[0]), __builtin_amdgcn_rcpf(sigma[1]),
+                        __builtin_amdgcn_rcpf(sigma[2])};
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-      U[i * 3 + j] = (F[i * 3] * V[j] + F[i * 3 + 1] * V[3 + j] + F[i * 3 + 2] * V[6 + j]) / sigma[j];
+      U[i * 3 + j] = (F[i * 3] * V[j] + F[i * 3 + 1] * V[3 + j] + F[i * 3 + 2] * V[6 + j]) * inv[j];
 }
 
 // Symmetric U diag(d) U^T as a 6-vector (xx, yy, zz, xy, xz, yz).

@@ -318,15 +318,16 @@ int mpm_create(const mpm_config* c, mpm_engine** out) {
   alloc((void**)&d.vals_alt, sizeof(uint32_t) * cap);
   alloc((void**)&d.hist, sizeof(uint32_t) * radix_hist_words(cap));
   alloc((void**)&d.partials, sizeof(uint32_t) * radix_partial_words(cap));
-  alloc((void**)&d.red, sizeof(double) * 8);
+  alloc((void**)&d.red, sizeof(double) * 4 * grid_blocks(g));
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
-  if (st == hipSuccess) st = hipHostMalloc((void**)&e->red_host, sizeof(double) * 8, hipHostMallocDefault);
+  if (st == hipSuccess)
+    st = hipHostMalloc((void**)&e->red_host, sizeof(double) * 4 * grid_blocks(g), hipHostMallocDefault);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, (Nn + 63) / 64, e->stream);
-  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 8, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * grid_blocks(g), e->stream);
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) {
     mpm_destroy(e);
@@ -577,7 +578,9 @@ int mpm_conservation_report(mpm_engine* e, mpm_conservation* out) {
   launch_particle_reduce(e->stream, e->d, e->n, (float)e->mass_scale, D, dbuf);
   std::vector<double> h((size_t)K * nb);
   hipError_t st = hipMemcpyAsync(h.data(), dbuf, sizeof(double) * K * nb, hipMemcpyDeviceToHost, e->stream);
-  if (st == hipSuccess) st = hipMemcpyAsync(e->red_host, e->d.red, sizeof(double) * 8, hipMemcpyDeviceToHost, e->stream);
+  const uint32_t gb = grid_blocks(e->g);
+  if (st == hipSuccess)
+    st = hipMemcpyAsync(e->red_host, e->d.red, sizeof(double) * 4 * gb, hipMemcpyDeviceToHost, e->stream);
   hipFreeAsync(dbuf, e->stream);
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "conservation: %s", hipGetErrorString(st));
@@ -591,15 +594,21 @@ int mpm_conservation_report(mpm_engine* e, mpm_conservation* out) {
   }
   out->particle_mass = acc[0];
   out->particle_mass_q = e->particle_mass_q;
-  int64_t gq;
-  std::memcpy(&gq, &e->red_host[0], sizeof gq);
+  int64_t gq = 0;
+  double gp[3] = {0, 0, 0};
+  for (uint32_t b = 0; b < gb; ++b) {
+    int64_t q;
+    std::memcpy(&q, &e->red_host[4 * b], sizeof q);
+    gq += q;
+    for (int c = 0; c < 3; ++c) gp[c] += e->red_host[4 * b + 1 + c];
+  }
   out->grid_mass_q = gq;
   out->mass_scale = e->mass_scale;
   out->grid_mass = (double)gq / e->mass_scale;
   for (int k = 0; k < 3; ++k) {
     out->momentum[k] = acc[2 + k];
     out->angular[k] = acc[5 + k];
-    out->grid_momentum[k] = e->red_host[1 + k];
+    out->grid_momentum[k] = gp[k];
   }
   out->kinetic = acc[8];
   out->affine = acc[9];

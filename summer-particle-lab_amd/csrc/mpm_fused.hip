@@ -9,11 +9,10 @@
 //      return for models >= 2 (src/three/mpm.js:126-131) and the stress of the
 //      next step (src/three/mpm.js:33-39);
 //   3. scatter of mass and APIC momentum into the next grid
-//      (src/three/mpm.js:66-85). Lanes whose particles share a cell are first
-//      summed inside the wave by a butterfly over aligned lane groups of equal
-//      cell key; one lane per group adds the group sum to an LDS box of nodes;
-//      the box is added to the global grid with one atomic per node and
-//      channel.
+//      (src/three/mpm.js:66-85). The node contributions of a wave are
+//      transposed through LDS and summed over each run of particles sharing a
+//      base cell; each run sum is added to an LDS box of nodes, and the box is
+//      added to the global grid with one atomic per node and channel.
 // Node mass is 64-bit fixed point: a particle's 27 parts sum exactly to its
 // own fixed-point mass (the centre node takes the remainder), so the grid
 // total equals the particle total bit for bit in any summation order.
@@ -33,7 +32,11 @@ namespace {
 constexpr int FB = 256;            // threads per workgroup = particles per chunk
 constexpr int PATCH_ACC = 1024;    // LDS node box for the scatter (20 B per node)
 constexpr int PATCH_VEL = 1280;    // LDS node box for the gather (16 B per node)
-constexpr int LDS_BYTES = 20480;
+constexpr int ACC_BYTES = 20 * PATCH_ACC;
+constexpr int TR_ROWS = 45;        // per-wave transpose: 18 mass words + 27 momentum rows
+constexpr int TR_STRIDE = 68;      // 64 lanes + 4 words of padding (16-byte rows, spread banks)
+constexpr int LDS_BYTES = ACC_BYTES + (FB / 64) * TR_ROWS * TR_STRIDE * 4;
+static_assert(LDS_BYTES >= 16 * PATCH_VEL, "gather box must fit the shared buffer");
 
 __device__ __forceinline__ float ld(const float* P, uint32_t cap, int k, uint32_t i) {
   return P[(size_t)k * cap + i];
@@ -97,24 +100,12 @@ __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restri
   return mats[id];
 }
 
-// Node visiting order of the scatter: the 26 outer nodes, then the centre,
-// which receives the remainder of the fixed-point mass.
-struct NodeOff {
-  int i, j, k;
-};
-__device__ constexpr NodeOff kOrder[27] = {
-    {0, 0, 0}, {1, 0, 0}, {2, 0, 0}, {0, 1, 0}, {1, 1, 0}, {2, 1, 0}, {0, 2, 0}, {1, 2, 0}, {2, 2, 0},
-    {0, 0, 1}, {1, 0, 1}, {2, 0, 1}, {0, 1, 1}, {2, 1, 1}, {0, 2, 1}, {1, 2, 1}, {2, 2, 1},
-    {0, 0, 2}, {1, 0, 2}, {2, 0, 2}, {0, 1, 2}, {1, 1, 2}, {2, 1, 2}, {0, 2, 2}, {1, 2, 2}, {2, 2, 2},
-    {1, 1, 1}};
-
 template <int MODE>
 __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
                                               uint32_t n, const DevMaterial* __restrict__ mats, GridDims g,
                                               StepParams sp, const float4* __restrict__ vel,
                                               unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-                                              uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
-                                              double* __restrict__ red) {
+                                              uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces) {
   __shared__ __align__(16) unsigned char s_raw[LDS_BYTES];
   __shared__ int s_box[6 * (FB / 64)];
   const uint32_t tid = threadIdx.x;
@@ -125,8 +116,6 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   const int N = g.N;
   const size_t Nn = (size_t)N * N * N;
   const float H = g.H, invH = g.invH, dt = sp.dt;
-
-  if (MODE != 2 && blockIdx.x == 0 && tid < 8) red[tid] = 0.0;  // grid sums of this step's P2G
 
   const uint32_t mt = meta[ic];
   const uint32_t flags = mt >> 8;
@@ -313,26 +302,14 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   const float Qf = valid ? m * sp.mass_scale : 0.f;
   const long long Q = (long long)__float2ll_rn(Qf);
 
-  // Aligned lane groups of equal cell key (butterfly): U[l] holds when the
-  // group of 2^(l+1) lanes containing this lane shares one key.
+  // Runs of equal base cell in this wave (particles are cell-sorted, so the
+  // lanes of one run are contiguous); a run ends where the next lane's cell
+  // differs.
   const uint32_t key = valid ? ((uint32_t)b[2] * (uint32_t)N + (uint32_t)b[1]) * (uint32_t)N + (uint32_t)b[0]
                              : 0xffffffffu - lane;
-  bool U[6];
-  bool u = true;
-  int levels = 0;
-#pragma unroll
-  for (int l = 0; l < 6; ++l) {
-    const uint32_t kp = __shfl_xor(key, 1 << l, 64);
-    const int up = __shfl_xor((int)u, 1 << l, 64);
-    u = u && up && kp == key;
-    U[l] = u;
-    if (__any(u)) levels = l + 1;
-  }
-  int gsize = 1;
-#pragma unroll
-  for (int l = 0; l < 6; ++l)
-    if (U[l]) gsize = 2 << l;
-  const bool owner = valid && ((lane & (uint32_t)(gsize - 1)) == 0);
+  const uint32_t knext = __shfl_down(key, 1, 64);
+  const uint64_t ends = __ballot(lane == 63u || knext != key);
+  const int valid_i = valid ? 1 : 0;
 
   int lo[3], hi[3];
   block_box(b, valid, lo, hi, s_box);
@@ -349,58 +326,92 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
       s_p[2 * PATCH_ACC + p] = 0.f;
     }
   }
-  __syncthreads();
   const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
+  const int gbase = (int)key;  // linear index of the base node
+
+  // Transpose through LDS: each lane writes the contributions of its particle
+  // to the 9 nodes of one z-slab as a column (rows 0..17: fixed-point mass,
+  // low/high words; rows 18..44: momentum, node*3 + component). Lanes 0..35
+  // then sum one row each over the wave's cell runs and add every run's sum
+  // to the node box (one atomic per run, node and channel). The centre slab
+  // goes last: its centre node takes the remainder of the fixed-point mass.
+  float* s_tr = reinterpret_cast<float*>(s_raw + ACC_BYTES) + (tid >> 6) * (TR_ROWS * TR_STRIDE);
   long long qsum = 0;
 #pragma unroll
-  for (int o = 0; o < 27; ++o) {
-    const int ii = kOrder[o].i, j = kOrder[o].j, k = kOrder[o].k;
-    const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
-    long long qm;
-    if (o < 26) {
-      qm = (long long)__float2ll_rn(wt * Qf);
-      qsum += qm;
-    } else {
-      qm = Q - qsum;
-    }
-    float px = wt * (q0[0] + (float)ii * c0[0] + (float)j * c1[0] + (float)k * c2[0]);
-    float py = wt * (q0[1] + (float)ii * c0[1] + (float)j * c1[1] + (float)k * c2[1]);
-    float pz = wt * (q0[2] + (float)ii * c0[2] + (float)j * c1[2] + (float)k * c2[2]);
+  for (int si = 0; si < 3; ++si) {
+    const int k = si == 0 ? 0 : (si == 1 ? 2 : 1);
+    __syncthreads();  // box cleared / previous slab consumed
 #pragma unroll
-    for (int l = 0; l < 6; ++l) {
-      if (l < levels) {  // wave-uniform
-        const long long om = __shfl_xor(qm, 1 << l, 64);
-        const float ox = __shfl_xor(px, 1 << l, 64);
-        const float oy = __shfl_xor(py, 1 << l, 64);
-        const float oz = __shfl_xor(pz, 1 << l, 64);
-        if (U[l]) {
-          qm += om;
-          px += ox;
-          py += oy;
-          pz += oz;
-        }
-      }
-    }
-    if (owner) {
-      if (staged) {
-        const int ln = lbase + (k * ey + j) * ex + ii;
-        atomicAdd((unsigned long long*)&s_m[ln], (unsigned long long)qm);
-        atomicAdd(&s_p[ln], px);
-        atomicAdd(&s_p[PATCH_ACC + ln], py);
-        atomicAdd(&s_p[2 * PATCH_ACC + ln], pz);
+    for (int r = 0; r < 9; ++r) {
+      // In the centre slab the centre node (nn = 4) is visited last.
+      const int nn = k != 1 ? r : (r < 4 ? r : (r < 8 ? r + 1 : 4));
+      const int ii = nn % 3, j = nn / 3;
+      const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
+      long long qm;
+      if (k == 1 && nn == 4) {
+        qm = Q - qsum;
       } else {
-        const size_t node = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + (b[0] + ii);
-        atomicAdd(&acc_m[node], (unsigned long long)qm);
-        atomicAdd(&acc_p[node], px);
-        atomicAdd(&acc_p[Nn + node], py);
-        atomicAdd(&acc_p[2 * Nn + node], pz);
-        if (ii == 0) {
-          pieces[node >> 6] = 1;
-          pieces[(node + 2) >> 6] = 1;
+        qm = (long long)__float2ll_rn(wt * Qf);
+        qsum += qm;
+      }
+      s_tr[(2 * nn) * TR_STRIDE + lane] = __int_as_float((int)(uint32_t)(unsigned long long)qm);
+      s_tr[(2 * nn + 1) * TR_STRIDE + lane] = __int_as_float((int)(qm >> 32));
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        s_tr[(18 + 3 * nn + c) * TR_STRIDE + lane] =
+            wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
+      if (pinned && valid && wt > 1e-8f) fixed[(size_t)gbase + ((size_t)k * N + j) * N + ii] = 1;
+    }
+    __syncthreads();
+    if (lane < 36u) {
+      const bool is_mass = lane < 9u;
+      const int nn = is_mass ? (int)lane : ((int)lane - 9) / 3;
+      const int c = is_mass ? 0 : ((int)lane - 9) % 3;
+      const int ii = nn % 3, j = nn / 3;
+      const float* rowA = s_tr + (is_mass ? 2 * nn : 18 + 3 * nn + c) * TR_STRIDE;
+      const float* rowB = s_tr + (2 * nn + 1) * TR_STRIDE;
+      const int off_patch = (k * ey + j) * ex + ii;
+      const size_t off_glob = ((size_t)k * N + j) * N + ii;
+      long long am = 0;
+      float ap = 0.f;
+      for (int p4 = 0; p4 < 64; p4 += 4) {
+        const float4 a4 = *reinterpret_cast<const float4*>(rowA + p4);
+        float4 b4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (is_mass) b4 = *reinterpret_cast<const float4*>(rowB + p4);
+        const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+        const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int p = p4 + q;
+          if (is_mass)
+            am += (long long)(((unsigned long long)(uint32_t)__float_as_int(bv[q]) << 32) |
+                              (unsigned long long)(uint32_t)__float_as_int(av[q]));
+          else
+            ap += av[q];
+          if ((ends >> p) & 1ull) {  // wave-uniform: a run ends at lane p
+            if (__builtin_amdgcn_readlane(valid_i, p)) {
+              if (staged) {
+                const int ln = __builtin_amdgcn_readlane(lbase, p) + off_patch;
+                if (is_mass)
+                  atomicAdd((unsigned long long*)&s_m[ln], (unsigned long long)am);
+                else
+                  atomicAdd(&s_p[c * PATCH_ACC + ln], ap);
+              } else {
+                const size_t node = (size_t)(uint32_t)__builtin_amdgcn_readlane(gbase, p) + off_glob;
+                if (is_mass) {
+                  atomicAdd(&acc_m[node], (unsigned long long)am);
+                  pieces[node >> 6] = 1;
+                } else {
+                  atomicAdd(&acc_p[(size_t)c * Nn + node], ap);
+                }
+              }
+            }
+            am = 0;
+            ap = 0.f;
+          }
         }
       }
     }
-    if (pinned && valid && wt > 1e-8f) fixed[((size_t)(b[2] + k) * N + (b[1] + j)) * N + (b[0] + ii)] = 1;
   }
   if (staged) {
     __syncthreads();
@@ -429,13 +440,13 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
   const dim3 grid(fblocks(n)), block(FB);
   if (mode == 0)
     hipLaunchKernelGGL(k_fused<0>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
-                       d.acc_p, d.fixed, d.pieces, d.red);
+                       d.acc_p, d.fixed, d.pieces);
   else if (mode == 1)
     hipLaunchKernelGGL(k_fused<1>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
-                       d.acc_p, d.fixed, d.pieces, d.red);
+                       d.acc_p, d.fixed, d.pieces);
   else
     hipLaunchKernelGGL(k_fused<2>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
-                       d.acc_p, d.fixed, d.pieces, d.red);
+                       d.acc_p, d.fixed, d.pieces);
 }
 
 }  // namespace mpm

@@ -62,7 +62,8 @@ struct DevState {
   uint8_t* pieces;            // active 64-node pieces [ceil(N^3/64)]
   // sort scratch
   uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *partials;
-  // reductions: [0] grid mass (int64), [1..3] grid momentum (double)
+  // grid sums of the last grid update, one record per grid block:
+  // {fixed-point mass (int64 bits), momentum x, y, z}
   double* red;
   DevMaterial* materials;
 };
@@ -75,7 +76,7 @@ void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const Step
 // Fused particle kernel (mpm_fused.hip): mode 0 P2G from the stored state,
 // 1 G2P of this step + P2G of the next, 2 G2P writing the full state.
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode);
-void launch_clear_red(hipStream_t s, const DevState& d);
+uint32_t grid_blocks(const GridDims& g);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 // Particle-side FP64 sums: particle_reduce_blocks() records of
 // particle_reduce_k() doubles each (see k_particle_reduce).

@@ -77,22 +77,24 @@ __device__ __forceinline__ void step_stress(const M3& F, const DevMaterial& m, u
   }
 }
 
-// One wave per 64-node piece of the dense grid; inactive pieces exit at once.
+// One wave per 64-node piece of the dense grid; waves of inactive pieces do
+// no memory work. Each block writes one record of FP64 grid sums (fixed-point
+// mass, momentum) so the sums need no same-address atomics.
 __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
                                                 uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
                                                 float4* __restrict__ vel, GridDims g, StepParams sp,
                                                 int lo_idx0, int lo_idx1, int lo_idx2, int hi_idx0, int hi_idx1,
-                                                int hi_idx2, double* __restrict__ red) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const size_t piece = (size_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+                                                int hi_idx2, double* __restrict__ partials) {
+  __shared__ double s_red[BLOCK / 64][4];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const size_t piece = (size_t)blockIdx.x * (BLOCK / 64) + wave;
   const size_t Nn = (size_t)g.N * g.N * g.N;
   const size_t npieces = (Nn + 63) / 64;
-  if (piece >= npieces) return;
-  if (pieces[piece] == 0) return;
+  const bool active = piece < npieces && pieces[piece] != 0;
   const size_t node = piece * 64 + lane;
   long long Mq = 0;
   double p0 = 0, p1 = 0, p2 = 0;
-  if (node < Nn) {
+  if (active && node < Nn) {
     const unsigned long long M = acc_m[node];
     Mq = (long long)M;
     const float px = acc_p[node], py = acc_p[Nn + node], pz = acc_p[2 * Nn + node];
@@ -103,9 +105,10 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
     float v[3] = {px, py, pz};
     if (m > g.mass_eps) {
       const float im = 1.f / m;
-      v[0] = px * im + g.gravity[0] * sp.dt * (1 - g.gravity_on_particles);
-      v[1] = py * im + g.gravity[1] * sp.dt * (1 - g.gravity_on_particles);
-      v[2] = pz * im + g.gravity[2] * sp.dt * (1 - g.gravity_on_particles);
+      const float gd = g.gravity_on_particles ? 0.f : sp.dt;
+      v[0] = px * im + g.gravity[0] * gd;
+      v[1] = py * im + g.gravity[1] * gd;
+      v[2] = pz * im + g.gravity[2] * gd;
       const int N = g.N;
       const int ix = (int)(node % N), iy = (int)((node / N) % N), iz = (int)(node / ((size_t)N * N));
       const int idx[3] = {ix, iy, iz};
@@ -132,7 +135,6 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
     acc_p[2 * Nn + node] = 0.f;
     fixed[node] = 0;
   }
-  // FP64 conservation sums of the accumulated grid (wave reduction).
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
     Mq += __shfl_down(Mq, d, 64);
@@ -141,16 +143,23 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
     p2 += __shfl_down(p2, d, 64);
   }
   if (lane == 0) {
-    atomicAdd((unsigned long long*)&red[0], (unsigned long long)Mq);
-    atomicAdd(&red[1], p0);
-    atomicAdd(&red[2], p1);
-    atomicAdd(&red[3], p2);
-    pieces[piece] = 0;
+    s_red[wave][0] = __longlong_as_double(Mq);
+    s_red[wave][1] = p0;
+    s_red[wave][2] = p1;
+    s_red[wave][3] = p2;
+    if (active) pieces[piece] = 0;
   }
-}
-
-__global__ void k_clear_red(double* red) {
-  if (threadIdx.x < 8) red[threadIdx.x] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long m = 0;
+    double q[3] = {0, 0, 0};
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      m += __double_as_longlong(s_red[w][0]);
+      for (int c = 0; c < 3; ++c) q[c] += s_red[w][1 + c];
+    }
+    partials[blockIdx.x * 4 + 0] = __longlong_as_double(m);
+    for (int c = 0; c < 3; ++c) partials[blockIdx.x * 4 + 1 + c] = q[c];
+  }
 }
 
 __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
@@ -345,6 +354,10 @@ inline unsigned blocks(size_t n, int b = BLOCK) { return (unsigned)((n + b - 1) 
 }  // namespace
 
 int particle_reduce_k() { return RED_K; }
+uint32_t grid_blocks(const GridDims& g) {
+  const size_t Nn = (size_t)g.N * g.N * g.N;
+  return (uint32_t)(((Nn + 63) / 64 + BLOCK / 64 - 1) / (BLOCK / 64));
+}
 int particle_reduce_blocks() { return RED_BLOCKS; }
 
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g) {
@@ -364,13 +377,8 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
                  const int hi_idx[3]) {
   const size_t Nn = (size_t)g.N * g.N * g.N;
-  const size_t npieces = (Nn + 63) / 64;
-  hipLaunchKernelGGL(k_grid, dim3(blocks(npieces, BLOCK / 64)), dim3(BLOCK), 0, s, d.acc_m, d.acc_p, d.fixed,
-                     d.pieces, d.vel, g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1], hi_idx[2], d.red);
-}
-
-void launch_clear_red(hipStream_t s, const DevState& d) {
-  hipLaunchKernelGGL(k_clear_red, dim3(1), dim3(64), 0, s, d.red);
+  hipLaunchKernelGGL(k_grid, dim3(grid_blocks(g)), dim3(BLOCK), 0, s, d.acc_m, d.acc_p, d.fixed, d.pieces, d.vel,
+                     g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1], hi_idx[2], d.red);
 }
 
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {

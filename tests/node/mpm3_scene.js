@@ -15,7 +15,7 @@ for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature
 s.model = new Uint8Array(n); s.pinned = new Uint8Array(n);
 for (let i = 0; i < n; i++) {
   [s.x[i], s.y[i], s.z[i]] = input.points[i];
-  [s.vx[i], s.vy[i], s.vz[i]] = input.velocity[i] || input.velocity;
+  [s.vx[i], s.vy[i], s.vz[i]] = Array.isArray(input.velocity[0]) ? input.velocity[i] : input.velocity;
   s.mass[i] = 1.1; s.volume[i] = 1; s.temperature[i] = input.temperature || 20; s.model[i] = input.models[i];
 }
 const mpm = new MPM3(n, {sortEvery: 1});
