@@ -23,16 +23,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per particle per launch of each kernel (FP32 SoA
-# planes; per-particle terms from the step's byte budget: x 12, v 12, C 36,
-# F 36, m 4, V 4, ep 4+4, T 4, cached stress 24+24, material/flags 4) and per
-# touched grid node (P2G 16 written, grid 16 read + 16 written, G2P 16 read).
-PARTICLE_BYTES = {
-    "p2g": 12 + 12 + 36 + 36 + 4 + 4 + 4 + 24,        # x v C F m V meta tau
-    "g2p": 12 + 36 + 4 + 4 + 4 + 12 + 12 + 36 + 36 + 4 + 24,  # reads x F ep T meta; writes x v C F ep tau
-    "g2p2g": 12 + 36 + 4 + 4 + 4 + 4 + 4 + 12 + 36 + 4,  # reads x F m V ep T meta; writes x F ep
-}
-NODE_BYTES = {"p2g": 16, "grid": 32, "g2p": 16, "g2p2g": 32}
+# Algorithmic HBM bytes per launch, from the step's per-unit byte budget:
+# 212 B per thermal-J2 particle-step (read x 12, v 12, C 36, F 36, m 4, V 4,
+# ep 4, T 4; write x, v, C, F 96, ep 4) and 56 B per touched grid node per
+# step (P2G writes 16, the grid update reads 16 and writes 12, G2P reads 12).
+# A launch of the fused kernel performs the particle part of one whole step
+# (G2P of step s, P2G of step s+1) and the node part of P2G (16 B) and G2P
+# (12 B); the grid kernel performs the other 28 B per node.
+PARTICLE_STEP_BYTES = 212
+PARTICLE_BYTES = {"g2p2g": PARTICLE_STEP_BYTES}
+NODE_BYTES = {"g2p2g": 28, "grid": 28}
 
 
 def parse():

@@ -106,6 +106,10 @@ def lib():
         L.orc_min_j.restype = D
         L.orc_min_j.argtypes = [C.POINTER(OrcScene)]
         L.orc_mechanics.argtypes = [C.POINTER(OrcScene), PD]
+        L.orc_mlp_param_count.restype = C.c_int
+        L.orc_mlp_forward.argtypes = [PD, C.c_int, PD, PD]
+        L.orc_mlp_loss_grad.restype = D
+        L.orc_mlp_loss_grad.argtypes = [PD, C.c_int, PD, PD, PD]
         _lib = L
     return _lib
 
@@ -357,3 +361,23 @@ class Scene:
 
     def velocities(self):
         return np.stack([self.vx, self.vy, self.vz], 1)
+
+
+# ---------------------------------------------------------------- surrogate
+def mlp_param_count():
+    return lib().orc_mlp_param_count()
+
+
+def mlp_forward(params, x):
+    params, x = _f64(params), _f64(x)
+    n = len(x)
+    y = np.zeros((n, 6))
+    lib().orc_mlp_forward(_p(params), n, _p(x), _p(y))
+    return y
+
+
+def mlp_loss_grad(params, x, t):
+    params, x, t = _f64(params), _f64(x), _f64(t)
+    g = np.zeros(len(params))
+    loss = lib().orc_mlp_loss_grad(_p(params), len(x), _p(x), _p(t), _p(g))
+    return loss, g

@@ -817,3 +817,100 @@ void orc_mechanics(const OrcScene* s, double* out) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- learned surrogate
+// FP64 reference of the constitutive surrogate network (no counterpart in the
+// reference): 7 -> 64 -> 64 -> 64 -> 6, SiLU between layers. Parameters are
+// packed W1[64x7], b1[64], W2[64x64], b2[64], W3[64x64], b3[64], W4[6x64],
+// b4[6] (weights row-major, output-major), 9222 values.
+namespace {
+constexpr int kIn = 7, kHid = 64, kOut = 6;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kIn, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
+              kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid * kHid, kOffW4 = kOffB3 + kHid,
+              kOffB4 = kOffW4 + kOut * kHid, kParams = kOffB4 + kOut;
+inline double silu(double z) { return z / (1 + std::exp(-z)); }
+inline double dsilu(double z) {
+  const double s = 1 / (1 + std::exp(-z));
+  return s * (1 + z * (1 - s));
+}
+void layer(const double* W, const double* b, int out, int in, const double* x, double* z) {
+  for (int o = 0; o < out; ++o) {
+    double acc = b[o];
+    for (int i = 0; i < in; ++i) acc += W[o * in + i] * x[i];
+    z[o] = acc;
+  }
+}
+}  // namespace
+
+extern "C" {
+int orc_mlp_param_count(void) { return kParams; }
+
+void orc_mlp_forward(const double* P, int n, const double* x, double* y) {
+  double z1[kHid], h1[kHid], z2[kHid], h2[kHid], z3[kHid], h3[kHid];
+  for (int s = 0; s < n; ++s) {
+    layer(P + kOffW1, P + kOffB1, kHid, kIn, x + s * kIn, z1);
+    for (int k = 0; k < kHid; ++k) h1[k] = silu(z1[k]);
+    layer(P + kOffW2, P + kOffB2, kHid, kHid, h1, z2);
+    for (int k = 0; k < kHid; ++k) h2[k] = silu(z2[k]);
+    layer(P + kOffW3, P + kOffB3, kHid, kHid, h2, z3);
+    for (int k = 0; k < kHid; ++k) h3[k] = silu(z3[k]);
+    layer(P + kOffW4, P + kOffB4, kOut, kHid, h3, y + s * kOut);
+  }
+}
+
+// Mean squared error over n samples (sum over the 6 outputs, mean over
+// samples) and its gradient with respect to the packed parameters.
+double orc_mlp_loss_grad(const double* P, int n, const double* x, const double* t, double* grad) {
+  for (int k = 0; k < kParams; ++k) grad[k] = 0;
+  double loss = 0;
+  double z1[kHid], h1[kHid], z2[kHid], h2[kHid], z3[kHid], h3[kHid], y[kOut];
+  double dy[kOut], d3[kHid], d2[kHid], d1[kHid];
+  for (int s = 0; s < n; ++s) {
+    const double* xs = x + s * kIn;
+    layer(P + kOffW1, P + kOffB1, kHid, kIn, xs, z1);
+    for (int k = 0; k < kHid; ++k) h1[k] = silu(z1[k]);
+    layer(P + kOffW2, P + kOffB2, kHid, kHid, h1, z2);
+    for (int k = 0; k < kHid; ++k) h2[k] = silu(z2[k]);
+    layer(P + kOffW3, P + kOffB3, kHid, kHid, h2, z3);
+    for (int k = 0; k < kHid; ++k) h3[k] = silu(z3[k]);
+    layer(P + kOffW4, P + kOffB4, kOut, kHid, h3, y);
+    for (int o = 0; o < kOut; ++o) {
+      const double e = y[o] - t[s * kOut + o];
+      loss += e * e;
+      dy[o] = 2 * e / n;
+    }
+    for (int o = 0; o < kOut; ++o) {
+      grad[kOffB4 + o] += dy[o];
+      for (int i = 0; i < kHid; ++i) grad[kOffW4 + o * kHid + i] += dy[o] * h3[i];
+    }
+    for (int i = 0; i < kHid; ++i) {
+      double acc = 0;
+      for (int o = 0; o < kOut; ++o) acc += P[kOffW4 + o * kHid + i] * dy[o];
+      d3[i] = acc * dsilu(z3[i]);
+    }
+    for (int o = 0; o < kHid; ++o) {
+      grad[kOffB3 + o] += d3[o];
+      for (int i = 0; i < kHid; ++i) grad[kOffW3 + o * kHid + i] += d3[o] * h2[i];
+    }
+    for (int i = 0; i < kHid; ++i) {
+      double acc = 0;
+      for (int o = 0; o < kHid; ++o) acc += P[kOffW3 + o * kHid + i] * d3[o];
+      d2[i] = acc * dsilu(z2[i]);
+    }
+    for (int o = 0; o < kHid; ++o) {
+      grad[kOffB2 + o] += d2[o];
+      for (int i = 0; i < kHid; ++i) grad[kOffW2 + o * kHid + i] += d2[o] * h1[i];
+    }
+    for (int i = 0; i < kHid; ++i) {
+      double acc = 0;
+      for (int o = 0; o < kHid; ++o) acc += P[kOffW2 + o * kHid + i] * d2[o];
+      d1[i] = acc * dsilu(z1[i]);
+    }
+    for (int o = 0; o < kHid; ++o) {
+      grad[kOffB1 + o] += d1[o];
+      for (int i = 0; i < kIn; ++i) grad[kOffW1 + o * kIn + i] += d1[o] * xs[i];
+    }
+  }
+  return loss / n;
+}
+}  // extern "C"

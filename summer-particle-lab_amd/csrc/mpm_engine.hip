@@ -392,13 +392,11 @@ int mpm_upload(mpm_engine* e, const mpm_particles* p) {
   const uint32_t cap = e->d.cap;
   std::vector<float> planes((size_t)NPLANES * cap, 0.f);
   std::vector<uint32_t> ids(n), meta(n);
-  double total = 0;
   for (uint32_t i = 0; i < n; ++i) {
     if (p->material[i] >= e->materials.size())
       return fail(e, MPM_ERR_INVALID_ARGUMENT, "particle %u: material %u out of range", i, p->material[i]);
     const double m = rdv(p->mass, dt, i), vol = rdv(p->volume, dt, i);
     if (!(m > 0) || !(vol > 0)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "particle %u: mass and volume must be > 0", i);
-    total += (double)(float)m;
     auto set = [&](int k, double v) { planes[(size_t)k * cap + i] = (float)v; };
     set(P_X, rdv(p->x, dt, i));
     set(P_Y, rdv(p->y, dt, i));
@@ -428,15 +426,18 @@ int mpm_upload(mpm_engine* e, const mpm_particles* p) {
     ids[i] = i;
     meta[i] = (uint32_t)p->material[i] | (flags << 8);
   }
-  // Fixed-point mass unit 2^k with the total mass at most 2^61, so every node
-  // sum and the grand total fit a signed 64-bit integer.
+  // Fixed-point mass unit 2^-k with the heaviest particle at most 2^24 units:
+  // a particle's 27 integer parts, and the sums of up to 64 of them inside a
+  // wave, fit 32 bits; node and grand totals are 64-bit.
+  double mmax = 0;
+  for (uint32_t i = 0; i < n; ++i) mmax = std::max(mmax, (double)planes[(size_t)P_MASS * cap + i]);
   int k = 0;
-  if (total > 0) k = (int)std::floor(61.0 - std::log2(total));
+  if (mmax > 0) k = (int)std::floor(24.0 - std::log2(mmax));
   k = std::max(-100, std::min(100, k));
   e->mass_scale = std::ldexp(1.0, k);
   int64_t q = 0;
   const float sf = (float)e->mass_scale;
-  for (uint32_t i = 0; i < n; ++i) q += (int64_t)std::llrint((double)(planes[(size_t)P_MASS * cap + i] * sf));
+  for (uint32_t i = 0; i < n; ++i) q += (int64_t)std::nearbyint(planes[(size_t)P_MASS * cap + i] * sf);
   e->particle_mass_q = q;
   e->n = n;
   HIP_OK(e, hipMemcpyAsync(e->d.planes, planes.data(), sizeof(float) * planes.size(), hipMemcpyHostToDevice,

@@ -30,12 +30,14 @@ namespace mpm {
 namespace {
 
 constexpr int FB = 256;            // threads per workgroup = particles per chunk
-constexpr int PATCH_ACC = 1024;    // LDS node box for the scatter (20 B per node)
-constexpr int PATCH_VEL = 1280;    // LDS node box for the gather (16 B per node)
+constexpr int PATCH_ACC = 512;     // LDS node box for the scatter (20 B per node)
+constexpr int PATCH_VEL = 2048;    // LDS node box for the gather (16 B per node)
 constexpr int ACC_BYTES = 20 * PATCH_ACC;
-constexpr int TR_ROWS = 45;        // per-wave transpose: 18 mass words + 27 momentum rows
-constexpr int TR_STRIDE = 68;      // 64 lanes + 4 words of padding (16-byte rows, spread banks)
-constexpr int LDS_BYTES = ACC_BYTES + (FB / 64) * TR_ROWS * TR_STRIDE * 4;
+constexpr int TR_ROWS = 36;        // per-wave transpose: 9 node masses + 27 momentum rows
+constexpr int TR_STRIDE = 65;      // 64 lanes + 1 word: rows start in different banks
+constexpr int CHUNK = 8;           // lanes per scatter chunk (run pieces summed by one lane)
+constexpr int WAVE_BYTES = TR_ROWS * TR_STRIDE * 4 + 64 * 16;
+constexpr int LDS_BYTES = ACC_BYTES + (FB / 64) * WAVE_BYTES;
 static_assert(LDS_BYTES >= 16 * PATCH_VEL, "gather box must fit the shared buffer");
 
 __device__ __forceinline__ float ld(const float* P, uint32_t cap, int k, uint32_t i) {
@@ -299,28 +301,32 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int k = 0; k < 3; ++k) wyz[j][k] = w[1][j] * w[2][k];
-  const float Qf = valid ? m * sp.mass_scale : 0.f;
-  const long long Q = (long long)__float2ll_rn(Qf);
+  // Particle mass in fixed-point units (at most 2^24, so every part and every
+  // sum of up to 64 parts fits 32 bits).
+  const float Qf = valid ? rintf(m * sp.mass_scale) : 0.f;
+  const int Q = (int)Qf;
 
   // Runs of equal base cell in this wave (particles are cell-sorted, so the
-  // lanes of one run are contiguous); a run ends where the next lane's cell
-  // differs.
+  // lanes of one run are contiguous), cut into chunks of at most CHUNK lanes.
   const uint32_t key = valid ? ((uint32_t)b[2] * (uint32_t)N + (uint32_t)b[1]) * (uint32_t)N + (uint32_t)b[0]
                              : 0xffffffffu - lane;
-  const uint32_t knext = __shfl_down(key, 1, 64);
-  const uint64_t ends = __ballot(lane == 63u || knext != key);
-  const int valid_i = valid ? 1 : 0;
+  const uint32_t kprev = __shfl_up(key, 1, 64);
+  const uint64_t run_starts = __ballot(lane == 0u || kprev != key);
+  const uint64_t upto = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
+  const int run_start = 63 - __clzll(run_starts & upto);
+  const uint64_t chunk_starts = __ballot(((int)lane - run_start) % CHUNK == 0);
+  const int nchunks = __popcll(chunk_starts);
 
   int lo[3], hi[3];
   block_box(b, valid, lo, hi, s_box);
   const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
   const int cnt = ex * ey * ez;
   const bool staged = hi[0] >= 0 && cnt <= PATCH_ACC;
-  long long* s_m = reinterpret_cast<long long*>(s_raw);
+  unsigned long long* s_m = reinterpret_cast<unsigned long long*>(s_raw);
   float* s_p = reinterpret_cast<float*>(s_raw + 8 * PATCH_ACC);
   if (staged) {
     for (int p = tid; p < cnt; p += FB) {
-      s_m[p] = 0;
+      s_m[p] = 0ull;
       s_p[p] = 0.f;
       s_p[PATCH_ACC + p] = 0.f;
       s_p[2 * PATCH_ACC + p] = 0.f;
@@ -329,99 +335,101 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
   const int gbase = (int)key;  // linear index of the base node
 
-  // Transpose through LDS: each lane writes the contributions of its particle
-  // to the 9 nodes of one z-slab as a column (rows 0..17: fixed-point mass,
-  // low/high words; rows 18..44: momentum, node*3 + component). Lanes 0..35
-  // then sum one row each over the wave's cell runs and add every run's sum
-  // to the node box (one atomic per run, node and channel). The centre slab
-  // goes last: its centre node takes the remainder of the fixed-point mass.
-  float* s_tr = reinterpret_cast<float*>(s_raw + ACC_BYTES) + (tid >> 6) * (TR_ROWS * TR_STRIDE);
-  long long qsum = 0;
+  // Per-wave LDS: 36 transpose rows (9 node masses, 27 node momenta) of 64
+  // particles, and the chunk table {first lane, last lane, box base, grid
+  // base (-1 when the chunk holds no particle)}.
+  unsigned char* wbuf = s_raw + ACC_BYTES + (tid >> 6) * WAVE_BYTES;
+  float* s_tr = reinterpret_cast<float*>(wbuf);
+  int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ROWS * TR_STRIDE * 4);
+  if ((chunk_starts >> lane) & 1ull) {
+    const int c = __popcll(chunk_starts & (upto >> 1));  // chunks before this lane
+    const uint64_t later = chunk_starts & ~upto;
+    const int next = later ? __ffsll((long long)later) - 1 : 64;
+    s_chunk[c] = make_int4((int)lane, next - 1, lbase, valid ? gbase : -1);
+  }
+
+  // Transpose through LDS: each lane writes its particle's contributions to
+  // the 9 nodes of one z-slab as a column; then every lane sums whole
+  // (row, chunk) tasks and adds each chunk sum to the node box with one LDS
+  // atomic. The centre slab goes last: its centre node takes the remainder
+  // of the fixed-point mass so the 27 parts sum exactly to Q.
+  int qsum = 0;
 #pragma unroll
   for (int si = 0; si < 3; ++si) {
     const int k = si == 0 ? 0 : (si == 1 ? 2 : 1);
-    __syncthreads();  // box cleared / previous slab consumed
+    if (si == 0) __syncthreads();  // node box cleared before any atomic
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
       // In the centre slab the centre node (nn = 4) is visited last.
       const int nn = k != 1 ? r : (r < 4 ? r : (r < 8 ? r + 1 : 4));
       const int ii = nn % 3, j = nn / 3;
       const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
-      long long qm;
+      int qm;
       if (k == 1 && nn == 4) {
         qm = Q - qsum;
       } else {
-        qm = (long long)__float2ll_rn(wt * Qf);
+        qm = __float2int_rn(wt * Qf);
         qsum += qm;
       }
-      s_tr[(2 * nn) * TR_STRIDE + lane] = __int_as_float((int)(uint32_t)(unsigned long long)qm);
-      s_tr[(2 * nn + 1) * TR_STRIDE + lane] = __int_as_float((int)(qm >> 32));
+      s_tr[nn * TR_STRIDE + lane] = __int_as_float(qm);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        s_tr[(18 + 3 * nn + c) * TR_STRIDE + lane] =
+        s_tr[(9 + 3 * nn + c) * TR_STRIDE + lane] =
             wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
       if (pinned && valid && wt > 1e-8f) fixed[(size_t)gbase + ((size_t)k * N + j) * N + ii] = 1;
     }
-    __syncthreads();
-    if (lane < 36u) {
-      const bool is_mass = lane < 9u;
-      const int nn = is_mass ? (int)lane : ((int)lane - 9) / 3;
-      const int c = is_mass ? 0 : ((int)lane - 9) % 3;
-      const int ii = nn % 3, j = nn / 3;
-      const float* rowA = s_tr + (is_mass ? 2 * nn : 18 + 3 * nn + c) * TR_STRIDE;
-      const float* rowB = s_tr + (2 * nn + 1) * TR_STRIDE;
-      const int off_patch = (k * ey + j) * ex + ii;
-      const size_t off_glob = ((size_t)k * N + j) * N + ii;
-      long long am = 0;
-      float ap = 0.f;
-      for (int p4 = 0; p4 < 64; p4 += 4) {
-        const float4 a4 = *reinterpret_cast<const float4*>(rowA + p4);
-        float4 b4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (is_mass) b4 = *reinterpret_cast<const float4*>(rowB + p4);
-        const float av[4] = {a4.x, a4.y, a4.z, a4.w};
-        const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int t = (int)lane; t < TR_ROWS * nchunks; t += 64) {
+      const int row = t % TR_ROWS;
+      const int4 ch = s_chunk[t / TR_ROWS];
+      const float* src = s_tr + row * TR_STRIDE;
+      int si32 = 0;
+      float sf = 0.f;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int p = p4 + q;
-          if (is_mass)
-            am += (long long)(((unsigned long long)(uint32_t)__float_as_int(bv[q]) << 32) |
-                              (unsigned long long)(uint32_t)__float_as_int(av[q]));
+      for (int q = 0; q < CHUNK; ++q) {
+        const int p = ch.x + q;
+        if (p <= ch.y) {
+          const float vq = src[p];
+          si32 += __float_as_int(vq);
+          sf += vq;
+        }
+      }
+      if (ch.w >= 0) {
+        const int nn = row < 9 ? row : (row - 9) / 3;
+        const int c = row < 9 ? 0 : (row - 9) % 3;
+        const int ii = nn % 3, j = nn / 3;
+        if (staged) {
+          const int ln = ch.z + (k * ey + j) * ex + ii;
+          if (row < 9)
+            atomicAdd(&s_m[ln], (unsigned long long)(unsigned int)si32);
           else
-            ap += av[q];
-          if ((ends >> p) & 1ull) {  // wave-uniform: a run ends at lane p
-            if (__builtin_amdgcn_readlane(valid_i, p)) {
-              if (staged) {
-                const int ln = __builtin_amdgcn_readlane(lbase, p) + off_patch;
-                if (is_mass)
-                  atomicAdd((unsigned long long*)&s_m[ln], (unsigned long long)am);
-                else
-                  atomicAdd(&s_p[c * PATCH_ACC + ln], ap);
-              } else {
-                const size_t node = (size_t)(uint32_t)__builtin_amdgcn_readlane(gbase, p) + off_glob;
-                if (is_mass) {
-                  atomicAdd(&acc_m[node], (unsigned long long)am);
-                  pieces[node >> 6] = 1;
-                } else {
-                  atomicAdd(&acc_p[(size_t)c * Nn + node], ap);
-                }
-              }
-            }
-            am = 0;
-            ap = 0.f;
+            atomicAdd(&s_p[c * PATCH_ACC + ln], sf);
+        } else {
+          const size_t node = (size_t)(uint32_t)ch.w + ((size_t)k * N + j) * N + ii;
+          if (row < 9) {
+            atomicAdd(&acc_m[node], (unsigned long long)(unsigned int)si32);
+            pieces[node >> 6] = 1;
+          } else {
+            atomicAdd(&acc_p[(size_t)c * Nn + node], sf);
           }
         }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   if (staged) {
     __syncthreads();
     for (int p = tid; p < cnt; p += FB) {
-      const long long mq = s_m[p];
+      const unsigned long long mq = s_m[p];
       const float px = s_p[p], py = s_p[PATCH_ACC + p], pz = s_p[2 * PATCH_ACC + p];
-      if (mq != 0 || px != 0.f || py != 0.f || pz != 0.f) {
+      if (mq != 0ull || px != 0.f || py != 0.f || pz != 0.f) {
         const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
         const size_t node = ((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx);
-        atomicAdd(&acc_m[node], (unsigned long long)mq);
+        atomicAdd(&acc_m[node], mq);
         atomicAdd(&acc_p[node], px);
         atomicAdd(&acc_p[Nn + node], py);
         atomicAdd(&acc_p[2 * Nn + node], pz);

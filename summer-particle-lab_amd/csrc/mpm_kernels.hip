@@ -215,7 +215,7 @@ __global__ __launch_bounds__(BLOCK) void k_particle_reduce(const float* __restri
       F[k] = plane(P, cap, P_F0 + k, i);
     }
     acc[0] += m;
-    acc[1] += (double)__float2ll_rn((float)m * mass_scale);
+    acc[1] += (double)rintf((float)m * mass_scale);
     const double P3[3] = {m * v[0], m * v[1], m * v[2]};
     acc[2] += P3[0];
     acc[3] += P3[1];
