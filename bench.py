@@ -10,6 +10,12 @@ Extra fields:
                 on the engine's stream (profiled pass after the timed one).
   cpu_baseline  the FP64 CPU restatement of the reference step (oracle),
                 single thread, on a bounded sample of the same workload.
+  surrogate     the same workload with the learned constitutive surrogate:
+                network trained on the device first (timed separately), then
+                K steps timed as above; its stress error on held-out trial
+                states against the engine's plain FP32 return map, and the
+                share of particle-steps per branch (network, closed-form
+                elastic, plain return outside the envelope).
 """
 import argparse
 import json
@@ -40,10 +46,99 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--sort-every", type=int, default=int(os.environ.get("MPM_SORT_EVERY", "8")))
+    p.add_argument("--sort-every", type=int, default=int(os.environ.get("MPM_SORT_EVERY", "16")))
     p.add_argument("--cpu-steps", type=int, default=8)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-surrogate", action="store_true")
     return p.parse_args()
+
+
+def timed_steps(eng, sc, steps, warmup):
+    import torch
+    eng.step(sc.dt, warmup)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(sc.dt, steps)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(eng, sc, steps):
+    """Per-kernel (launches, total ms) from HIP events on the engine stream."""
+    eng.profile(True, True)
+    eng.step(sc.dt, steps)
+    eng.synchronize()
+    times = eng.kernel_times()
+    eng.profile(False, True)
+    return times
+
+
+def held_out_states(n, sigma, T_lo, T_hi, seed=2024):
+    """Trial states F = exp(eps) R: eps components N(0, sigma), R a random
+    rotation, T uniform."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    e = rng.normal(0, sigma, (n, 6))
+    E = np.stack([e[:, 0], e[:, 3], e[:, 4], e[:, 3], e[:, 1], e[:, 5], e[:, 4], e[:, 5], e[:, 2]], 1)
+    lam, V = np.linalg.eigh(E.reshape(n, 3, 3))
+    S = np.einsum("nij,nj,nkj->nik", V, np.exp(lam), V)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1).reshape(n, 3, 3)
+    return (S @ R).reshape(n, 9), rng.uniform(T_lo, T_hi, n)
+
+
+def surrogate_leg(sc, args, plain_value):
+    """Train the surrogate for the workload's material, check its stress error
+    against the plain FP32 return, and time the same steps with it."""
+    import numpy as np
+    from mpm_amd import surrogate as S
+    from mpm_amd.engine import return_map_batch
+    mat = sc.materials[0]
+    # Strain scale of the training distribution: the material's yield strain
+    # scale A (sigma_y / E), so that the yield surface sits inside the
+    # sampled range as it does for A = 0.02 with N(0, 0.02).
+    cfg = S.TrainConfig(strain_sigma=mat.thermal["A"], dt=sc.dt, T_lo=300.0, T_hi=900.0)
+    t = time.perf_counter()
+    P, norm, hist = S.train(mat, cfg)
+    train_s = time.perf_counter() - t
+    F, T = held_out_states(20000, cfg.strain_sigma, cfg.T_lo, cfg.T_hi)
+    _, _, tau, br = S.return_batch(mat, P, norm, F, 0.0, T, sc.dt)
+    _, _, tau_p = return_map_batch(mat, F, 0.0, T, sc.dt)
+    inside = br != 2
+    w = np.array([1, 1, 1, 2, 2, 2], np.float64)  # symmetric 6-vectors: off-diagonals count twice
+    d = (tau[inside].astype(np.float64) - tau_p[inside]) ** 2 @ w
+    err = float(np.sqrt(d.sum() / ((tau_p[inside].astype(np.float64) ** 2) @ w).sum()))
+    net = br == 0
+    err_net = float(np.sqrt((((tau[net].astype(np.float64) - tau_p[net]) ** 2) @ w).sum()
+                            / (((tau_p[net].astype(np.float64)) ** 2) @ w).sum())) if net.any() else None
+
+    eng = sc.engine(sort_every=args.sort_every)
+    eng.set_surrogate(0, P, norm)
+    eng.step(sc.dt, args.warmup)
+    eng.surrogate_counts(reset=True)
+    elapsed = timed_steps(eng, sc, args.steps, 0)
+    counts = eng.surrogate_counts(reset=True)
+    cons = eng.conservation()
+    assert cons["grid_mass_q"] == cons["particle_mass_q"] and cons["nonfinite"] == 0, cons
+    times = kernel_profile(eng, sc, min(args.steps, 100))
+    value = sc.n * args.steps / elapsed
+    tot = max(sum(counts.values()), 1)
+    return dict(value=round(value, 1), ms_per_step=round(elapsed / args.steps * 1e3, 4),
+                speedup_vs_plain=round(value / plain_value, 3),
+                branches={k: round(v / tot, 4) for k, v in counts.items()},
+                stress_rel_l2_vs_plain_fp32=round(err, 5), stress_rel_l2_network_states=(
+                    None if err_net is None else round(err_net, 5)),
+                held_out=dict(states=len(F), network=int(net.sum()), elastic=int((br == 1).sum()),
+                              fallback=int((br == 2).sum())),
+                training=dict(samples=cfg.samples, epochs=cfg.epochs, batch=cfg.batch, strain_sigma=cfg.strain_sigma,
+                              seconds=round(train_s, 2), loss_first=float(hist[0]), loss_last=float(hist[-1])),
+                kernels={k: dict(launches=c, avg_us=round(ms / c * 1e3, 2)) for k, (c, ms) in times.items()})
 
 
 def cpu_baseline(sc, steps):
@@ -77,26 +172,14 @@ def main():
     torch.cuda.set_device(0)
     sc = scenes.thermal_boxes()
     eng = sc.engine(sort_every=args.sort_every)
-    eng.step(sc.dt, args.warmup)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.step(sc.dt, args.steps)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed = timed_steps(eng, sc, args.steps, args.warmup)
     value = sc.n * args.steps / elapsed
     cons = eng.conservation()
     assert cons["grid_mass_q"] == cons["particle_mass_q"] and cons["nonfinite"] == 0, cons
 
     # Profiled pass: per-kernel average duration from HIP events on the
     # engine stream.
-    prof_steps = min(args.steps, 100)
-    eng.profile(True, True)
-    eng.step(sc.dt, prof_steps)
-    eng.synchronize()
-    times = eng.kernel_times()
-    eng.profile(False, True)
+    times = kernel_profile(eng, sc, min(args.steps, 100))
     # Touched nodes: nodes within the 3x3x3 stencils of the particles.
     x = eng.download()["positions"]
     origin, nodes = scenes.domain(sc.lower, sc.upper, sc.h)
@@ -124,6 +207,8 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
                     algorithmic_bytes_per_launch=d["bytes"], avg_launch_us=d["avg_us"])
+    del eng
+    sur = None if args.no_surrogate else surrogate_leg(sc, args, value)
     cpu = None if args.no_cpu else cpu_baseline(sc, args.cpu_steps)
     out = {
         "metric": "particle-steps/s (P2G+grid+G2P), plain vs MLP-constitutive at <=1% stress err",
@@ -144,6 +229,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": per_kernel,
+        "surrogate": sur,
     }
     print(json.dumps(out))
 

@@ -227,6 +227,85 @@ int mpm_return_map_batch(const mpm_material* m, uint32_t n, float dt, const floa
  * decomposition for plastic models. tau_out [6n]. */
 int mpm_stress_batch(const mpm_material* m, uint32_t n, const float* F, float* tau_out);
 
+/* ---- learned constitutive surrogate (no reference counterpart) ----
+ *
+ * A 7 -> 64 -> 64 -> 64 -> 6 network with hard-swish z min(max(z/6 + 1/2, 0), 1)
+ * between layers (MPM_MLP_NPARAM FP32 parameters,
+ * packed W1[64x7] b1 W2[64x64] b2 W3[64x64] b3 W4[6x64] b4, weights
+ * output-major) maps the trial elastic logarithmic strain eps (xx, yy, zz,
+ * xy, xz, yz) and the temperature T to the deviatoric part of the
+ * return-mapped Kirchhoff stress. Inputs are eps * in_scale and
+ * (T - T_center) * T_scale; outputs are in units of out_scale. It replaces
+ * the plastic return of projectPlastic (src/three/constitutive.js:52-78) for
+ * rate-independent J2 models without hardening (model 4; model 10 with
+ * H = 0), where the return depends on eps and T only. */
+#define MPM_MLP_NPARAM 9222
+
+typedef struct {
+  uint32_t samples;    /* training samples drawn per run */
+  uint32_t batch;      /* minibatch size */
+  uint32_t epochs;
+  uint32_t seed;
+  float lr, lr_final;  /* Adam step, cosine-annealed from lr to lr_final */
+  float strain_sigma;  /* eps components ~ N(0, strain_sigma) */
+  float T_lo, T_hi;    /* T ~ U(T_lo, T_hi) */
+  float dt;            /* step passed to the return map */
+  float eps_max;       /* envelope half-width of every eps component; <= 0: 4 sigma */
+} mpm_mlp_train_config;
+
+typedef struct {
+  float in_scale, T_center, T_scale, out_scale;
+  float eps_max;       /* envelope: |eps_ij| <= eps_max */
+  float T_lo, T_hi;    /* envelope: T_lo <= T <= T_hi */
+  int32_t material;    /* set by mpm_set_surrogate */
+} mpm_mlp_norm;
+
+/* Normalisation for material m and a training config: in_scale 1/sigma,
+ * T mapped to [-1, 1], out_scale = A E (the cold yield stress). */
+int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, mpm_mlp_norm* out);
+
+/* n training samples generated on the device by the plain return map
+ * (mpm_return_map_batch's update with F = exp(eps), ep = 0): X [7n] inputs,
+ * Y [6n] targets, sample-major. */
+int mpm_mlp_samples(const mpm_material* m, const mpm_mlp_train_config* c, const mpm_mlp_norm* norm, uint32_t n,
+                    uint32_t seed, float* X, float* Y);
+
+/* Train the network on the device (FP32 forward/backward kernels, Adam) on
+ * c->samples samples from mpm_mlp_samples' generator. params receives the
+ * MPM_MLP_NPARAM trained parameters; loss_history (may be NULL) the mean
+ * squared error of every epoch [c->epochs]. */
+int mpm_mlp_train(const mpm_material* m, const mpm_mlp_train_config* c, const mpm_mlp_norm* norm, float* params,
+                  float* loss_history);
+
+/* Mean squared error (summed over the 6 outputs, averaged over the n
+ * samples) of the network on X [7n], Y [6n] and its gradient with respect to
+ * every parameter, by the training kernels. */
+int mpm_mlp_loss_grad(const float* params, uint32_t n, const float* X, const float* Y, float* grad, float* loss);
+
+/* Network outputs for X [7n] by the BF16 MFMA inference of the step kernel
+ * (BF16 weights and activations, FP32 accumulation). Y is [6n]. */
+int mpm_mlp_infer(const float* params, uint32_t n, const float* X, float* Y);
+
+/* The step's surrogate return on a batch of n trial states F [9n] with ep
+ * [n] and T [n] of material m: closed-form elastic branch inside the yield
+ * surface, network inside the envelope, plain return (as
+ * mpm_return_map_batch) otherwise. Outputs as mpm_return_map_batch, plus
+ * branch [n]: 0 network, 1 elastic, 2 plain return. */
+int mpm_surrogate_return_batch(const mpm_material* m, const float* params, const mpm_mlp_norm* norm, uint32_t n,
+                               float dt, const float* F, const float* ep, const float* T, float* F_out, float* ep_out,
+                               float* tau_out, int32_t* branch);
+
+/* Use the network for particles of material index `material` whose trial
+ * strain and temperature lie inside norm's envelope; the others keep the
+ * plain return. params == NULL switches the surrogate off, and so does
+ * mpm_set_materials. */
+int mpm_set_surrogate(mpm_engine* e, int32_t material, const float* params, const mpm_mlp_norm* norm);
+
+/* Particle-steps of the surrogate's material since the last reset that took
+ * the network (used), the closed-form elastic branch (elastic) and the plain
+ * return outside the envelope (fallback). reset != 0 clears the counters. */
+int mpm_surrogate_counts(mpm_engine* e, uint64_t* used, uint64_t* elastic, uint64_t* fallback, int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -820,7 +820,8 @@ void orc_mechanics(const OrcScene* s, double* out) {
 
 // ------------------------------------------------------- learned surrogate
 // FP64 reference of the constitutive surrogate network (no counterpart in the
-// reference): 7 -> 64 -> 64 -> 64 -> 6, SiLU between layers. Parameters are
+// reference): 7 -> 64 -> 64 -> 64 -> 6, hard-swish z min(max(z/6 + 1/2, 0), 1)
+// between layers. Parameters are
 // packed W1[64x7], b1[64], W2[64x64], b2[64], W3[64x64], b3[64], W4[6x64],
 // b4[6] (weights row-major, output-major), 9222 values.
 namespace {
@@ -828,11 +829,8 @@ constexpr int kIn = 7, kHid = 64, kOut = 6;
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kIn, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
               kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid * kHid, kOffW4 = kOffB3 + kHid,
               kOffB4 = kOffW4 + kOut * kHid, kParams = kOffB4 + kOut;
-inline double silu(double z) { return z / (1 + std::exp(-z)); }
-inline double dsilu(double z) {
-  const double s = 1 / (1 + std::exp(-z));
-  return s * (1 + z * (1 - s));
-}
+inline double act(double z) { return z <= -3 ? 0 : (z >= 3 ? z : z * (z + 3) / 6); }
+inline double dact(double z) { return z < -3 ? 0 : (z > 3 ? 1 : z / 3 + 0.5); }
 void layer(const double* W, const double* b, int out, int in, const double* x, double* z) {
   for (int o = 0; o < out; ++o) {
     double acc = b[o];
@@ -849,11 +847,11 @@ void orc_mlp_forward(const double* P, int n, const double* x, double* y) {
   double z1[kHid], h1[kHid], z2[kHid], h2[kHid], z3[kHid], h3[kHid];
   for (int s = 0; s < n; ++s) {
     layer(P + kOffW1, P + kOffB1, kHid, kIn, x + s * kIn, z1);
-    for (int k = 0; k < kHid; ++k) h1[k] = silu(z1[k]);
+    for (int k = 0; k < kHid; ++k) h1[k] = act(z1[k]);
     layer(P + kOffW2, P + kOffB2, kHid, kHid, h1, z2);
-    for (int k = 0; k < kHid; ++k) h2[k] = silu(z2[k]);
+    for (int k = 0; k < kHid; ++k) h2[k] = act(z2[k]);
     layer(P + kOffW3, P + kOffB3, kHid, kHid, h2, z3);
-    for (int k = 0; k < kHid; ++k) h3[k] = silu(z3[k]);
+    for (int k = 0; k < kHid; ++k) h3[k] = act(z3[k]);
     layer(P + kOffW4, P + kOffB4, kOut, kHid, h3, y + s * kOut);
   }
 }
@@ -868,11 +866,11 @@ double orc_mlp_loss_grad(const double* P, int n, const double* x, const double* 
   for (int s = 0; s < n; ++s) {
     const double* xs = x + s * kIn;
     layer(P + kOffW1, P + kOffB1, kHid, kIn, xs, z1);
-    for (int k = 0; k < kHid; ++k) h1[k] = silu(z1[k]);
+    for (int k = 0; k < kHid; ++k) h1[k] = act(z1[k]);
     layer(P + kOffW2, P + kOffB2, kHid, kHid, h1, z2);
-    for (int k = 0; k < kHid; ++k) h2[k] = silu(z2[k]);
+    for (int k = 0; k < kHid; ++k) h2[k] = act(z2[k]);
     layer(P + kOffW3, P + kOffB3, kHid, kHid, h2, z3);
-    for (int k = 0; k < kHid; ++k) h3[k] = silu(z3[k]);
+    for (int k = 0; k < kHid; ++k) h3[k] = act(z3[k]);
     layer(P + kOffW4, P + kOffB4, kOut, kHid, h3, y);
     for (int o = 0; o < kOut; ++o) {
       const double e = y[o] - t[s * kOut + o];
@@ -886,7 +884,7 @@ double orc_mlp_loss_grad(const double* P, int n, const double* x, const double* 
     for (int i = 0; i < kHid; ++i) {
       double acc = 0;
       for (int o = 0; o < kOut; ++o) acc += P[kOffW4 + o * kHid + i] * dy[o];
-      d3[i] = acc * dsilu(z3[i]);
+      d3[i] = acc * dact(z3[i]);
     }
     for (int o = 0; o < kHid; ++o) {
       grad[kOffB3 + o] += d3[o];
@@ -895,7 +893,7 @@ double orc_mlp_loss_grad(const double* P, int n, const double* x, const double* 
     for (int i = 0; i < kHid; ++i) {
       double acc = 0;
       for (int o = 0; o < kHid; ++o) acc += P[kOffW3 + o * kHid + i] * d3[o];
-      d2[i] = acc * dsilu(z2[i]);
+      d2[i] = acc * dact(z2[i]);
     }
     for (int o = 0; o < kHid; ++o) {
       grad[kOffB2 + o] += d2[o];
@@ -904,7 +902,7 @@ double orc_mlp_loss_grad(const double* P, int n, const double* x, const double* 
     for (int i = 0; i < kHid; ++i) {
       double acc = 0;
       for (int o = 0; o < kHid; ++o) acc += P[kOffW2 + o * kHid + i] * d2[o];
-      d1[i] = acc * dsilu(z1[i]);
+      d1[i] = acc * dact(z1[i]);
     }
     for (int o = 0; o < kHid; ++o) {
       grad[kOffB1 + o] += d1[o];

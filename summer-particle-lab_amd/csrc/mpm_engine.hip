@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mpm_engine.h"
+#include "mpm_mlp_device.h"
 #include "mpm_internal.h"
 
 using namespace mpm;
@@ -349,7 +350,7 @@ int mpm_destroy(mpm_engine* e) {
   DevState& d = e->d;
   void* ptrs[] = {d.planes, d.planes_alt, d.ids,  d.ids_alt,  d.meta,     d.meta_alt, d.acc_m,
                   d.acc_p,  d.vel,        d.fixed, d.pieces,  d.keys,     d.keys_alt, d.vals,
-                  d.vals_alt, d.hist,     d.partials, d.red,  d.materials};
+                  d.vals_alt, d.hist,     d.partials, d.red,  d.materials, d.mlp, d.sur_counts};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (e->red_host) hipHostFree(e->red_host);
@@ -374,6 +375,11 @@ int mpm_set_materials(mpm_engine* e, const mpm_material* mats, int32_t count) {
     dm[i] = to_dev_material(mats[i]);
   }
   e->materials.assign(mats, mats + count);
+  if (e->d.mlp) {  // a new table switches the surrogate off
+    HIP_OK(e, hipStreamSynchronize(e->stream));
+    HIP_OK(e, hipFree(e->d.mlp));
+    e->d.mlp = nullptr;
+  }
   HIP_OK(e, hipMemcpyAsync(e->d.materials, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice,
                            e->stream));
   HIP_OK(e, hipStreamSynchronize(e->stream));
@@ -759,6 +765,72 @@ int mpm_debug_p2g(mpm_engine* e, double dt, double* grid_mass, double* grid_mome
     if (grid_momentum)
       for (int a = 0; a < 3; ++a) grid_momentum[i * 3 + a] = p[a * Nn + i];
   }
+  return MPM_OK;
+}
+
+int mpm_set_surrogate(mpm_engine* e, int32_t material, const float* params, const mpm_mlp_norm* norm) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  if (!params) {
+    if (e->d.mlp) {
+      HIP_OK(e, hipStreamSynchronize(e->stream));
+      HIP_OK(e, hipFree(e->d.mlp));
+      e->d.mlp = nullptr;
+    }
+    return MPM_OK;
+  }
+  if (!norm || material < 0 || material >= (int32_t)e->materials.size())
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "surrogate: material index %d outside the table", material);
+  const mpm_material& m = e->materials[material];
+  const bool rate_free = (m.model == MPM_MODEL_J2) || (m.model == MPM_MODEL_THERMAL_J2 && m.th_H == 0.f);
+  if (!rate_free)
+    return fail(e, MPM_ERR_UNSUPPORTED,
+                "surrogate: material %d (model %d) is not rate-independent J2 without hardening", material, m.model);
+  if (!(norm->in_scale > 0) || !(norm->out_scale > 0) || !(norm->eps_max > 0) || !(norm->T_hi >= norm->T_lo))
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "surrogate: invalid normalisation");
+  for (int k = 0; k < MPM_MLP_NPARAM; ++k)
+    if (!std::isfinite(params[k])) return fail(e, MPM_ERR_INVALID_ARGUMENT, "surrogate: parameter %d not finite", k);
+  MlpDev* host = new MlpDev;
+  MlpNorm nm;
+  nm.in_scale = norm->in_scale;
+  nm.T_center = norm->T_center;
+  nm.T_scale = norm->T_scale;
+  nm.out_scale = norm->out_scale;
+  nm.eps_max = norm->eps_max;
+  nm.T_lo = norm->T_lo;
+  nm.T_hi = norm->T_hi;
+  nm.material = material;
+  pack_mlp(params, nm, *host);
+  hipError_t st = hipSuccess;
+  if (!e->d.mlp) st = hipMalloc((void**)&e->d.mlp, sizeof(MlpDev));
+  if (st == hipSuccess && !e->d.sur_counts) {
+    st = hipMalloc((void**)&e->d.sur_counts, sizeof(unsigned long long) * 4 * SUR_SLOTS);
+    if (st == hipSuccess) st = hipMemsetAsync(e->d.sur_counts, 0, sizeof(unsigned long long) * 4 * SUR_SLOTS, e->stream);
+  }
+  if (st == hipSuccess) st = hipMemcpyAsync(e->d.mlp, host, sizeof(MlpDev), hipMemcpyHostToDevice, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  delete host;
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "surrogate upload: %s", hipGetErrorString(st));
+  return MPM_OK;
+}
+
+int mpm_surrogate_counts(mpm_engine* e, uint64_t* used, uint64_t* elastic, uint64_t* fallback, int32_t reset) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  uint64_t tot[3] = {0, 0, 0};
+  if (e->d.sur_counts) {
+    std::vector<unsigned long long> h(4 * SUR_SLOTS);
+    HIP_OK(e, hipMemcpyAsync(h.data(), e->d.sur_counts, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
+                             e->stream));
+    HIP_OK(e, hipStreamSynchronize(e->stream));
+    for (int s = 0; s < SUR_SLOTS; ++s)
+      for (int k = 0; k < 3; ++k) tot[k] += h[4 * s + k];
+    if (reset) {
+      HIP_OK(e, hipMemsetAsync(e->d.sur_counts, 0, sizeof(unsigned long long) * h.size(), e->stream));
+      HIP_OK(e, hipStreamSynchronize(e->stream));
+    }
+  }
+  if (used) *used = tot[0];
+  if (elastic) *elastic = tot[1];
+  if (fallback) *fallback = tot[2];
   return MPM_OK;
 }
 

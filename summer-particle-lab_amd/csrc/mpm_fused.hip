@@ -25,6 +25,7 @@
 #include <stdint.h>
 
 #include "mpm_internal.h"
+#include "mpm_mlp_device.h"
 
 namespace mpm {
 namespace {
@@ -39,6 +40,8 @@ constexpr int CHUNK = 8;           // lanes per scatter chunk (run pieces summed
 constexpr int WAVE_BYTES = TR_ROWS * TR_STRIDE * 4 + 64 * 16;
 constexpr int LDS_BYTES = ACC_BYTES + (FB / 64) * WAVE_BYTES;
 static_assert(LDS_BYTES >= 16 * PATCH_VEL, "gather box must fit the shared buffer");
+static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0 && ACC_BYTES % 16 == 0,
+              "surrogate activations must fit the per-wave buffer");
 
 __device__ __forceinline__ float ld(const float* P, uint32_t cap, int k, uint32_t i) {
   return P[(size_t)k * cap + i];
@@ -102,12 +105,14 @@ __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restri
   return mats[id];
 }
 
-template <int MODE>
+template <int MODE, bool SUR>
 __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
                                               uint32_t n, const DevMaterial* __restrict__ mats, GridDims g,
                                               StepParams sp, const float4* __restrict__ vel,
                                               unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-                                              uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces) {
+                                              uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
+                                              const MlpDev* __restrict__ mlp,
+                                              unsigned long long* __restrict__ sur_counts) {
   __shared__ __align__(16) unsigned char s_raw[LDS_BYTES];
   __shared__ int s_box[6 * (FB / 64)];
   const uint32_t tid = threadIdx.x;
@@ -173,6 +178,7 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
       }
     }
     __syncthreads();
+    M3 Fn = F;
     if (gather) {
       float V[3] = {0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0}, Z[3] = {0, 0, 0};
 #pragma unroll
@@ -225,7 +231,6 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
         Cm[c * 3 + 2] = s4 * (Z[c] - V[c] * t[2]);
         v[c] = V[c];
       }
-      M3 Fn;
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -234,6 +239,34 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
               F[a * 3 + c] + dt * (Cm[a * 3] * F[c] + Cm[a * 3 + 1] * F[3 + c] + Cm[a * 3 + 2] * F[6 + c]);
 #pragma unroll
       for (int a = 0; a < 3; ++a) x[a] += dt * V[a];
+    }
+    __syncthreads();  // the LDS box is free: the surrogate and the scatter reuse it
+
+    bool done = false;
+    if constexpr (SUR) {
+      // Learned return for the surrogate's material (J2 without hardening).
+      const bool mine = gather && (int)(mt & 255u) == mlp->norm.material && (mat.model == 4 || mat.model == 10);
+      float Tp = 0.f;
+      if (mine) {
+        Tp = ld(P, cap, P_TEMP, ic);
+        ep = ld(P, cap, P_EP, ic);
+      }
+      const int br = surrogate_return(mlp, mine, Fn, ep, Tp, mat, tau, s_raw + ACC_BYTES + (tid >> 6) * WAVE_BYTES);
+      if (br == SUR_NETWORK || br == SUR_ELASTIC) {
+        new_flags |= FLAG_CACHE;
+        F = Fn;
+        done = true;
+      }
+      const uint64_t bu = __ballot(br == SUR_NETWORK), be = __ballot(br == SUR_ELASTIC),
+                     bf = __ballot(br == SUR_FALLBACK);
+      if (lane == 0 && (bu | be | bf)) {
+        unsigned long long* slot = sur_counts + 4 * (blockIdx.x % SUR_SLOTS);
+        if (bu) atomicAdd(slot + 0, (unsigned long long)__popcll(bu));
+        if (be) atomicAdd(slot + 1, (unsigned long long)__popcll(be));
+        if (bf) atomicAdd(slot + 2, (unsigned long long)__popcll(bf));
+      }
+    }
+    if (gather && !done) {
       if (mat.model >= 2) {
         ep = ld(P, cap, P_EP, ic);
         ep += plastic_project(Fn, ep, dt, ld(P, cap, P_TEMP, ic), mat, tau);
@@ -243,7 +276,6 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
       }
       F = Fn;
     }
-    __syncthreads();  // the LDS box is reused by the scatter
 
     if (valid && !pinned) {
       st(P, cap, P_X, i, x[0]);
@@ -446,15 +478,21 @@ inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode) {
   if (!n) return;
   const dim3 grid(fblocks(n)), block(FB);
+#define MPM_FUSED(M, S)                                                                                          \
+  hipLaunchKernelGGL((k_fused<M, S>), grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, \
+                     d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts)
+  const bool sur = d.mlp != nullptr;
   if (mode == 0)
-    hipLaunchKernelGGL(k_fused<0>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
-                       d.acc_p, d.fixed, d.pieces);
+    MPM_FUSED(0, false);
+  else if (mode == 1 && sur)
+    MPM_FUSED(1, true);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_fused<1>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
-                       d.acc_p, d.fixed, d.pieces);
+    MPM_FUSED(1, false);
+  else if (sur)
+    MPM_FUSED(2, true);
   else
-    hipLaunchKernelGGL(k_fused<2>, grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, d.acc_m,
-                       d.acc_p, d.fixed, d.pieces);
+    MPM_FUSED(2, false);
+#undef MPM_FUSED
 }
 
 }  // namespace mpm

@@ -11,6 +11,9 @@
 
 namespace mpm {
 
+struct MlpDev;
+struct MlpNorm;
+
 // Particle state planes (FP32, one plane of `cap` floats each, slot order).
 enum Plane : int {
   P_X = 0, P_Y, P_Z,
@@ -66,7 +69,13 @@ struct DevState {
   // {fixed-point mass (int64 bits), momentum x, y, z}
   double* red;
   DevMaterial* materials;
+  // learned constitutive surrogate (nullptr: off) and its per-branch
+  // particle-step counters, SUR_SLOTS slots of {network, elastic, fallback, -}
+  MlpDev* mlp;
+  unsigned long long* sur_counts;
 };
+
+constexpr int SUR_SLOTS = 64;
 
 // Kernel launchers (mpm_kernels.hip)
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
@@ -74,9 +83,12 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
                  const int hi_idx[3]);
 // Fused particle kernel (mpm_fused.hip): mode 0 P2G from the stored state,
-// 1 G2P of this step + P2G of the next, 2 G2P writing the full state.
+// 1 G2P of this step + P2G of the next, 2 G2P writing the full state. With
+// d.mlp set, the G2P modes take the surrogate variant.
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode);
 uint32_t grid_blocks(const GridDims& g);
+// Trained parameters packed into the device record (mpm_mlp.hip).
+void pack_mlp(const float* params, const MlpNorm& norm, MlpDev& out);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 // Particle-side FP64 sums: particle_reduce_blocks() records of
 // particle_reduce_k() doubles each (see k_particle_reduce).

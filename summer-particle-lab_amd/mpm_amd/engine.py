@@ -190,6 +190,23 @@ class Engine:
                                             perm.ctypes.data_as(P)))
         return kin, kout, perm
 
+    def set_surrogate(self, material_index, params=None, norm=None):
+        """Use a trained surrogate network for particles of one material
+        (params None switches it off)."""
+        if params is None:
+            self._check(self._lib.mpm_set_surrogate(self.h, -1, None, None))
+            return
+        P = np.ascontiguousarray(np.asarray(params, np.float32).reshape(L.MLP_NPARAM))
+        self._check(self._lib.mpm_set_surrogate(self.h, material_index, P.ctypes.data_as(C.POINTER(C.c_float)),
+                                                C.byref(norm)))
+
+    def surrogate_counts(self, reset=False):
+        """Particle-steps since the last reset by branch: network, elastic,
+        plain return."""
+        u, e, f = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._check(self._lib.mpm_surrogate_counts(self.h, C.byref(u), C.byref(e), C.byref(f), int(reset)))
+        return dict(network=u.value, elastic=e.value, fallback=f.value)
+
     def debug_p2g(self, dt):
         N = C.c_int32()
         self._check(self._lib.mpm_debug_p2g(self.h, dt, None, None, C.byref(N)))

@@ -66,6 +66,20 @@ class MpmConservation(C.Structure):
     ]
 
 
+class MpmMlpTrainConfig(C.Structure):
+    _fields_ = [("samples", C.c_uint32), ("batch", C.c_uint32), ("epochs", C.c_uint32), ("seed", C.c_uint32),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("strain_sigma", C.c_float), ("T_lo", C.c_float),
+                ("T_hi", C.c_float), ("dt", C.c_float), ("eps_max", C.c_float)]
+
+
+class MpmMlpNorm(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("in_scale", "T_center", "T_scale", "out_scale", "eps_max", "T_lo",
+                                         "T_hi")] + [("material", C.c_int32)]
+
+
+MLP_NPARAM = 9222
+
+
 class MpmKernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
@@ -95,6 +109,21 @@ SIGNATURES = {
     "mpm_return_map_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.c_float] + [C.POINTER(C.c_float)] * 6),
     "mpm_stress_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.POINTER(C.c_float),
                                    C.POINTER(C.c_float)]),
+    "mpm_mlp_default_norm": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig),
+                                       C.POINTER(MpmMlpNorm)]),
+    "mpm_mlp_samples": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig), C.POINTER(MpmMlpNorm),
+                                  C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mpm_mlp_train": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig), C.POINTER(MpmMlpNorm),
+                                C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mpm_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mpm_mlp_infer": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mpm_surrogate_return_batch": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(C.c_float), C.POINTER(MpmMlpNorm),
+                                             C.c_uint32, C.c_float] + [C.POINTER(C.c_float)] * 6
+                                   + [C.POINTER(C.c_int32)]),
+    "mpm_set_surrogate": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.POINTER(MpmMlpNorm)]),
+    "mpm_surrogate_counts": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       C.c_int32]),
 }
 
 _lib = None

@@ -179,6 +179,30 @@ def elastic_cube(nu=0.3, E=1e5, rho=1000.0, sticky_floor=True, seed=42, strength
                  info=dict(E=E, nu=nu, rho=rho, spacing=h / 2, particle_volume=vp))
 
 
+# Thermal J2 of the surrogate scenes: yield A = 0.02 E with linear softening to
+# 10% between 300 K and 1200 K, perfectly plastic (H = 0), so that the trial
+# strains of the elastic-cube scene fall in the surrogate's training range.
+SURROGATE_THERMAL = dict(Tref=300.0, Thot=1200.0, hotRatio=0.1, A=0.02, H=0.0, kelvin_offset=0.0)
+
+
+def j2_cube(nu=0.3, E=1e5, rho=1000.0, seed=42, strength=0.2, T=(300.0, 900.0), thermal=None, steps=200,
+            drop=0.5):
+    """The elastic-cube scene with a thermal-J2 material (SURROGATE_THERMAL),
+    per-particle temperature uniform in [300, 900] K and a downward start
+    velocity `drop` (m/s) onto the sticky floor, so that the impact stress
+    (about rho c drop, 5 kPa) exceeds the yield stress (0.8-2 kPa): the scene
+    of the learned constitutive surrogate."""
+    sc = elastic_cube(nu=nu, E=E, rho=rho, seed=seed, strength=strength)
+    n = sc.n
+    sc.name = "j2_cube"
+    sc.v = np.tile(np.array([0.0, -drop, 0.0]), (n, 1))
+    sc.materials = [Material.from_E_nu(10, E, nu, thermal=dict(thermal or SURROGATE_THERMAL))]
+    sc.temperature = uniform(seed + 3, np.arange(n), 0, 0, T[0], T[1])
+    sc.steps = steps
+    sc.info.update(T=T, thermal=dict(thermal or SURROGATE_THERMAL), drop=drop)
+    return sc
+
+
 # Thermal-J2 steel in scaled units: E = 2e11 Pa, nu = 0.3, yield 2.5e8 Pa
 # (A = 1.25e-3 E) with linear softening to 10% between 300 K and 1200 K,
 # perfectly plastic (H = 0). The density is scaled to 2.5e7 kg/m^3 so that the

@@ -1,0 +1,288 @@
+"""GPU tests of the learned constitutive surrogate, through the C ABI.
+
+Bars:
+  * MFMA inference vs a BF16-rounded float64 forward (same roundings as the
+    kernel): max abs error <= 5e-3 of the largest output; vs the unrounded
+    FP64 oracle forward: <= 5e-2 (the BF16 rounding itself);
+  * training loss and gradient vs the FP64 oracle: loss to 1e-5 relative,
+    gradient to 1e-4 relative L2 overall and 1e-3 per parameter block;
+  * training samples vs the FP64 return map: 2e-4 in units of the yield stress;
+  * trained surrogate on held-out trial states (rotated, log strain N(0, 0.02),
+    T uniform 300-900 K) vs the FP64 return map: relative L2 of the stress
+    over the set <= 1e-2; states of the closed-form elastic branch <= 1e-5;
+  * the untrained network is at least 20 times worse than the trained one;
+  * a surrogate whose envelope holds no particle leaves the step equal to the
+    plain step to FP32 rounding (the two kernel variants contract FMAs
+    differently, so the last bit may differ);
+  * in a scene, grid mass stays exactly conserved and the trajectory stays
+    within 3e-2 relative L2 of displacement of the FP64 plain-return oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mpm_amd import Material, scenes
+from mpm_amd import lib as L
+from mpm_amd import surrogate as S
+from mpm_amd.engine import sym6_to_full
+from oracle import binding as o
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.3
+MAT = Material.from_E_nu(10, 1.0, NU, thermal=dict(scenes.SURROGATE_THERMAL))
+CFG = S.TrainConfig()
+SHAPES = [(64, 7), (64,), (64, 64), (64,), (64, 64), (64,), (6, 64), (6,)]
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest, ties to even), as float32."""
+    b = np.ascontiguousarray(np.asarray(a, np.float32)).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def unpack(p):
+    out, k = [], 0
+    for s in SHAPES:
+        n = int(np.prod(s))
+        out.append(np.asarray(p[k:k + n], np.float64).reshape(s))
+        k += n
+    return out
+
+
+def hswish(z):
+    return z * np.clip(z / 6 + 0.5, 0, 1)
+
+
+def bf16_forward(p, x):
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
+    h = bf16(x).astype(np.float64)
+    for W, b, last in ((W1, b1, False), (W2, b2, False), (W3, b3, False), (W4, b4, True)):
+        z = h @ bf16(W).astype(np.float64).T + b
+        h = z if last else bf16(hswish(z)).astype(np.float64)
+    return h
+
+
+def random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in SHAPES:
+        fan_in = s[1] if len(s) == 2 else 64
+        out.append(rng.uniform(-1, 1, int(np.prod(s))) * scale * math.sqrt(6 / fan_in) * (len(s) == 2))
+    return np.concatenate(out).astype(np.float32)
+
+
+def part_of(mat):
+    return o.make_part(E=mat.E, G=mat.mu / mat.E, L=mat.lam / mat.E, thermal=o.thermal_params(**mat.thermal))
+
+
+def oracle_return(mat, F, T, ep=0.0, dt=1e-4):
+    """FP64 plain return (src/three/constitutive.js:52-78 as restated by the
+    oracle): returned F, plastic increment and Kirchhoff stress (absolute)."""
+    part = part_of(mat)
+    r = o.project_plastic(F, mat.model, 1, 0, .025, 35, ep, dt, T, part)
+    tau = np.asarray(o.constitutive_stress(r["F"], mat.model, mat.mu, mat.lam)).reshape(3, 3)
+    return r["F"], r["plasticIncrement"], tau
+
+
+def rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def sym_exp(e6):
+    E = sym6_to_full(np.asarray(e6, np.float64)).reshape(3, 3)
+    lam, V = np.linalg.eigh(E)
+    return (V * np.exp(lam)) @ V.T
+
+
+@pytest.fixture(scope="module")
+def trained(gpu):
+    P, norm, hist = S.train(MAT, CFG)
+    print(f"\ntraining: {CFG.samples} samples, {CFG.epochs} epochs, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    return P, norm, hist
+
+
+def test_mfma_inference_matches_bf16_forward(gpu):
+    P = random_params(3)
+    rng = np.random.default_rng(5)
+    X = rng.normal(0, 1, (1000, 7)).astype(np.float32)  # 1000: a ragged last wave
+    Y = S.infer(P, X)
+    ref = bf16_forward(P, X)
+    exact = o.mlp_forward(P.astype(np.float64), X.astype(np.float64))
+    scale = np.abs(ref).max()
+    assert np.abs(Y - ref).max() <= 5e-3 * scale
+    assert np.abs(Y - exact).max() <= 5e-2 * scale
+    # a single sample (63 idle lanes) gives the same answer as inside a batch
+    assert np.allclose(S.infer(P, X[:1]), Y[:1], rtol=0, atol=1e-6 * scale)
+
+
+def test_training_gradient_matches_oracle(gpu):
+    P = random_params(11)
+    X, Y = S.samples(MAT, CFG, 512, seed=99)
+    loss, g = S.loss_grad(P, X, Y)
+    loss_o, g_o = o.mlp_loss_grad(P.astype(np.float64), X.astype(np.float64), Y.astype(np.float64))
+    assert abs(loss - loss_o) <= 1e-5 * loss_o
+    assert np.linalg.norm(g - g_o) <= 1e-4 * np.linalg.norm(g_o)
+    k = 0
+    for s in SHAPES:
+        n = int(np.prod(s))
+        assert np.linalg.norm(g[k:k + n] - g_o[k:k + n]) <= 1e-3 * np.linalg.norm(g_o[k:k + n]) + 1e-9, s
+        k += n
+
+
+def test_samples_match_fp64_return_map(gpu):
+    norm = S.default_norm(MAT, CFG)
+    X, Y = S.samples(MAT, CFG, 200, seed=123, norm=norm)
+    sigma_y = MAT.thermal["A"] * MAT.E
+    assert norm.out_scale == pytest.approx(sigma_y)
+    n_plastic = 0
+    for x, y in zip(X.astype(np.float64), Y.astype(np.float64)):
+        eps = x[:6] / norm.in_scale
+        T = x[6] / norm.T_scale + norm.T_center
+        assert 300 <= T <= 900
+        F = sym_exp(eps)
+        _, inc, tau = oracle_return(MAT, F.reshape(9), T, dt=CFG.dt)
+        dev = tau - np.trace(tau) / 3 * np.eye(3)
+        want = np.array([dev[0, 0], dev[1, 1], dev[2, 2], dev[0, 1], dev[0, 2], dev[1, 2]]) / sigma_y
+        assert np.abs(y - want).max() <= 2e-4, (x, y, want)
+        n_plastic += inc > 0
+    assert 0 < n_plastic < len(X)  # the sample set covers both branches
+
+
+def held_out(n, seed):
+    rng = np.random.default_rng(seed)
+    eps = rng.normal(0, CFG.strain_sigma, (n, 6))
+    T = rng.uniform(CFG.T_lo, CFG.T_hi, n)
+    F = np.stack([(sym_exp(e) @ rotation(rng)).reshape(9) for e in eps])
+    return F, T
+
+
+def surrogate_error(P, norm, F, T):
+    Fo, epo, tau, br = S.return_batch(MAT, P, norm, F, 0.0, T, CFG.dt)
+    ref_tau, ref_F, ref_ep = [], [], []
+    for f, t in zip(F, T):
+        Fr, inc, tr = oracle_return(MAT, f, t, dt=CFG.dt)
+        ref_tau.append(tr)
+        ref_F.append(Fr)
+        ref_ep.append(inc)
+    ref_tau = np.array(ref_tau)
+    got = sym6_to_full(tau.astype(np.float64)).reshape(-1, 3, 3)
+    return got, ref_tau, Fo, np.array(ref_F), epo, np.array(ref_ep), br
+
+
+def test_trained_surrogate_stress_error_below_one_percent(trained):
+    P, norm, hist = trained
+    assert hist[-1] < 0.05 * hist[0]
+    F, T = held_out(4000, 2024)
+    got, ref, Fo, Fr, epo, epr, br = surrogate_error(P, norm, F, T)
+    net, ela, fb = br == 0, br == 1, br == 2
+    err = np.linalg.norm((got - ref)[net | ela]) / np.linalg.norm(ref[net | ela])
+    err_net = np.linalg.norm((got - ref)[net]) / np.linalg.norm(ref[net])
+    print(f"\nheld-out: network {net.sum()}, elastic {ela.sum()}, fallback {fb.sum()}; "
+          f"stress rel L2 {err:.2e} (network states {err_net:.2e})")
+    assert net.sum() > 0.5 * len(F) and ela.sum() > 0
+    assert err <= 1e-2
+    # closed-form elastic branch: series logarithm vs the FP64 decomposition
+    e = np.linalg.norm((got - ref)[ela].reshape(-1, 9), axis=1) / np.linalg.norm(ref[ela].reshape(-1, 9), axis=1)
+    assert e.max() <= 1e-5
+    assert np.allclose(Fo[ela], F[ela], rtol=0, atol=1e-6)
+    # states outside the envelope took the plain return
+    if fb.any():
+        assert np.abs(got[fb] - ref[fb]).max() <= 5e-4 * np.abs(ref[fb]).max()
+    # returned F and plastic strain of the network states
+    assert np.linalg.norm(Fo[net] - Fr[net]) <= 1e-3 * np.linalg.norm(Fr[net])
+    assert np.abs(epo[net] - epr[net]).max() <= 2e-2 * max(epr[net].max(), 1e-12) + 2e-4
+
+
+def test_training_gains_over_untrained_network(trained):
+    P, norm, _ = trained
+    F, T = held_out(1000, 77)
+    got, ref, *_rest, br = surrogate_error(P, norm, F, T)
+    got0, ref0, *_rest0, br0 = surrogate_error(random_params(1, 0.5), norm, F, T)
+    net = br == 0
+    assert np.array_equal(br, br0)  # branch choice does not depend on the network
+    e_trained = np.linalg.norm((got - ref)[net]) / np.linalg.norm(ref[net])
+    e_untrained = np.linalg.norm((got0 - ref0)[net]) / np.linalg.norm(ref0[net])
+    print(f"\nstress rel L2 on network states: trained {e_trained:.2e}, untrained {e_untrained:.2e}")
+    assert e_untrained >= 20 * e_trained
+
+
+def j2_engine(sc, P=None, norm=None):
+    eng = sc.engine(sort_every=8)
+    if P is not None:
+        eng.set_surrogate(0, P, norm)
+    return eng
+
+
+def test_surrogate_outside_envelope_equals_plain_step(trained):
+    P, norm, _ = trained
+    sc = scenes.j2_cube(steps=40)
+    off = L.MpmMlpNorm()
+    for k in ("in_scale", "T_center", "T_scale", "out_scale", "eps_max"):
+        setattr(off, k, getattr(norm, k))
+    off.T_lo, off.T_hi = -2.0, -1.0  # no particle is this cold
+    a, b = j2_engine(sc), j2_engine(sc, P, off)
+    a.step(sc.dt, sc.steps)
+    b.step(sc.dt, sc.steps)
+    sa, sb = a.download(), b.download()
+    for k in ("positions", "velocities", "F", "plastic"):
+        assert np.abs(sa[k] - sb[k]).max() <= 1e-6 * max(np.abs(sa[k]).max(), 1e-30), k
+    c = b.surrogate_counts()
+    assert c["network"] == 0 and c["elastic"] == 0 and c["fallback"] == sc.n * sc.steps
+
+
+def test_surrogate_scene_tracks_fp64_oracle(trained):
+    from test_gpu_engine import oracle_for, rel_l2
+    P, norm, _ = trained
+    sc = scenes.j2_cube()
+    eng = j2_engine(sc, P, norm)
+    plain = j2_engine(sc)
+    xg0 = eng.download()["positions"]
+    orc = oracle_for(sc, False)
+    x0 = orc.positions().copy()
+    for _ in range(4):
+        eng.step(sc.dt, sc.steps // 4)
+        plain.step(sc.dt, sc.steps // 4)
+        c = eng.conservation()
+        assert c["grid_mass_q"] == c["particle_mass_q"]
+    for _ in range(sc.steps):
+        orc.step(sc.dt)
+    counts = eng.surrogate_counts()
+    dxo = orc.positions() - x0
+    e_sur = rel_l2(eng.download()["positions"] - xg0, dxo)
+    e_plain = rel_l2(plain.download()["positions"] - xg0, dxo)
+    print(f"\nj2_cube {sc.steps} steps: branches {counts}; displacement rel L2 vs FP64 oracle: "
+          f"surrogate {e_sur:.2e}, plain {e_plain:.2e}")
+    assert counts["network"] > 0 and counts["elastic"] > 0
+    assert e_plain <= 1e-2
+    assert e_sur <= 3e-2
+
+
+def test_set_surrogate_rejects_unsupported_materials(gpu):
+    P = random_params(2)
+    norm = S.default_norm(MAT, CFG)
+    sc = scenes.elastic_cube()
+    eng = sc.engine()
+    with pytest.raises(L.MpmError) as ei:
+        eng.set_surrogate(0, P, norm)  # fixed-corotated
+    assert ei.value.code == L.MPM_ERR_UNSUPPORTED
+    hard = Material.from_E_nu(10, 1e5, NU, thermal=dict(scenes.SURROGATE_THERMAL, H=0.1))
+    eng.set_materials([hard])
+    with pytest.raises(L.MpmError):
+        eng.set_surrogate(0, P, norm)  # hardening: the return depends on ep
+    eng.set_materials([Material.from_E_nu(10, 1e5, NU, thermal=dict(scenes.SURROGATE_THERMAL))])
+    with pytest.raises(L.MpmError):
+        eng.set_surrogate(1, P, norm)  # no such material
+    bad = P.copy()
+    bad[5] = np.nan
+    with pytest.raises(L.MpmError):
+        eng.set_surrogate(0, bad, norm)
+    eng.set_surrogate(0, P, norm)
+    eng.set_surrogate(0, None)

@@ -22,7 +22,7 @@ def header_symbols():
 def test_library_exports_every_declared_symbol():
     lib = L.load()
     syms = header_symbols()
-    assert len(syms) >= 19
+    assert len(syms) >= 27
     for s in syms:
         assert hasattr(lib, s), s
     assert syms == set(L.SIGNATURES), syms ^ set(L.SIGNATURES)
@@ -30,7 +30,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layout_matches_c(tmp_path):
     structs = {"mpm_config": L.MpmConfig, "mpm_material": L.MpmMaterial, "mpm_particles": L.MpmParticles,
-               "mpm_conservation": L.MpmConservation, "mpm_kernel_time": L.MpmKernelTime}
+               "mpm_conservation": L.MpmConservation, "mpm_kernel_time": L.MpmKernelTime,
+               "mpm_mlp_train_config": L.MpmMlpTrainConfig, "mpm_mlp_norm": L.MpmMlpNorm}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_engine.h"', "int main(void){"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -79,3 +80,33 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(RuntimeError, match="not built"):
         L.load()
+
+
+def test_surrogate_entry_points_validate_on_the_host():
+    lib = L.load()
+    m = L.MpmMaterial()
+    m.model, m.E, m.th_A = 10, 2.0, 0.02
+    c = L.MpmMlpTrainConfig(samples=1024, batch=256, epochs=1, seed=1, lr=1e-3, lr_final=1e-5, strain_sigma=0.02,
+                            T_lo=300.0, T_hi=900.0, dt=1e-4, eps_max=0.0)
+    n = L.MpmMlpNorm()
+    assert lib.mpm_mlp_default_norm(C.byref(m), C.byref(c), C.byref(n)) == L.MPM_OK
+    assert n.in_scale == pytest.approx(50.0) and n.T_center == 600.0 and n.T_scale == pytest.approx(1 / 300)
+    assert n.out_scale == pytest.approx(0.04) and n.eps_max == pytest.approx(0.08) and n.material == -1
+    assert (n.T_lo, n.T_hi) == (300.0, 900.0)
+    c.T_hi = 100.0
+    assert lib.mpm_mlp_default_norm(C.byref(m), C.byref(c), C.byref(n)) == L.MPM_ERR_INVALID_ARGUMENT
+    c.T_hi = 900.0
+    P = (C.c_float * L.MLP_NPARAM)()
+    assert lib.mpm_mlp_train(None, C.byref(c), C.byref(n), P, None) == L.MPM_ERR_INVALID_ARGUMENT
+    c.batch = 4096  # larger than the sample count
+    assert lib.mpm_mlp_train(C.byref(m), C.byref(c), C.byref(n), P, None) == L.MPM_ERR_INVALID_ARGUMENT
+    m.model = 0  # no plastic return to learn
+    assert lib.mpm_mlp_samples(C.byref(m), C.byref(c), C.byref(n), 0, 1, None, None) == L.MPM_ERR_UNSUPPORTED
+    assert lib.mpm_surrogate_return_batch(C.byref(m), P, C.byref(n), 0, 0.0, None, None, None, None, None, None,
+                                          None) == L.MPM_ERR_UNSUPPORTED
+    m.model, m.th_H = 10, 0.5  # hardening: the return depends on the plastic history
+    assert lib.mpm_surrogate_return_batch(C.byref(m), P, C.byref(n), 0, 0.0, None, None, None, None, None, None,
+                                          None) == L.MPM_ERR_UNSUPPORTED
+    assert lib.mpm_set_surrogate(None, 0, P, C.byref(n)) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_surrogate_counts(None, None, None, None, 0) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_mlp_loss_grad(None, 0, None, None, None, None) == L.MPM_ERR_INVALID_ARGUMENT

@@ -1,5 +1,5 @@
 """Pins the FP64 surrogate-network oracle (forward and backward of the
-7-64-64-64-6 SiLU network) against the mathematics: an independent numpy
+7-64-64-64-6 hard-swish network) against the mathematics: an independent numpy
 forward, and central finite differences of the loss for every parameter
 block. CPU only."""
 import numpy as np
@@ -20,10 +20,11 @@ def unpack(p):
 
 def numpy_forward(p, x):
     W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
-    silu = lambda z: z / (1 + np.exp(-z))  # noqa: E731
-    h = silu(x @ W1.T + b1)
-    h = silu(h @ W2.T + b2)
-    h = silu(h @ W3.T + b3)
+    def act(z):  # hard-swish, written as x * relu6(x + 3) / 6
+        return z * np.minimum(np.maximum(z + 3, 0), 6) / 6
+    h = act(x @ W1.T + b1)
+    h = act(h @ W2.T + b2)
+    h = act(h @ W3.T + b3)
     return h @ W4.T + b4
 
 
