@@ -10,6 +10,11 @@ Extra fields:
                 on the engine's stream (profiled pass after the timed one).
   cpu_baseline  the FP64 CPU restatement of the reference step (oracle),
                 single thread, on a bounded sample of the same workload.
+  batched       256 independent scenes (2k-200k particles, 32^3/64^3/96^3
+                grids, mixed elastic / thermal-J2) in one batch: autotuned,
+                300 steps timed with graph replay, the same without graphs,
+                and the same scenes stepped one after another by single-scene
+                engines.
   surrogate     the same workload with the learned constitutive surrogate:
                 network trained on the device first (timed separately), then
                 K steps timed as above; its stress error on held-out trial
@@ -50,6 +55,11 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=8)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--batched-scenes", type=int, default=256)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                   help="first copy MB megabytes device to device (a kernel of known traffic for scaling "
+                        "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -91,6 +101,56 @@ def held_out_states(n, sigma, T_lo, T_hi, seed=2024):
                   2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
                   2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1).reshape(n, 3, 3)
     return (S @ R).reshape(n, 9), rng.uniform(T_lo, T_hi, n)
+
+
+def batched_leg(args):
+    """256 scenes in one batch vs the same scenes one by one."""
+    import torch
+    from mpm_amd import scenes
+    bs = scenes.batched_scenes(nscenes=args.batched_scenes)
+    steps = bs.steps
+    total = bs.total
+    out = dict(workload=f"{len(bs.scenes)} scenes, counts log-uniform 2k-200k, grids 32^3/64^3/96^3 cells, "
+                        f"mixed fixed-corotated / thermal J2, {steps} steps", scenes=len(bs.scenes), particles=total)
+    b = bs.batch(sort_every=16, use_graph=True)
+    out["tuning"] = b.tune(trial_steps=8)
+    b.step(2 * out["tuning"]["sort_every"])
+    b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    for i in range(len(bs.scenes)):
+        c = b.conservation(i)
+        assert c["grid_mass_q"] == c["particle_mass_q"] and c["nonfinite"] == 0, (i, c)
+    out.update(value=round(total * steps / el, 1), unit="particle-steps/s", ms_per_step=round(el / steps * 1e3, 4))
+    del b
+    e = bs.batch(sort_every=out["tuning"]["sort_every"], use_graph=False)
+    e.set_tuning(particles_per_group=out["tuning"]["particles_per_group"])
+    e.step(8)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.synchronize()
+    el_e = time.perf_counter() - t0
+    out["eager"] = dict(value=round(total * steps / el_e, 1), ms_per_step=round(el_e / steps * 1e3, 4))
+    del e
+    # the same scenes, one single-scene engine each, stepped one after another
+    engines = [sc.engine(sort_every=out["tuning"]["sort_every"]) for sc in bs.scenes]
+    for eng, sc in zip(engines, bs.scenes):
+        eng.step(sc.dt, 2)
+    for eng in engines:
+        eng.synchronize()
+    t0 = time.perf_counter()
+    for eng, sc in zip(engines, bs.scenes):
+        eng.step(sc.dt, steps)
+    for eng in engines:
+        eng.synchronize()
+    el_s = time.perf_counter() - t0
+    out["one_by_one"] = dict(value=round(total * steps / el_s, 1), ms_per_step=round(el_s / steps * 1e3, 4))
+    out["speedup_vs_one_by_one"] = round(el_s / el, 2)
+    return out
 
 
 def surrogate_leg(sc, args, plain_value):
@@ -170,6 +230,11 @@ def main():
     from mpm_amd import scenes
 
     torch.cuda.set_device(0)
+    if args.copy_calibration > 0:
+        a = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        del a
     sc = scenes.thermal_boxes()
     eng = sc.engine(sort_every=args.sort_every)
     elapsed = timed_steps(eng, sc, args.steps, args.warmup)
@@ -209,6 +274,7 @@ def main():
                     algorithmic_bytes_per_launch=d["bytes"], avg_launch_us=d["avg_us"])
     del eng
     sur = None if args.no_surrogate else surrogate_leg(sc, args, value)
+    batched = None if args.no_batched else batched_leg(args)
     cpu = None if args.no_cpu else cpu_baseline(sc, args.cpu_steps)
     out = {
         "metric": "particle-steps/s (P2G+grid+G2P), plain vs MLP-constitutive at <=1% stress err",
@@ -230,6 +296,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": per_kernel,
         "surrogate": sur,
+        "batched": batched,
     }
     print(json.dumps(out))
 

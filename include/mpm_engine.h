@@ -235,7 +235,7 @@ int mpm_stress_batch(const mpm_material* m, uint32_t n, const float* F, float* t
  * output-major) maps the trial elastic logarithmic strain eps (xx, yy, zz,
  * xy, xz, yz) and the temperature T to the deviatoric part of the
  * return-mapped Kirchhoff stress. Inputs are eps * in_scale and
- * (T - T_center) * T_scale; outputs are in units of out_scale. It replaces
+ * (T - T_center) * T_scale; outputs are in units of out_scale * E. It replaces
  * the plastic return of projectPlastic (src/three/constitutive.js:52-78) for
  * rate-independent J2 models without hardening (model 4; model 10 with
  * H = 0), where the return depends on eps and T only. */
@@ -254,15 +254,28 @@ typedef struct {
 } mpm_mlp_train_config;
 
 typedef struct {
-  float in_scale, T_center, T_scale, out_scale;
+  float in_scale, T_center, T_scale;
+  float out_scale;     /* output unit in units of E (the yield ratio A);
+                          the stress is output * out_scale * E */
   float eps_max;       /* envelope: |eps_ij| <= eps_max */
   float T_lo, T_hi;    /* envelope: T_lo <= T <= T_hi */
   int32_t material;    /* set by mpm_set_surrogate */
+  /* the training material, in E-normalised form: a network applies to any
+   * material with the same model, mu/E, lambda/E, yield ratio and thermal
+   * softening (the return map scales with E) */
+  int32_t model;
+  float mu_over_E, lambda_over_E, A, Tref, Thot, hotRatio;
 } mpm_mlp_norm;
 
 /* Normalisation for material m and a training config: in_scale 1/sigma,
- * T mapped to [-1, 1], out_scale = A E (the cold yield stress). */
+ * T mapped to [-1, 1], out_scale = A (the cold yield stress over E), and m's
+ * E-normalised fingerprint. */
 int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, mpm_mlp_norm* out);
+
+/* MPM_OK when material m matches norm's training fingerprint (E-normalised
+ * parameters within a relative 1e-5), else MPM_ERR_INVALID_ARGUMENT. Host
+ * only. */
+int mpm_mlp_check_material(const mpm_material* m, const mpm_mlp_norm* norm);
 
 /* n training samples generated on the device by the plain return map
  * (mpm_return_map_batch's update with F = exp(eps), ep = 0): X [7n] inputs,
@@ -290,21 +303,79 @@ int mpm_mlp_infer(const float* params, uint32_t n, const float* X, float* Y);
  * [n] and T [n] of material m: closed-form elastic branch inside the yield
  * surface, network inside the envelope, plain return (as
  * mpm_return_map_batch) otherwise. Outputs as mpm_return_map_batch, plus
- * branch [n]: 0 network, 1 elastic, 2 plain return. */
+ * branch [n]: 0 network, 1 elastic, 2 plain return. m must match norm's
+ * fingerprint. */
 int mpm_surrogate_return_batch(const mpm_material* m, const float* params, const mpm_mlp_norm* norm, uint32_t n,
                                float dt, const float* F, const float* ep, const float* T, float* F_out, float* ep_out,
                                float* tau_out, int32_t* branch);
 
 /* Use the network for particles of material index `material` whose trial
  * strain and temperature lie inside norm's envelope; the others keep the
- * plain return. params == NULL switches the surrogate off, and so does
- * mpm_set_materials. */
+ * plain return. The material must match norm's fingerprint (relative
+ * tolerance 1e-5), else MPM_ERR_INVALID_ARGUMENT. params == NULL switches
+ * the surrogate off, and so does mpm_set_materials. */
 int mpm_set_surrogate(mpm_engine* e, int32_t material, const float* params, const mpm_mlp_norm* norm);
 
 /* Particle-steps of the surrogate's material since the last reset that took
  * the network (used), the closed-form elastic branch (elastic) and the plain
  * return outside the envelope (fallback). reset != 0 clears the counters. */
 int mpm_surrogate_counts(mpm_engine* e, uint64_t* used, uint64_t* elastic, uint64_t* fallback, int32_t reset);
+
+/* ---- batched scenes (no reference counterpart) ----
+ *
+ * Many independent scenes advanced together: one particle pool and one
+ * concatenated grid on one GPU, one persistent particle launch per size
+ * bucket and one grid launch per step for all scenes, and optionally the
+ * whole re-bin period replayed as a captured HIP graph. The reference runs
+ * one scene per Worker or Python process (gpu/server.py:34-40); these calls
+ * batch what would be many MPM3 instances. Scenes share one material table;
+ * each has its own domain (mpm_config; capacity and sort_every are ignored),
+ * particle count (fixed at creation) and step size. */
+typedef struct mpm_batch mpm_batch;
+
+typedef struct {
+  int32_t sort_every;      /* re-bin period in steps (>= 1; default 16) */
+  int32_t use_graph;       /* 1: replay whole re-bin periods as HIP graphs */
+  int32_t particle_blocks; /* workgroups of the persistent particle launch
+                              (0: three per compute unit) */
+} mpm_batch_options;
+
+/* Launch configuration: scenes are bucketed by grid edge (nodes per axis
+ * <= bucket_edge[0], <= bucket_edge[1], larger); each bucket has its own
+ * particles per workgroup (64, 128 or 256). */
+typedef struct {
+  int32_t buckets;
+  int32_t bucket_edge[4];
+  int32_t bucket_scenes[4];
+  int32_t particles_per_group[4];
+  int32_t sort_every;
+  int32_t particle_blocks;
+  double ms_per_step;   /* from mpm_batch_tune: whole-step time of the choice */
+  double bucket_ms[4];  /* from mpm_batch_tune: particle launch time per step */
+} mpm_batch_tuning;
+
+int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t nscenes,
+                     const mpm_batch_options* options, mpm_batch** out);
+int mpm_batch_destroy(mpm_batch* b);
+const char* mpm_batch_last_error(const mpm_batch* b);
+int mpm_batch_scene_count(const mpm_batch* b, int32_t* scenes, uint32_t* particles);
+int mpm_batch_set_materials(mpm_batch* b, const mpm_material* materials, int32_t count);
+/* Upload / download one scene; p->count must equal the scene's count. */
+int mpm_batch_upload(mpm_batch* b, int32_t scene, const mpm_particles* p);
+int mpm_batch_download(mpm_batch* b, int32_t scene, mpm_particles* p);
+/* Step size of every scene, dt[nscenes]. */
+int mpm_batch_set_dt(mpm_batch* b, const double* dt);
+/* Advance every scene nsteps steps (asynchronous on the batch stream). */
+int mpm_batch_step(mpm_batch* b, int32_t nsteps);
+int mpm_batch_synchronize(mpm_batch* b);
+/* FP64 conservation report of one scene (as mpm_conservation_report). */
+int mpm_batch_conservation(mpm_batch* b, int32_t scene, mpm_conservation* out);
+int mpm_batch_get_tuning(const mpm_batch* b, mpm_batch_tuning* out);
+int mpm_batch_set_tuning(mpm_batch* b, const mpm_batch_tuning* t);
+/* Autotuner: times particles per workgroup {64, 128, 256} per bucket on
+ * trial_steps steps each, then the re-bin period {4, 8, 16, 32}, on the
+ * resident scenes (which it advances), and keeps the fastest. */
+int mpm_batch_tune(mpm_batch* b, int32_t trial_steps, mpm_batch_tuning* out);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mpm_engine.h"
+#include "mpm_host.h"
 #include "mpm_mlp_device.h"
 #include "mpm_internal.h"
 
@@ -73,21 +74,25 @@ hipEvent_t take_event(mpm_engine* e) {
   return ev;
 }
 
-// Brackets one launch with events when profiling is on.
+// Times one launch when profiling is on. Launchers that take LaunchEvents
+// have the events stamped by the kernel dispatch itself (stamped = true);
+// otherwise the events are recorded on the stream around the launches.
 struct Timed {
   mpm_engine* e;
   const char* name;
+  bool stamped;
   hipEvent_t a = nullptr, b = nullptr;
-  Timed(mpm_engine* e_, const char* n) : e(e_), name(n) {
+  Timed(mpm_engine* e_, const char* n, bool stamped_ = false) : e(e_), name(n), stamped(stamped_) {
     if (e->profiling) {
       a = take_event(e);
       b = take_event(e);
-      hipEventRecord(a, e->stream);
+      if (!stamped) hipEventRecord(a, e->stream);
     }
   }
+  LaunchEvents ev() const { return LaunchEvents{a, b}; }
   ~Timed() {
     if (e->profiling) {
-      hipEventRecord(b, e->stream);
+      if (!stamped) hipEventRecord(b, e->stream);
       e->pending.push_back({name, a, b});
     }
   }
@@ -155,20 +160,6 @@ DevMaterial to_dev_material(const mpm_material& m) {
 namespace {
 
 bool supported_model(int m) { return m == 0 || (m >= 4 && m <= 7) || m == 10 || m == 11; }
-
-template <typename T>
-inline double rd(const void* p, size_t i) {
-  return (double)static_cast<const T*>(p)[i];
-}
-inline double rdv(const void* p, int dtype, size_t i) {
-  return dtype == MPM_F64 ? rd<double>(p, i) : rd<float>(p, i);
-}
-inline void wrv(void* p, int dtype, size_t i, double v) {
-  if (dtype == MPM_F64)
-    static_cast<double*>(p)[i] = v;
-  else
-    static_cast<float*>(p)[i] = (float)v;
-}
 
 // Bin, sort and permute the resident particles into cell order.
 int rebin(mpm_engine* e) {
@@ -239,60 +230,19 @@ const char* mpm_last_error(const mpm_engine* e) { return e ? e->err.c_str() : "n
 int mpm_create(const mpm_config* c, mpm_engine** out) {
   if (!c || !out) return MPM_ERR_INVALID_ARGUMENT;
   *out = nullptr;
-  const double h = c->cell_size;
-  if (!(h > 0) || !std::isfinite(h)) return MPM_ERR_INVALID_ARGUMENT;
-  for (int a = 0; a < 3; ++a) {
-    if (!std::isfinite(c->lower[a]) || !std::isfinite(c->upper[a]) || c->upper[a] - c->lower[a] < h)
-      return MPM_ERR_INVALID_ARGUMENT;
-  }
-  for (int f = 0; f < 6; ++f)
-    if (c->wall_mode[f] < 0 || c->wall_mode[f] > 2) return MPM_ERR_INVALID_ARGUMENT;
-  if (c->strict_domain) {  // src/three/discretization.js:8-10
-    if (h < 4 || h > 16) return MPM_ERR_INVALID_ARGUMENT;
-    for (int a = 0; a < 3; ++a) {
-      const double cells = std::ceil(c->upper[a] / h - 1e-10) - std::floor(c->lower[a] / h + 1e-10);
-      if (c->lower[a] < -2400 || c->upper[a] > 2400 || cells > 60) return MPM_ERR_INVALID_ARGUMENT;
-    }
-  }
+  GridDims g;
+  int lo_idx[3], hi_idx[3];
+  const int rc = grid_geometry(c, g, lo_idx, hi_idx);
+  if (rc) return rc;
+  const uint64_t Nn = (uint64_t)g.N * g.N * g.N;
   auto* e = new mpm_engine();
   e->cfg = *c;
   if (e->cfg.sort_every < 1) e->cfg.sort_every = 1;
-  // Geometry of src/three/discretization.js:13 with a cubic node block of
-  // edge max(nodes) (src/three/mpm.js:18).
-  int N = 0;
-  double origin[3];
+  e->g = g;
   for (int a = 0; a < 3; ++a) {
-    origin[a] = std::floor(c->lower[a] / h) * h - 2 * h;
-    N = std::max(N, (int)(std::ceil((c->upper[a] - origin[a]) / h) + 3));
+    e->lo_idx[a] = lo_idx[a];
+    e->hi_idx[a] = hi_idx[a];
   }
-  const uint64_t Nn = (uint64_t)N * N * N;
-  if (Nn > (1ull << 31)) {
-    delete e;
-    return MPM_ERR_CAPACITY;
-  }
-  GridDims& g = e->g;
-  g.N = N;
-  for (int a = 0; a < 3; ++a) {
-    g.O[a] = (float)origin[a];
-    g.lower[a] = (float)c->lower[a];
-    g.upper[a] = (float)c->upper[a];
-    g.gravity[a] = (float)c->gravity[a];
-    // Wall bands as node-index thresholds, evaluated in FP64 exactly as the
-    // reference compares node positions O + i h with lower + h / upper - h.
-    int lo = -1, hi = N;
-    for (int i = 0; i < N; ++i) {
-      const double q = origin[a] + (double)i * h;
-      if (q < c->lower[a] + h) lo = i;
-      if (q > c->upper[a] - h && hi == N) hi = i;
-    }
-    e->lo_idx[a] = lo;
-    e->hi_idx[a] = hi;
-  }
-  for (int f = 0; f < 6; ++f) g.wall[f] = c->wall_mode[f];
-  g.H = (float)h;
-  g.invH = (float)(1.0 / h);
-  g.gravity_on_particles = c->gravity_on_particles ? 1 : 0;
-  g.mass_eps = (float)c->mass_eps;
 
   hipError_t st = hipGetDevice(&e->device);
   if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -390,57 +340,15 @@ int mpm_upload(mpm_engine* e, const mpm_particles* p) {
   if (!e || !p) return MPM_ERR_INVALID_ARGUMENT;
   const uint32_t n = p->count;
   if (n > e->d.cap) return fail(e, MPM_ERR_CAPACITY, "count %u exceeds capacity %u", n, e->d.cap);
-  if (p->dtype != MPM_F64 && p->dtype != MPM_F32) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dtype");
-  if (n && (!p->x || !p->y || !p->z || !p->vx || !p->vy || !p->vz || !p->F || !p->C || !p->mass || !p->volume ||
-            !p->temperature || !p->plastic || !p->material || !p->pinned))
-    return fail(e, MPM_ERR_INVALID_ARGUMENT, "missing particle array");
-  const int dt = p->dtype;
+  if (!particles_complete(p)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dtype or missing particle array");
   const uint32_t cap = e->d.cap;
   std::vector<float> planes((size_t)NPLANES * cap, 0.f);
   std::vector<uint32_t> ids(n), meta(n);
-  for (uint32_t i = 0; i < n; ++i) {
-    if (p->material[i] >= e->materials.size())
-      return fail(e, MPM_ERR_INVALID_ARGUMENT, "particle %u: material %u out of range", i, p->material[i]);
-    const double m = rdv(p->mass, dt, i), vol = rdv(p->volume, dt, i);
-    if (!(m > 0) || !(vol > 0)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "particle %u: mass and volume must be > 0", i);
-    auto set = [&](int k, double v) { planes[(size_t)k * cap + i] = (float)v; };
-    set(P_X, rdv(p->x, dt, i));
-    set(P_Y, rdv(p->y, dt, i));
-    set(P_Z, rdv(p->z, dt, i));
-    set(P_VX, rdv(p->vx, dt, i));
-    set(P_VY, rdv(p->vy, dt, i));
-    set(P_VZ, rdv(p->vz, dt, i));
-    for (int k = 0; k < 9; ++k) {
-      set(P_C0 + k, rdv(p->C, dt, (size_t)i * 9 + k));
-      set(P_F0 + k, rdv(p->F, dt, (size_t)i * 9 + k));
-    }
-    uint32_t flags = p->pinned[i] ? FLAG_PINNED : 0u;
-    if (p->cached_tau && p->cache_valid && p->cache_valid[i]) {
-      const size_t b = (size_t)i * 9;
-      set(P_T0 + 0, rdv(p->cached_tau, dt, b + 0));
-      set(P_T0 + 1, rdv(p->cached_tau, dt, b + 4));
-      set(P_T0 + 2, rdv(p->cached_tau, dt, b + 8));
-      set(P_T0 + 3, 0.5 * (rdv(p->cached_tau, dt, b + 1) + rdv(p->cached_tau, dt, b + 3)));
-      set(P_T0 + 4, 0.5 * (rdv(p->cached_tau, dt, b + 2) + rdv(p->cached_tau, dt, b + 6)));
-      set(P_T0 + 5, 0.5 * (rdv(p->cached_tau, dt, b + 5) + rdv(p->cached_tau, dt, b + 7)));
-      flags |= FLAG_CACHE;
-    }
-    set(P_MASS, m);
-    set(P_VOL, vol);
-    set(P_TEMP, rdv(p->temperature, dt, i));
-    set(P_EP, rdv(p->plastic, dt, i));
-    ids[i] = i;
-    meta[i] = (uint32_t)p->material[i] | (flags << 8);
-  }
-  // Fixed-point mass unit 2^-k with the heaviest particle at most 2^24 units:
-  // a particle's 27 integer parts, and the sums of up to 64 of them inside a
-  // wave, fit 32 bits; node and grand totals are 64-bit.
+  const std::string why = pack_particles(p, e->materials.size(), planes.data(), cap, 0, 0, ids.data(), meta.data());
+  if (!why.empty()) return fail(e, MPM_ERR_INVALID_ARGUMENT, "%s", why.c_str());
   double mmax = 0;
   for (uint32_t i = 0; i < n; ++i) mmax = std::max(mmax, (double)planes[(size_t)P_MASS * cap + i]);
-  int k = 0;
-  if (mmax > 0) k = (int)std::floor(24.0 - std::log2(mmax));
-  k = std::max(-100, std::min(100, k));
-  e->mass_scale = std::ldexp(1.0, k);
+  e->mass_scale = std::ldexp(1.0, mass_exponent(mmax));
   int64_t q = 0;
   const float sf = (float)e->mass_scale;
   for (uint32_t i = 0; i < n; ++i) q += (int64_t)std::nearbyint(planes[(size_t)P_MASS * cap + i] * sf);
@@ -462,8 +370,7 @@ int mpm_download(mpm_engine* e, mpm_particles* p) {
   if (!e || !p) return MPM_ERR_INVALID_ARGUMENT;
   if (p->count != e->n) return fail(e, MPM_ERR_INVALID_ARGUMENT, "count %u != resident %u", p->count, e->n);
   const uint32_t n = e->n, cap = e->d.cap;
-  const int dt = p->dtype;
-  if (dt != MPM_F64 && dt != MPM_F32) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dtype");
+  if (p->dtype != MPM_F64 && p->dtype != MPM_F32) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dtype");
   std::vector<float> planes((size_t)NPLANES * cap);
   std::vector<uint32_t> ids(n), meta(n);
   HIP_OK(e, hipMemcpyAsync(planes.data(), e->d.planes, sizeof(float) * planes.size(), hipMemcpyDeviceToHost,
@@ -473,32 +380,7 @@ int mpm_download(mpm_engine* e, mpm_particles* p) {
     HIP_OK(e, hipMemcpyAsync(meta.data(), e->d.meta, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream));
   }
   HIP_OK(e, hipStreamSynchronize(e->stream));
-  for (uint32_t s = 0; s < n; ++s) {
-    const uint32_t i = ids[s];
-    auto get = [&](int k) { return (double)planes[(size_t)k * cap + s]; };
-    if (p->x) wrv(p->x, dt, i, get(P_X));
-    if (p->y) wrv(p->y, dt, i, get(P_Y));
-    if (p->z) wrv(p->z, dt, i, get(P_Z));
-    if (p->vx) wrv(p->vx, dt, i, get(P_VX));
-    if (p->vy) wrv(p->vy, dt, i, get(P_VY));
-    if (p->vz) wrv(p->vz, dt, i, get(P_VZ));
-    for (int k = 0; k < 9; ++k) {
-      if (p->F) wrv(p->F, dt, (size_t)i * 9 + k, get(P_F0 + k));
-      if (p->C) wrv(p->C, dt, (size_t)i * 9 + k, get(P_C0 + k));
-    }
-    if (p->mass) wrv(p->mass, dt, i, get(P_MASS));
-    if (p->volume) wrv(p->volume, dt, i, get(P_VOL));
-    if (p->temperature) wrv(p->temperature, dt, i, get(P_TEMP));
-    if (p->plastic) wrv(p->plastic, dt, i, get(P_EP));
-    if (p->cached_tau) {
-      const double t[6] = {get(P_T0), get(P_T0 + 1), get(P_T0 + 2), get(P_T0 + 3), get(P_T0 + 4), get(P_T0 + 5)};
-      const double full[9] = {t[0], t[3], t[4], t[3], t[1], t[5], t[4], t[5], t[2]};
-      for (int k = 0; k < 9; ++k) wrv(p->cached_tau, dt, (size_t)i * 9 + k, full[k]);
-    }
-    if (p->material) p->material[i] = (uint8_t)(meta[s] & 255u);
-    if (p->pinned) p->pinned[i] = (meta[s] >> 8) & FLAG_PINNED ? 1 : 0;
-    if (p->cache_valid) p->cache_valid[i] = (meta[s] >> 8) & FLAG_CACHE ? 1 : 0;
-  }
+  unpack_particles(planes.data(), cap, 0, ids.data(), meta.data(), n, 0, p);
   return MPM_OK;
 }
 
@@ -512,31 +394,31 @@ int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
     if (rc) return rc;
   }
   {
-    Timed t(e, "p2g");
-    launch_fused(e->stream, e->d, e->n, e->g, sp, 0);
+    Timed t(e, "p2g", true);
+    launch_fused(e->stream, e->d, e->n, e->g, sp, 0, t.ev());
   }
   for (int s = 0; s < nsteps; ++s) {
     {
-      Timed t(e, "grid");
-      launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx);
+      Timed t(e, "grid", true);
+      launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx, t.ev());
     }
     e->steps_since_sort++;
     const bool last = s == nsteps - 1;
     const bool resort = !last && e->steps_since_sort >= e->cfg.sort_every;
     if (last || resort) {
       {
-        Timed t(e, "g2p");
-        launch_fused(e->stream, e->d, e->n, e->g, sp, 2);
+        Timed t(e, "g2p", true);
+        launch_fused(e->stream, e->d, e->n, e->g, sp, 2, t.ev());
       }
       if (resort) {
         int rc = rebin(e);
         if (rc) return rc;
-        Timed t(e, "p2g");
-        launch_fused(e->stream, e->d, e->n, e->g, sp, 0);
+        Timed t(e, "p2g", true);
+        launch_fused(e->stream, e->d, e->n, e->g, sp, 0, t.ev());
       }
     } else {
-      Timed t(e, "g2p2g");
-      launch_fused(e->stream, e->d, e->n, e->g, sp, 1);
+      Timed t(e, "g2p2g", true);
+      launch_fused(e->stream, e->d, e->n, e->g, sp, 1, t.ev());
     }
   }
   HIP_OK(e, hipGetLastError());
@@ -591,37 +473,7 @@ int mpm_conservation_report(mpm_engine* e, mpm_conservation* out) {
   hipFreeAsync(dbuf, e->stream);
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "conservation: %s", hipGetErrorString(st));
-  std::memset(out, 0, sizeof *out);
-  double acc[16] = {0};
-  double minJ = 1.0, maxv = 0;
-  for (int b = 0; b < nb; ++b) {
-    for (int k = 0; k < 11; ++k) acc[k] += h[(size_t)b * K + k];
-    minJ = std::min(minJ, h[(size_t)b * K + 11]);
-    maxv = std::max(maxv, h[(size_t)b * K + 12]);
-  }
-  out->particle_mass = acc[0];
-  out->particle_mass_q = e->particle_mass_q;
-  int64_t gq = 0;
-  double gp[3] = {0, 0, 0};
-  for (uint32_t b = 0; b < gb; ++b) {
-    int64_t q;
-    std::memcpy(&q, &e->red_host[4 * b], sizeof q);
-    gq += q;
-    for (int c = 0; c < 3; ++c) gp[c] += e->red_host[4 * b + 1 + c];
-  }
-  out->grid_mass_q = gq;
-  out->mass_scale = e->mass_scale;
-  out->grid_mass = (double)gq / e->mass_scale;
-  for (int k = 0; k < 3; ++k) {
-    out->momentum[k] = acc[2 + k];
-    out->angular[k] = acc[5 + k];
-    out->grid_momentum[k] = gp[k];
-  }
-  out->kinetic = acc[8];
-  out->affine = acc[9];
-  out->nonfinite = (int64_t)acc[10];
-  out->min_J = e->n ? minJ : 1.0;
-  out->max_speed = maxv;
+  finish_conservation(h.data(), K, nb, e->red_host, 0, gb, e->particle_mass_q, e->mass_scale, e->n, out);
   return MPM_OK;
 }
 
@@ -787,6 +639,10 @@ int mpm_set_surrogate(mpm_engine* e, int32_t material, const float* params, cons
                 "surrogate: material %d (model %d) is not rate-independent J2 without hardening", material, m.model);
   if (!(norm->in_scale > 0) || !(norm->out_scale > 0) || !(norm->eps_max > 0) || !(norm->T_hi >= norm->T_lo))
     return fail(e, MPM_ERR_INVALID_ARGUMENT, "surrogate: invalid normalisation");
+  if (mpm_mlp_check_material(&m, norm) != MPM_OK)
+    return fail(e, MPM_ERR_INVALID_ARGUMENT,
+                "surrogate: material %d differs from the network's training material (model, mu/E, lambda/E, "
+                "yield ratio, thermal softening)", material);
   for (int k = 0; k < MPM_MLP_NPARAM; ++k)
     if (!std::isfinite(params[k])) return fail(e, MPM_ERR_INVALID_ARGUMENT, "surrogate: parameter %d not finite", k);
   MlpDev* host = new MlpDev;

@@ -21,8 +21,11 @@
 // MODE 1: G2P of this step and P2G of the next (steady state); only x, F
 //         and ep are written back, v and C stay in registers.
 // MODE 2: G2P only, writing the full state (last step of a call).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "mpm_internal.h"
 #include "mpm_mlp_device.h"
@@ -30,18 +33,25 @@
 namespace mpm {
 namespace {
 
-constexpr int FB = 256;            // threads per workgroup = particles per chunk
-constexpr int PATCH_ACC = 512;     // LDS node box for the scatter (20 B per node)
-constexpr int PATCH_VEL = 2048;    // LDS node box for the gather (16 B per node)
-constexpr int ACC_BYTES = 20 * PATCH_ACC;
+constexpr int FB = 256;            // particles per chunk of the single-scene launch
 constexpr int TR_ROWS = 36;        // per-wave transpose: 9 node masses + 27 momentum rows
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 word: rows start in different banks
 constexpr int CHUNK = 8;           // lanes per scatter chunk (run pieces summed by one lane)
 constexpr int WAVE_BYTES = TR_ROWS * TR_STRIDE * 4 + 64 * 16;
-constexpr int LDS_BYTES = ACC_BYTES + (FB / 64) * WAVE_BYTES;
-static_assert(LDS_BYTES >= 16 * PATCH_VEL, "gather box must fit the shared buffer");
-static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0 && ACC_BYTES % 16 == 0,
-              "surrogate activations must fit the per-wave buffer");
+static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0, "surrogate activations must fit the wave buffer");
+
+// LDS plan of a workgroup of FBK particles (one per lane): a node box for the
+// scatter (20 B per node, 2 nodes per particle) and one transpose buffer per
+// wave; the gather box (16 B per node) reuses the whole buffer.
+template <int FBK>
+struct Shape {
+  static constexpr int WAVES = FBK / 64;
+  static constexpr int PATCH_ACC = 2 * FBK;
+  static constexpr int ACC_BYTES = 20 * PATCH_ACC;
+  static constexpr int LDS_BYTES = ACC_BYTES + WAVES * WAVE_BYTES;
+  static constexpr int PATCH_VEL = LDS_BYTES / 16;
+  static_assert(ACC_BYTES % 16 == 0, "wave buffers must stay 16-byte aligned");
+};
 
 __device__ __forceinline__ float ld(const float* P, uint32_t cap, int k, uint32_t i) {
   return P[(size_t)k * cap + i];
@@ -60,6 +70,7 @@ __device__ __forceinline__ int base_axis(float x, float O, float invH, int N, fl
 
 // Block-wide box of base cells: lo[a] = min b[a], hi[a] = max b[a] over the
 // lanes with `use` set. Returns false when no lane is in use.
+template <int WAVES>
 __device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], int hi[3], int* s_box) {
   int mn[3], mx[3];
 #pragma unroll
@@ -90,7 +101,7 @@ __device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], i
     lo[a] = s_box[a];
     hi[a] = s_box[3 + a];
 #pragma unroll
-    for (int w = 1; w < FB / 64; ++w) {
+    for (int w = 1; w < WAVES; ++w) {
       lo[a] = min(lo[a], s_box[w * 6 + a]);
       hi[a] = max(hi[a], s_box[w * 6 + 3 + a]);
     }
@@ -105,23 +116,25 @@ __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restri
   return mats[id];
 }
 
-template <int MODE, bool SUR>
-__global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
-                                              uint32_t n, const DevMaterial* __restrict__ mats, GridDims g,
-                                              StepParams sp, const float4* __restrict__ vel,
-                                              unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-                                              uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
-                                              const MlpDev* __restrict__ mlp,
-                                              unsigned long long* __restrict__ sur_counts) {
-  __shared__ __align__(16) unsigned char s_raw[LDS_BYTES];
-  __shared__ int s_box[6 * (FB / 64)];
+// The particle work of one chunk of FBK cell-sorted particles in slots
+// [first, min(first + FBK, end)). Grid pointers are those of the chunk's
+// scene; momentum planes are `pstride` floats apart.
+template <int MODE, bool SUR, int FBK>
+__device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
+                                            uint32_t first, uint32_t end, const DevMaterial* __restrict__ mats,
+                                            const GridDims& g, const StepParams& sp, size_t pstride,
+                                            const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m,
+                                            float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
+                                            uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp,
+                                            unsigned long long* __restrict__ sur_counts, unsigned char* s_raw,
+                                            int* s_box) {
+  using S = Shape<FBK>;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
-  const uint32_t i = blockIdx.x * FB + tid;
-  const bool valid = i < n;
-  const uint32_t ic = valid ? i : 0u;
+  const uint32_t i = first + tid;
+  const bool valid = i < end;
+  const uint32_t ic = valid ? i : first;
   const int N = g.N;
-  const size_t Nn = (size_t)N * N * N;
   const float H = g.H, invH = g.invH, dt = sp.dt;
 
   const uint32_t mt = meta[ic];
@@ -166,13 +179,13 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
     }
     int lo[3], hi[3];
     const bool gather = valid && !pinned;
-    block_box(b, gather, lo, hi, s_box);
+    block_box<S::WAVES>(b, gather, lo, hi, s_box);
     const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
-    const bool staged = hi[0] >= 0 && ex * ey * ez <= PATCH_VEL;
+    const bool staged = hi[0] >= 0 && ex * ey * ez <= S::PATCH_VEL;
     float4* s_vel = reinterpret_cast<float4*>(s_raw);
     if (staged) {
       const int cnt = ex * ey * ez;
-      for (int p = tid; p < cnt; p += FB) {
+      for (int p = tid; p < cnt; p += FBK) {
         const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
         s_vel[p] = vel[((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx)];
       }
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
         Tp = ld(P, cap, P_TEMP, ic);
         ep = ld(P, cap, P_EP, ic);
       }
-      const int br = surrogate_return(mlp, mine, Fn, ep, Tp, mat, tau, s_raw + ACC_BYTES + (tid >> 6) * WAVE_BYTES);
+      const int br = surrogate_return(mlp, mine, Fn, ep, Tp, mat, tau, s_raw + S::ACC_BYTES + (tid >> 6) * WAVE_BYTES);
       if (br == SUR_NETWORK || br == SUR_ELASTIC) {
         new_flags |= FLAG_CACHE;
         F = Fn;
@@ -350,18 +363,18 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   const int nchunks = __popcll(chunk_starts);
 
   int lo[3], hi[3];
-  block_box(b, valid, lo, hi, s_box);
+  block_box<S::WAVES>(b, valid, lo, hi, s_box);
   const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
   const int cnt = ex * ey * ez;
-  const bool staged = hi[0] >= 0 && cnt <= PATCH_ACC;
+  const bool staged = hi[0] >= 0 && cnt <= S::PATCH_ACC;
   unsigned long long* s_m = reinterpret_cast<unsigned long long*>(s_raw);
-  float* s_p = reinterpret_cast<float*>(s_raw + 8 * PATCH_ACC);
+  float* s_p = reinterpret_cast<float*>(s_raw + 8 * S::PATCH_ACC);
   if (staged) {
-    for (int p = tid; p < cnt; p += FB) {
+    for (int p = tid; p < cnt; p += FBK) {
       s_m[p] = 0ull;
       s_p[p] = 0.f;
-      s_p[PATCH_ACC + p] = 0.f;
-      s_p[2 * PATCH_ACC + p] = 0.f;
+      s_p[S::PATCH_ACC + p] = 0.f;
+      s_p[2 * S::PATCH_ACC + p] = 0.f;
     }
   }
   const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
@@ -370,7 +383,7 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   // Per-wave LDS: 36 transpose rows (9 node masses, 27 node momenta) of 64
   // particles, and the chunk table {first lane, last lane, box base, grid
   // base (-1 when the chunk holds no particle)}.
-  unsigned char* wbuf = s_raw + ACC_BYTES + (tid >> 6) * WAVE_BYTES;
+  unsigned char* wbuf = s_raw + S::ACC_BYTES + (tid >> 6) * WAVE_BYTES;
   float* s_tr = reinterpret_cast<float*>(wbuf);
   int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ROWS * TR_STRIDE * 4);
   if ((chunk_starts >> lane) & 1ull) {
@@ -437,14 +450,14 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
           if (row < 9)
             atomicAdd(&s_m[ln], (unsigned long long)(unsigned int)si32);
           else
-            atomicAdd(&s_p[c * PATCH_ACC + ln], sf);
+            atomicAdd(&s_p[c * S::PATCH_ACC + ln], sf);
         } else {
           const size_t node = (size_t)(uint32_t)ch.w + ((size_t)k * N + j) * N + ii;
           if (row < 9) {
             atomicAdd(&acc_m[node], (unsigned long long)(unsigned int)si32);
             pieces[node >> 6] = 1;
           } else {
-            atomicAdd(&acc_p[(size_t)c * Nn + node], sf);
+            atomicAdd(&acc_p[(size_t)c * pstride + node], sf);
           }
         }
       }
@@ -455,19 +468,66 @@ __global__ __launch_bounds__(FB) void k_fused(float* __restrict__ P, uint32_t* _
   }
   if (staged) {
     __syncthreads();
-    for (int p = tid; p < cnt; p += FB) {
+    for (int p = tid; p < cnt; p += FBK) {
       const unsigned long long mq = s_m[p];
-      const float px = s_p[p], py = s_p[PATCH_ACC + p], pz = s_p[2 * PATCH_ACC + p];
+      const float px = s_p[p], py = s_p[S::PATCH_ACC + p], pz = s_p[2 * S::PATCH_ACC + p];
       if (mq != 0ull || px != 0.f || py != 0.f || pz != 0.f) {
         const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
         const size_t node = ((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx);
         atomicAdd(&acc_m[node], mq);
         atomicAdd(&acc_p[node], px);
-        atomicAdd(&acc_p[Nn + node], py);
-        atomicAdd(&acc_p[2 * Nn + node], pz);
+        atomicAdd(&acc_p[pstride + node], py);
+        atomicAdd(&acc_p[2 * pstride + node], pz);
         pieces[node >> 6] = 1;
       }
     }
+  }
+}
+
+// Single scene: one chunk of FB particles per workgroup. Three workgroups
+// per CU is the LDS limit (3 x 51.8 KB of 160 KB); the register budget is
+// held to the same occupancy so the surrogate variant's MFMA operands do not
+// cost a wave.
+template <int MODE, bool SUR>
+__global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused(
+    float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, uint32_t n,
+    const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
+    unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
+    uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts) {
+  __shared__ __align__(16) unsigned char s_raw[Shape<FB>::LDS_BYTES];
+  __shared__ int s_box[6 * Shape<FB>::WAVES];
+  fused_chunk<MODE, SUR, FB>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel, acc_m,
+                             acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box);
+}
+
+// Many scenes in one persistent launch: each workgroup takes chunks from a
+// shared counter until the list is exhausted (every workgroup reaches the
+// exit); the workgroup that takes the last ticket resets the counter for the
+// next launch, so no clearing operation sits between launches.
+template <int MODE, int FBK>
+__global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_batch(
+    float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
+    BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
+    uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces) {
+  __shared__ __align__(16) unsigned char s_raw[Shape<FBK>::LDS_BYTES];
+  __shared__ int s_box[6 * Shape<FBK>::WAVES];
+  __shared__ uint32_t s_ticket;
+  for (;;) {
+    __syncthreads();  // the previous chunk is done with LDS and s_ticket
+    if (threadIdx.x == 0) {
+      const uint32_t t = atomicAdd(b.work, 1u);
+      if (t == b.nchunks + gridDim.x - 1) atomicExch(b.work, 0u);
+      s_ticket = t;
+    }
+    __syncthreads();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(s_ticket);
+    if (t >= b.nchunks) break;
+    const uint2 ch = b.chunks[t];
+    const SceneDesc* sd = b.scenes + ch.x;
+    const uint32_t node0 = sd->node0;
+    fused_chunk<MODE, false, FBK>(P, meta, cap, ch.y, sd->p0 + sd->n, mats, sd->g, sd->sp, b.node_stride,
+                                  vel + node0, acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> 6),
+                                  nullptr, nullptr, s_raw, s_box);
   }
 }
 
@@ -475,12 +535,13 @@ inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
 
 }  // namespace
 
-void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode) {
+void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode,
+                  LaunchEvents ev) {
   if (!n) return;
   const dim3 grid(fblocks(n)), block(FB);
 #define MPM_FUSED(M, S)                                                                                          \
-  hipLaunchKernelGGL((k_fused<M, S>), grid, block, 0, s, d.planes, d.meta, d.cap, n, d.materials, g, p, d.vel, \
-                     d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts)
+  hipExtLaunchKernelGGL((k_fused<M, S>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n,   \
+                        d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts)
   const bool sur = d.mlp != nullptr;
   if (mode == 0)
     MPM_FUSED(0, false);
@@ -493,6 +554,31 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
   else
     MPM_FUSED(2, false);
 #undef MPM_FUSED
+}
+
+void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, uint32_t particle_blocks, int mode,
+                        int fb, LaunchEvents ev) {
+  if (!b.nchunks) return;
+  const dim3 grid(std::min(particle_blocks, b.nchunks));
+#define MPM_FUSED_B(M, K)                                                                                           \
+  hipExtLaunchKernelGGL((k_fused_batch<M, K>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, \
+                        d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces)
+#define MPM_FUSED_M(M)        \
+  if (fb == 64)               \
+    MPM_FUSED_B(M, 64);       \
+  else if (fb == 128)         \
+    MPM_FUSED_B(M, 128);      \
+  else                        \
+    MPM_FUSED_B(M, 256)
+  if (mode == 0) {
+    MPM_FUSED_M(0);
+  } else if (mode == 1) {
+    MPM_FUSED_M(1);
+  } else {
+    MPM_FUSED_M(2);
+  }
+#undef MPM_FUSED_M
+#undef MPM_FUSED_B
 }
 
 }  // namespace mpm

@@ -77,16 +77,53 @@ struct DevState {
 
 constexpr int SUR_SLOTS = 64;
 
+// One scene of a batch: its grid and step parameters, its particle slot
+// range [p0, p0 + n) and the first node / grid block of its grid in the
+// concatenated grid arrays (node0 is a multiple of 256, one grid block).
+struct SceneDesc {
+  GridDims g;
+  StepParams sp;
+  uint32_t p0, n;
+  uint32_t node0, block0;
+  int lo_idx[3], hi_idx[3];
+};
+
+// Particle-kernel arguments of a batched launch: the scene table, the chunk
+// list {scene, first slot} (chunks never straddle scenes), the plane stride of
+// the momentum accumulators (total nodes) and the self-resetting work
+// counter of the persistent launch.
+struct BatchArgs {
+  const SceneDesc* scenes;
+  const uint2* chunks;
+  uint32_t nchunks;
+  uint32_t node_stride;
+  unsigned int* work;
+};
+
+// Optional events that a launch stamps with its own start and end (profiling
+// without barrier packets between launches).
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // Kernel launchers (mpm_kernels.hip)
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
 void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
-                 const int hi_idx[3]);
+                 const int hi_idx[3], LaunchEvents ev = {});
 // Fused particle kernel (mpm_fused.hip): mode 0 P2G from the stored state,
 // 1 G2P of this step + P2G of the next, 2 G2P writing the full state. With
 // d.mlp set, the G2P modes take the surrogate variant.
-void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode);
+void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode,
+                  LaunchEvents ev = {});
 uint32_t grid_blocks(const GridDims& g);
+// Batched launchers (mpm_kernels.hip, mpm_fused.hip): all scenes of a batch
+// in one launch; `particle_blocks` is the persistent grid size.
+void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b);
+void launch_grid_batch(hipStream_t s, const DevState& d, const SceneDesc* scenes, const uint32_t* block_scene,
+                       uint32_t nblocks, uint32_t node_stride, LaunchEvents ev = {});
+void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, uint32_t particle_blocks, int mode,
+                        int fb, LaunchEvents ev = {});
 // Trained parameters packed into the device record (mpm_mlp.hip).
 void pack_mlp(const float* params, const MlpNorm& norm, MlpDev& out);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);

@@ -3,6 +3,7 @@
 // Grid update, binning, permutation, reductions and batch constitutive
 // kernels. The particle transfers are in mpm_fused.hip.
 //   k_grid  v = p/m, gravity, wall rules (src/three/mpm.js:89-102)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -49,6 +50,29 @@ __global__ __launch_bounds__(BLOCK) void k_bin_keys(const float* __restrict__ P,
   vals[id] = i;
 }
 
+// Batched keys: chunk -> scene; the key is the base cell's node index in the
+// concatenated grid (node0 + cell), so one sort orders scenes by node0 and
+// cells within each scene.
+__global__ __launch_bounds__(BLOCK) void k_bin_keys_batch(const float* __restrict__ P,
+                                                          const uint32_t* __restrict__ ids, uint32_t cap,
+                                                          BatchArgs b, uint32_t chunk_size,
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t c = blockIdx.x / (chunk_size / BLOCK);
+  if (c >= b.nchunks) return;
+  const uint2 ch = b.chunks[c];
+  const SceneDesc* sd = b.scenes + ch.x;
+  const uint32_t i = ch.y + (blockIdx.x % (chunk_size / BLOCK)) * BLOCK + threadIdx.x;
+  if (i >= sd->p0 + sd->n || i >= ch.y + chunk_size) return;
+  const GridDims& g = sd->g;
+  float t;
+  const int bx = base_of(plane(P, cap, P_X, i), g.O[0], g.invH, g.N, t);
+  const int by = base_of(plane(P, cap, P_Y, i), g.O[1], g.invH, g.N, t);
+  const int bz = base_of(plane(P, cap, P_Z, i), g.O[2], g.invH, g.N, t);
+  const uint32_t id = ids[i];
+  keys[id] = sd->node0 + ((uint32_t)bz * (uint32_t)g.N + (uint32_t)by) * (uint32_t)g.N + (uint32_t)bx;
+  vals[id] = i;
+}
+
 // Gather every plane of the particle state into sorted slot order.
 __global__ __launch_bounds__(BLOCK) void k_permute(const float* __restrict__ P, float* __restrict__ Q,
                                                    const uint32_t* __restrict__ ids, uint32_t* __restrict__ ids_out,
@@ -79,15 +103,17 @@ __device__ __forceinline__ void step_stress(const M3& F, const DevMaterial& m, u
 
 // One wave per 64-node piece of the dense grid; waves of inactive pieces do
 // no memory work. Each block writes one record of FP64 grid sums (fixed-point
-// mass, momentum) so the sums need no same-address atomics.
-__global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-                                                uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
-                                                float4* __restrict__ vel, GridDims g, StepParams sp,
-                                                int lo_idx0, int lo_idx1, int lo_idx2, int hi_idx0, int hi_idx1,
-                                                int hi_idx2, double* __restrict__ partials) {
+// mass, momentum) so the sums need no same-address atomics. `blk` is the
+// block's index within its scene's grid, `out` its record; the momentum
+// planes are `pstride` floats apart.
+__device__ __forceinline__ void grid_block(uint32_t blk, uint32_t out, unsigned long long* __restrict__ acc_m,
+                                           float* __restrict__ acc_p, size_t pstride, uint8_t* __restrict__ fixed,
+                                           uint8_t* __restrict__ pieces, float4* __restrict__ vel, const GridDims& g,
+                                           const StepParams& sp, const int lo_idx[3], const int hi_idx[3],
+                                           double* __restrict__ partials) {
   __shared__ double s_red[BLOCK / 64][4];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const size_t piece = (size_t)blockIdx.x * (BLOCK / 64) + wave;
+  const size_t piece = (size_t)blk * (BLOCK / 64) + wave;
   const size_t Nn = (size_t)g.N * g.N * g.N;
   const size_t npieces = (Nn + 63) / 64;
   const bool active = piece < npieces && pieces[piece] != 0;
@@ -97,7 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
   if (active && node < Nn) {
     const unsigned long long M = acc_m[node];
     Mq = (long long)M;
-    const float px = acc_p[node], py = acc_p[Nn + node], pz = acc_p[2 * Nn + node];
+    const float px = acc_p[node], py = acc_p[pstride + node], pz = acc_p[2 * pstride + node];
     p0 = px;
     p1 = py;
     p2 = pz;
@@ -112,8 +138,6 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
       const int N = g.N;
       const int ix = (int)(node % N), iy = (int)((node / N) % N), iz = (int)(node / ((size_t)N * N));
       const int idx[3] = {ix, iy, iz};
-      const int lo_idx[3] = {lo_idx0, lo_idx1, lo_idx2};
-      const int hi_idx[3] = {hi_idx0, hi_idx1, hi_idx2};
       bool sticky = fixed[node] != 0;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -131,8 +155,8 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
     vel[node] = make_float4(v[0], v[1], v[2], m);
     acc_m[node] = 0ull;
     acc_p[node] = 0.f;
-    acc_p[Nn + node] = 0.f;
-    acc_p[2 * Nn + node] = 0.f;
+    acc_p[pstride + node] = 0.f;
+    acc_p[2 * pstride + node] = 0.f;
     fixed[node] = 0;
   }
 #pragma unroll
@@ -157,9 +181,34 @@ __global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__
       m += __double_as_longlong(s_red[w][0]);
       for (int c = 0; c < 3; ++c) q[c] += s_red[w][1 + c];
     }
-    partials[blockIdx.x * 4 + 0] = __longlong_as_double(m);
-    for (int c = 0; c < 3; ++c) partials[blockIdx.x * 4 + 1 + c] = q[c];
+    partials[(size_t)out * 4 + 0] = __longlong_as_double(m);
+    for (int c = 0; c < 3; ++c) partials[(size_t)out * 4 + 1 + c] = q[c];
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
+                                                uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
+                                                float4* __restrict__ vel, GridDims g, StepParams sp,
+                                                int lo_idx0, int lo_idx1, int lo_idx2, int hi_idx0, int hi_idx1,
+                                                int hi_idx2, double* __restrict__ partials) {
+  const int lo_idx[3] = {lo_idx0, lo_idx1, lo_idx2};
+  const int hi_idx[3] = {hi_idx0, hi_idx1, hi_idx2};
+  grid_block(blockIdx.x, blockIdx.x, acc_m, acc_p, (size_t)g.N * g.N * g.N, fixed, pieces, vel, g, sp, lo_idx, hi_idx,
+             partials);
+}
+
+// Grid update of many scenes in one launch: block -> scene from a table; the
+// scenes' grids are concatenated (each starts on a block boundary).
+__global__ __launch_bounds__(BLOCK) void k_grid_batch(unsigned long long* __restrict__ acc_m,
+                                                      float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
+                                                      uint8_t* __restrict__ pieces, float4* __restrict__ vel,
+                                                      const SceneDesc* __restrict__ scenes,
+                                                      const uint32_t* __restrict__ block_scene, uint32_t node_stride,
+                                                      double* __restrict__ partials) {
+  const SceneDesc* sd = scenes + block_scene[blockIdx.x];
+  const uint32_t node0 = sd->node0;
+  grid_block(blockIdx.x - sd->block0, blockIdx.x, acc_m + node0, acc_p + node0, node_stride, fixed + node0,
+             pieces + (node0 >> 6), vel + node0, sd->g, sd->sp, sd->lo_idx, sd->hi_idx, partials);
 }
 
 __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
@@ -375,10 +424,24 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
 }
 
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
-                 const int hi_idx[3]) {
-  const size_t Nn = (size_t)g.N * g.N * g.N;
-  hipLaunchKernelGGL(k_grid, dim3(grid_blocks(g)), dim3(BLOCK), 0, s, d.acc_m, d.acc_p, d.fixed, d.pieces, d.vel,
-                     g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1], hi_idx[2], d.red);
+                 const int hi_idx[3], LaunchEvents ev) {
+  hipExtLaunchKernelGGL(k_grid, dim3(grid_blocks(g)), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
+                        d.fixed, d.pieces, d.vel, g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1],
+                        hi_idx[2], d.red);
+}
+
+void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b) {
+  // Chunks of the batch binning pass are BLOCK-aligned slices of the fused
+  // chunks: chunk_size is the largest fused chunk (256 slots).
+  constexpr uint32_t chunk = 256;
+  hipLaunchKernelGGL(k_bin_keys_batch, dim3(b.nchunks * (chunk / BLOCK)), dim3(BLOCK), 0, s, d.planes, d.ids, d.cap,
+                     b, chunk, d.keys, d.vals);
+}
+
+void launch_grid_batch(hipStream_t s, const DevState& d, const SceneDesc* scenes, const uint32_t* block_scene,
+                       uint32_t nblocks, uint32_t node_stride, LaunchEvents ev) {
+  hipExtLaunchKernelGGL(k_grid_batch, dim3(nblocks), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
+                        d.fixed, d.pieces, d.vel, scenes, block_scene, node_stride, d.red);
 }
 
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {

@@ -41,7 +41,7 @@ __device__ __forceinline__ void sym_eig(const float e[6], float V[9], float lam[
 // One training sample: log strain eps ~ N(0, sigma) per component, T uniform
 // in [T_lo, T_hi]; F = exp(eps) (the trial state with no rotation); the plain
 // return map of the step gives the Kirchhoff stress; the target is its
-// deviatoric part in units of out_scale. Inputs and targets feature-major.
+// deviatoric part in units of out_scale * E. Inputs and targets feature-major.
 __global__ __launch_bounds__(TB) void k_mlp_samples(uint32_t seed, uint32_t n, float sigma, float T_lo, float T_hi,
                                                     float dt, DevMaterial mat, MlpNorm nrm, float* __restrict__ X,
                                                     float* __restrict__ Y) {
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(TB) void k_mlp_samples(uint32_t seed, uint32_t n, f
 #pragma unroll
   for (int c = 0; c < 6; ++c) X[(size_t)c * n + i] = e[c] * nrm.in_scale;
   X[(size_t)6 * n + i] = (T - nrm.T_center) * nrm.T_scale;
-  const float inv = 1.f / nrm.out_scale;
+  const float inv = 1.f / (nrm.out_scale * mat.E);
 #pragma unroll
   for (int c = 0; c < 6; ++c) Y[(size_t)c * n + i] = dev[c] * inv;
 }
@@ -356,12 +356,32 @@ int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, m
   out->T_center = 0.5f * (c->T_lo + c->T_hi);
   out->T_scale = 2.f / (c->T_hi - c->T_lo);
   const float A = (m->model >= 10) ? m->th_A : m->pl_A;
-  out->out_scale = A * m->E;
+  out->out_scale = A;
   out->eps_max = c->eps_max > 0 ? c->eps_max : 4.f * c->strain_sigma;
   out->T_lo = c->T_lo;
   out->T_hi = c->T_hi;
   out->material = -1;
+  out->model = m->model;
+  out->mu_over_E = m->mu / m->E;
+  out->lambda_over_E = m->lambda / m->E;
+  out->A = A;
+  out->Tref = m->th_Tref;
+  out->Thot = m->th_Thot;
+  out->hotRatio = m->th_hotRatio;
   return MPM_OK;
+}
+
+// 0 when material m matches the training fingerprint of norm (E-normalised
+// parameters within a relative 1e-5), else MPM_ERR_INVALID_ARGUMENT.
+int mpm_mlp_check_material(const mpm_material* m, const mpm_mlp_norm* n) {
+  if (!m || !n) return MPM_ERR_INVALID_ARGUMENT;
+  auto close = [](float a, float b) { return std::fabs(a - b) <= 1e-5f * std::max(std::fabs(a), std::fabs(b)); };
+  const float A = (m->model >= 10) ? m->th_A : m->pl_A;
+  bool ok = m->model == n->model && m->E > 0 && close(m->mu / m->E, n->mu_over_E) &&
+            close(m->lambda / m->E, n->lambda_over_E) && close(A, n->A);
+  if (m->model == MPM_MODEL_THERMAL_J2)
+    ok = ok && close(m->th_Tref, n->Tref) && close(m->th_Thot, n->Thot) && close(m->th_hotRatio, n->hotRatio);
+  return ok ? MPM_OK : MPM_ERR_INVALID_ARGUMENT;
 }
 
 int mpm_mlp_samples(const mpm_material* m, const mpm_mlp_train_config* c, const mpm_mlp_norm* norm, uint32_t n,
@@ -524,6 +544,7 @@ int mpm_surrogate_return_batch(const mpm_material* m, const float* params, const
     return MPM_ERR_INVALID_ARGUMENT;
   if (!(m->model == MPM_MODEL_J2 || (m->model == MPM_MODEL_THERMAL_J2 && m->th_H == 0.f)))
     return MPM_ERR_UNSUPPORTED;
+  if (mpm_mlp_check_material(m, norm) != MPM_OK) return MPM_ERR_INVALID_ARGUMENT;
   if (!n) return MPM_OK;
   MlpDev* host = new MlpDev;
   pack_mlp(params, to_norm(*norm), *host);

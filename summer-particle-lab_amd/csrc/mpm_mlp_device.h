@@ -33,7 +33,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 struct MlpNorm {
   float in_scale;   // 1 / sigma of the strain components
   float T_center, T_scale;
-  float out_scale;  // stress unit of the outputs (cold yield stress)
+  float out_scale;  // output unit over E (the yield ratio A)
   float eps_max;    // envelope: |eps_ij| <= eps_max
   float T_lo, T_hi; // envelope: T_lo <= T <= T_hi
   int32_t material; // material index the surrogate replaces (-1: none)
@@ -291,8 +291,9 @@ __device__ __forceinline__ int surrogate_return(const MlpDev* __restrict__ M, bo
     for (int k = 0; k < 6; ++k) xin[k] = eps[k] * nm.in_scale;
     xin[6] = (T - nm.T_center) * nm.T_scale;
     mlp_forward_wave(M, use, xin, yout, wave_lds);
+    const float unit = nm.out_scale * mat.E;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) dtau[k] = yout[k] * nm.out_scale;
+    for (int k = 0; k < 6; ++k) dtau[k] = yout[k] * unit;
   }
   if (use) {
     ep += surrogate_apply(F, eps, lnJ, dtau, mat, tau);

@@ -53,23 +53,93 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
+                gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False):
+    cfg = L.MpmConfig()
+    cfg.lower[:] = list(map(float, lower))
+    cfg.upper[:] = list(map(float, upper))
+    cfg.cell_size = float(cell_size)
+    cfg.capacity = int(capacity)
+    cfg.wall_mode[:] = list(map(int, wall_mode))
+    cfg.gravity[:] = list(map(float, gravity))
+    cfg.gravity_on_particles = int(bool(gravity_on_particles))
+    cfg.mass_eps = float(mass_eps)
+    cfg.sort_every = int(sort_every)
+    cfg.strict_domain = int(bool(strict_domain))
+    return cfg
+
+
+def particle_view(x, v, mass, volume, *, F=None, Cmat=None, temperature=None, plastic=None, material=None,
+                  pinned=None, cached_tau=None, cache_valid=None, dtype=np.float64):
+    """An mpm_particles view of host arrays; returns (view, arrays to keep alive)."""
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    f = np.float64 if dtype == np.float64 else np.float32
+    cols = lambda a: [np.ascontiguousarray(a[:, k], f) for k in range(3)]  # noqa: E731
+    xs = cols(x)
+    vs = cols(np.broadcast_to(np.asarray(v, np.float64), (n, 3)))
+    F = np.tile(np.eye(3).reshape(9), n) if F is None else np.asarray(F).reshape(n * 9)
+    Cm = np.zeros(9 * n) if Cmat is None else np.asarray(Cmat).reshape(n * 9)
+    arr = lambda a, fill: np.ascontiguousarray(np.broadcast_to(np.asarray(fill if a is None else a, f), (n,)))  # noqa
+    keep = dict(
+        x=xs[0], y=xs[1], z=xs[2], vx=vs[0], vy=vs[1], vz=vs[2],
+        F=np.ascontiguousarray(F, f), C=np.ascontiguousarray(Cm, f),
+        mass=arr(mass, 0), volume=arr(volume, 0), temperature=arr(temperature, 20.0),
+        plastic=arr(plastic, 0.0),
+        material=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if material is None else material, np.uint8),
+                                                      (n,))),
+        pinned=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if pinned is None else pinned, np.uint8), (n,))),
+    )
+    if cached_tau is not None:
+        keep["cached_tau"] = np.ascontiguousarray(np.asarray(cached_tau).reshape(n * 9), f)
+        keep["cache_valid"] = np.ascontiguousarray(np.broadcast_to(np.asarray(cache_valid, np.uint8), (n,)))
+    p = L.MpmParticles()
+    p.count = n
+    p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
+    for k, a in keep.items():
+        setattr(p, k, _ptr(a))
+    return p, keep
+
+
+def download_view(n, dtype=np.float64):
+    """An mpm_particles view of fresh host arrays for n particles."""
+    f = dtype
+    out = dict(x=np.zeros(n, f), y=np.zeros(n, f), z=np.zeros(n, f), vx=np.zeros(n, f), vy=np.zeros(n, f),
+               vz=np.zeros(n, f), F=np.zeros(9 * n, f), C=np.zeros(9 * n, f), mass=np.zeros(n, f),
+               volume=np.zeros(n, f), temperature=np.zeros(n, f), plastic=np.zeros(n, f),
+               cached_tau=np.zeros(9 * n, f), material=np.zeros(n, np.uint8), pinned=np.zeros(n, np.uint8),
+               cache_valid=np.zeros(n, np.uint8))
+    p = L.MpmParticles()
+    p.count = n
+    p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
+    for k, a in out.items():
+        setattr(p, k, _ptr(a))
+    return p, out
+
+
+def finish_download(out):
+    out["positions"] = np.stack([out["x"], out["y"], out["z"]], 1)
+    out["velocities"] = np.stack([out["vx"], out["vy"], out["vz"]], 1)
+    return out
+
+
+def conservation_dict(r):
+    return dict(particle_mass=r.particle_mass, grid_mass=r.grid_mass, particle_mass_q=r.particle_mass_q,
+                grid_mass_q=r.grid_mass_q, mass_scale=r.mass_scale, momentum=np.array(r.momentum[:]),
+                grid_momentum=np.array(r.grid_momentum[:]), angular=np.array(r.angular[:]),
+                kinetic=r.kinetic, affine=r.affine, min_J=r.min_J, max_speed=r.max_speed,
+                nonfinite=r.nonfinite)
+
+
 class Engine:
     """One MLS-MPM engine on the current HIP device."""
 
     def __init__(self, lower, upper, cell_size, capacity, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
                  gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False):
         self._lib = L.load()
-        cfg = L.MpmConfig()
-        cfg.lower[:] = list(map(float, lower))
-        cfg.upper[:] = list(map(float, upper))
-        cfg.cell_size = float(cell_size)
-        cfg.capacity = int(capacity)
-        cfg.wall_mode[:] = list(map(int, wall_mode))
-        cfg.gravity[:] = list(map(float, gravity))
-        cfg.gravity_on_particles = int(bool(gravity_on_particles))
-        cfg.mass_eps = float(mass_eps)
-        cfg.sort_every = int(sort_every)
-        cfg.strict_domain = int(bool(strict_domain))
+        cfg = make_config(lower, upper, cell_size, capacity, wall_mode=wall_mode, gravity=gravity,
+                          gravity_on_particles=gravity_on_particles, mass_eps=mass_eps, sort_every=sort_every,
+                          strict_domain=strict_domain)
         h = C.c_void_p()
         L.check(self._lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -94,55 +164,15 @@ class Engine:
         arr = (L.MpmMaterial * len(materials))(*[m.to_c() for m in materials])
         self._check(self._lib.mpm_set_materials(self.h, arr, len(materials)))
 
-    def upload(self, x, v, mass, volume, *, F=None, Cmat=None, temperature=None, plastic=None, material=None,
-               pinned=None, cached_tau=None, cache_valid=None, dtype=np.float64):
-        x = np.asarray(x, np.float64)
-        n = len(x)
-        f = np.float64 if dtype == np.float64 else np.float32
-        cols = lambda a: [np.ascontiguousarray(a[:, k], f) for k in range(3)]
-        xs = cols(x)
-        vs = cols(np.broadcast_to(np.asarray(v, np.float64), (n, 3)))
-        F = np.tile(np.eye(3).reshape(9), n) if F is None else np.asarray(F).reshape(n * 9)
-        Cm = np.zeros(9 * n) if Cmat is None else np.asarray(Cmat).reshape(n * 9)
-        arr = lambda a, fill: np.ascontiguousarray(np.broadcast_to(np.asarray(fill if a is None else a, f), (n,)))
-        p = L.MpmParticles()
-        keep = dict(
-            x=xs[0], y=xs[1], z=xs[2], vx=vs[0], vy=vs[1], vz=vs[2],
-            F=np.ascontiguousarray(F, f), C=np.ascontiguousarray(Cm, f),
-            mass=arr(mass, 0), volume=arr(volume, 0), temperature=arr(temperature, 20.0),
-            plastic=arr(plastic, 0.0),
-            material=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if material is None else material, np.uint8),
-                                                          (n,))),
-            pinned=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if pinned is None else pinned, np.uint8),
-                                                        (n,))),
-        )
-        if cached_tau is not None:
-            keep["cached_tau"] = np.ascontiguousarray(np.asarray(cached_tau).reshape(n * 9), f)
-            keep["cache_valid"] = np.ascontiguousarray(np.broadcast_to(np.asarray(cache_valid, np.uint8), (n,)))
-        p.count = n
-        p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
-        for k, a in keep.items():
-            setattr(p, k, _ptr(a))
+    def upload(self, x, v, mass, volume, **kw):
+        p, keep = particle_view(x, v, mass, volume, **kw)
         self._check(self._lib.mpm_upload(self.h, C.byref(p)))
-        self.count = n
+        self.count = p.count
 
     def download(self, dtype=np.float64):
-        n = self.count
-        f = dtype
-        out = dict(x=np.zeros(n, f), y=np.zeros(n, f), z=np.zeros(n, f), vx=np.zeros(n, f), vy=np.zeros(n, f),
-                   vz=np.zeros(n, f), F=np.zeros(9 * n, f), C=np.zeros(9 * n, f), mass=np.zeros(n, f),
-                   volume=np.zeros(n, f), temperature=np.zeros(n, f), plastic=np.zeros(n, f),
-                   cached_tau=np.zeros(9 * n, f), material=np.zeros(n, np.uint8), pinned=np.zeros(n, np.uint8),
-                   cache_valid=np.zeros(n, np.uint8))
-        p = L.MpmParticles()
-        p.count = n
-        p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
-        for k, a in out.items():
-            setattr(p, k, _ptr(a))
+        p, out = download_view(self.count, dtype)
         self._check(self._lib.mpm_download(self.h, C.byref(p)))
-        out["positions"] = np.stack([out["x"], out["y"], out["z"]], 1)
-        out["velocities"] = np.stack([out["vx"], out["vy"], out["vz"]], 1)
-        return out
+        return finish_download(out)
 
     def step(self, dt, nsteps=1):
         self._check(self._lib.mpm_step(self.h, float(dt), int(nsteps)))
@@ -158,11 +188,7 @@ class Engine:
     def conservation(self):
         r = L.MpmConservation()
         self._check(self._lib.mpm_conservation_report(self.h, C.byref(r)))
-        return dict(particle_mass=r.particle_mass, grid_mass=r.grid_mass, particle_mass_q=r.particle_mass_q,
-                    grid_mass_q=r.grid_mass_q, mass_scale=r.mass_scale, momentum=np.array(r.momentum[:]),
-                    grid_momentum=np.array(r.grid_momentum[:]), angular=np.array(r.angular[:]),
-                    kinetic=r.kinetic, affine=r.affine, min_J=r.min_J, max_speed=r.max_speed,
-                    nonfinite=r.nonfinite)
+        return conservation_dict(r)
 
     def min_j(self):
         return self.conservation()["min_J"]
@@ -263,3 +289,93 @@ def sym6_to_full(t):
     t = np.asarray(t)
     return np.stack([t[..., 0], t[..., 3], t[..., 4], t[..., 3], t[..., 1], t[..., 5], t[..., 4], t[..., 5],
                      t[..., 2]], -1)
+
+
+class Batch:
+    """Many independent scenes on one GPU (mpm_batch_*): scene i has its own
+    domain, particle count and step size; all scenes share one material table
+    and step together."""
+
+    def __init__(self, configs, counts, *, sort_every=16, use_graph=False, particle_blocks=0):
+        self._lib = L.load()
+        cfgs = (L.MpmConfig * len(configs))(*configs)
+        cnt = np.ascontiguousarray(counts, np.uint32)
+        opt = L.MpmBatchOptions(int(sort_every), int(bool(use_graph)), int(particle_blocks))
+        h = C.c_void_p()
+        L.check(self._lib.mpm_batch_create(cfgs, cnt.ctypes.data_as(C.POINTER(C.c_uint32)), len(configs),
+                                           C.byref(opt), C.byref(h)))
+        self.h = h
+        self.counts = cnt
+        self.scenes = len(configs)
+
+    def close(self):
+        if self.h:
+            self._lib.mpm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        L.check(rc, batch=self.h)
+
+    def set_materials(self, materials):
+        arr = (L.MpmMaterial * len(materials))(*[m.to_c() for m in materials])
+        self._check(self._lib.mpm_batch_set_materials(self.h, arr, len(materials)))
+
+    def upload(self, scene, x, v, mass, volume, **kw):
+        p, keep = particle_view(x, v, mass, volume, **kw)
+        self._check(self._lib.mpm_batch_upload(self.h, int(scene), C.byref(p)))
+
+    def download(self, scene, dtype=np.float64):
+        p, out = download_view(int(self.counts[scene]), dtype)
+        self._check(self._lib.mpm_batch_download(self.h, int(scene), C.byref(p)))
+        return finish_download(out)
+
+    def set_dt(self, dt):
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (self.scenes,)))
+        self._check(self._lib.mpm_batch_set_dt(self.h, d.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def step(self, nsteps=1):
+        self._check(self._lib.mpm_batch_step(self.h, int(nsteps)))
+
+    def synchronize(self):
+        self._check(self._lib.mpm_batch_synchronize(self.h))
+
+    def conservation(self, scene):
+        r = L.MpmConservation()
+        self._check(self._lib.mpm_batch_conservation(self.h, int(scene), C.byref(r)))
+        return conservation_dict(r)
+
+    @staticmethod
+    def _tuning_dict(t):
+        return dict(buckets=t.buckets, bucket_edge=list(t.bucket_edge)[:t.buckets - 1],
+                    bucket_scenes=list(t.bucket_scenes)[:t.buckets],
+                    particles_per_group=list(t.particles_per_group)[:t.buckets], sort_every=t.sort_every,
+                    particle_blocks=t.particle_blocks, ms_per_step=t.ms_per_step,
+                    bucket_ms=list(t.bucket_ms)[:t.buckets])
+
+    def tuning(self):
+        t = L.MpmBatchTuning()
+        self._check(self._lib.mpm_batch_get_tuning(self.h, C.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, particles_per_group=None, sort_every=None, particle_blocks=None):
+        t = L.MpmBatchTuning()
+        self._check(self._lib.mpm_batch_get_tuning(self.h, C.byref(t)))
+        if particles_per_group is not None:
+            for k, v in enumerate(particles_per_group):
+                t.particles_per_group[k] = int(v)
+        if sort_every is not None:
+            t.sort_every = int(sort_every)
+        if particle_blocks is not None:
+            t.particle_blocks = int(particle_blocks)
+        self._check(self._lib.mpm_batch_set_tuning(self.h, C.byref(t)))
+
+    def tune(self, trial_steps=8):
+        t = L.MpmBatchTuning()
+        self._check(self._lib.mpm_batch_tune(self.h, int(trial_steps), C.byref(t)))
+        return self._tuning_dict(t)

@@ -73,11 +73,22 @@ class MpmMlpTrainConfig(C.Structure):
 
 
 class MpmMlpNorm(C.Structure):
-    _fields_ = [(k, C.c_float) for k in ("in_scale", "T_center", "T_scale", "out_scale", "eps_max", "T_lo",
-                                         "T_hi")] + [("material", C.c_int32)]
+    _fields_ = ([(k, C.c_float) for k in ("in_scale", "T_center", "T_scale", "out_scale", "eps_max", "T_lo", "T_hi")]
+                + [("material", C.c_int32), ("model", C.c_int32)]
+                + [(k, C.c_float) for k in ("mu_over_E", "lambda_over_E", "A", "Tref", "Thot", "hotRatio")])
 
 
 MLP_NPARAM = 9222
+
+
+class MpmBatchOptions(C.Structure):
+    _fields_ = [("sort_every", C.c_int32), ("use_graph", C.c_int32), ("particle_blocks", C.c_int32)]
+
+
+class MpmBatchTuning(C.Structure):
+    _fields_ = [("buckets", C.c_int32), ("bucket_edge", C.c_int32 * 4), ("bucket_scenes", C.c_int32 * 4),
+                ("particles_per_group", C.c_int32 * 4), ("sort_every", C.c_int32), ("particle_blocks", C.c_int32),
+                ("ms_per_step", C.c_double), ("bucket_ms", C.c_double * 4)]
 
 
 class MpmKernelTime(C.Structure):
@@ -111,6 +122,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_float)]),
     "mpm_mlp_default_norm": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig),
                                        C.POINTER(MpmMlpNorm)]),
+    "mpm_mlp_check_material": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpNorm)]),
     "mpm_mlp_samples": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig), C.POINTER(MpmMlpNorm),
                                   C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mpm_mlp_train": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig), C.POINTER(MpmMlpNorm),
@@ -121,6 +133,21 @@ SIGNATURES = {
     "mpm_surrogate_return_batch": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(C.c_float), C.POINTER(MpmMlpNorm),
                                              C.c_uint32, C.c_float] + [C.POINTER(C.c_float)] * 6
                                    + [C.POINTER(C.c_int32)]),
+    "mpm_batch_create": (C.c_int, [C.POINTER(MpmConfig), C.POINTER(C.c_uint32), C.c_int32,
+                                   C.POINTER(MpmBatchOptions), C.POINTER(_P)]),
+    "mpm_batch_destroy": (C.c_int, [_P]),
+    "mpm_batch_last_error": (C.c_char_p, [_P]),
+    "mpm_batch_scene_count": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]),
+    "mpm_batch_set_materials": (C.c_int, [_P, C.POINTER(MpmMaterial), C.c_int32]),
+    "mpm_batch_upload": (C.c_int, [_P, C.c_int32, C.POINTER(MpmParticles)]),
+    "mpm_batch_download": (C.c_int, [_P, C.c_int32, C.POINTER(MpmParticles)]),
+    "mpm_batch_set_dt": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mpm_batch_step": (C.c_int, [_P, C.c_int32]),
+    "mpm_batch_synchronize": (C.c_int, [_P]),
+    "mpm_batch_conservation": (C.c_int, [_P, C.c_int32, C.POINTER(MpmConservation)]),
+    "mpm_batch_get_tuning": (C.c_int, [_P, C.POINTER(MpmBatchTuning)]),
+    "mpm_batch_set_tuning": (C.c_int, [_P, C.POINTER(MpmBatchTuning)]),
+    "mpm_batch_tune": (C.c_int, [_P, C.c_int32, C.POINTER(MpmBatchTuning)]),
     "mpm_set_surrogate": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.POINTER(MpmMlpNorm)]),
     "mpm_surrogate_counts": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                        C.c_int32]),
@@ -150,7 +177,12 @@ def load():
     return _lib
 
 
-def check(rc, handle=None):
+def check(rc, handle=None, batch=None):
     if rc != MPM_OK:
-        msg = load().mpm_last_error(handle).decode() if handle else ""
+        if handle:
+            msg = load().mpm_last_error(handle).decode()
+        elif batch:
+            msg = load().mpm_batch_last_error(batch).decode()
+        else:
+            msg = ""
         raise MpmError(rc, msg)

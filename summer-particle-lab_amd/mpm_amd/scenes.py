@@ -186,20 +186,22 @@ SURROGATE_THERMAL = dict(Tref=300.0, Thot=1200.0, hotRatio=0.1, A=0.02, H=0.0, k
 
 
 def j2_cube(nu=0.3, E=1e5, rho=1000.0, seed=42, strength=0.2, T=(300.0, 900.0), thermal=None, steps=200,
-            drop=0.5):
+            squeeze=0.5):
     """The elastic-cube scene with a thermal-J2 material (SURROGATE_THERMAL),
-    per-particle temperature uniform in [300, 900] K and a downward start
-    velocity `drop` (m/s) onto the sticky floor, so that the impact stress
-    (about rho c drop, 5 kPa) exceeds the yield stress (0.8-2 kPa): the scene
-    of the learned constitutive surrogate."""
+    per-particle temperature uniform in [300, 900] K and a vertical squeeze:
+    start velocity v_y = -squeeze (y - y_c) / (side / 2), a uniaxial strain
+    rate of 3.3/s at squeeze = 0.5 m/s, so that the 200 steps reach about
+    6.7% strain against a yield strain of 0.8-2%: the scene of the learned
+    constitutive surrogate."""
     sc = elastic_cube(nu=nu, E=E, rho=rho, seed=seed, strength=strength)
     n = sc.n
     sc.name = "j2_cube"
-    sc.v = np.tile(np.array([0.0, -drop, 0.0]), (n, 1))
+    sc.v = np.zeros((n, 3))
+    sc.v[:, 1] = -squeeze * (sc.x[:, 1] - 0.19) / 0.15
     sc.materials = [Material.from_E_nu(10, E, nu, thermal=dict(thermal or SURROGATE_THERMAL))]
     sc.temperature = uniform(seed + 3, np.arange(n), 0, 0, T[0], T[1])
     sc.steps = steps
-    sc.info.update(T=T, thermal=dict(thermal or SURROGATE_THERMAL), drop=drop)
+    sc.info.update(T=T, thermal=dict(thermal or SURROGATE_THERMAL), squeeze=squeeze)
     return sc
 
 
@@ -246,3 +248,116 @@ def thermal_boxes(cells=128, nboxes=6, ppa=4, seed=6, side=(0.05, 0.2), T=(300.0
     return Scene("thermal_boxes", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
                  np.concatenate(vols), temp, np.zeros(n, np.uint8), [mat], dt, steps,
                  info=dict(boxes=[(c.tolist(), s) for c, s in boxes], rho=rho, spacing=spacing))
+
+
+def large_scene(cells=256, ppa=2, seed=11, nspheres=32, radius=(0.03, 0.08), T=(300.0, 1200.0), dt=1e-5,
+                steps=1000, rho=STEEL_RHO_SCALED, block=(0.84, 0.14, 0.84), drop=1.0):
+    """Large single scene: 256^3 cells, 2 particles per axis per cell; a
+    seeded dense thermal-J2 block (STEEL_THERMAL, material 0) on the floor
+    plus `nspheres` fixed-corotated elastic spheres (material 1, radii
+    uniform in `radius`, non-overlapping, falling at `drop` m/s) above it;
+    temperature from T[0] at x = 0 to T[1] at x = 1 with seeded noise of 30 K
+    (clipped to T); gravity -9.81, separating walls, dt = 1e-5, 1000 steps.
+    The density is scaled as in thermal_boxes (P-wave CFL 0.27 at h = 1/256,
+    dt = 1e-5). About 16M particles."""
+    h = 1.0 / cells
+    spacing = h / ppa
+    xs, vs, ms, vols, mats = [], [], [], [], []
+    bc = (0.5, 2 * h + block[1] / 2, 0.5)
+    pts, vp = geometry(bc, block, "box", spacing, seed, 0.2)
+    xs.append(pts)
+    vs.append(np.zeros_like(pts))
+    ms.append(np.full(len(pts), rho * vp))
+    vols.append(np.full(len(pts), vp))
+    mats.append(np.zeros(len(pts), np.uint8))
+    top = 2 * h + block[1]
+    spheres = []
+    attempt = 0
+    while len(spheres) < nspheres and attempt < 10000:
+        k = len(spheres)
+        r = uniform(seed, k, attempt, 0, radius[0], radius[1])
+        c = np.array([uniform(seed, k, attempt, 1, 0.05 + r, 0.95 - r),
+                      uniform(seed, k, attempt, 2, top + 2 * h + r, 0.95 - r),
+                      uniform(seed, k, attempt, 3, 0.05 + r, 0.95 - r)])
+        if all(np.linalg.norm(c - c2) > r + r2 + 2 * h for c2, r2 in spheres):
+            spheres.append((c, float(r)))
+        attempt += 1
+    for k, (c, r) in enumerate(spheres):
+        pts, vp = geometry(c, (2 * r, 2 * r, 2 * r), "ellipsoid", spacing, seed + 1 + k, 0.2)
+        xs.append(pts)
+        vs.append(np.tile(np.array([0.0, -drop, 0.0]), (len(pts), 1)))
+        ms.append(np.full(len(pts), rho * vp))
+        vols.append(np.full(len(pts), vp))
+        mats.append(np.ones(len(pts), np.uint8))
+    x = np.concatenate(xs)
+    n = len(x)
+    temp = T[0] + (T[1] - T[0]) * x[:, 0] + 30.0 * (uniform(seed + 5, np.arange(n), 0, 0, -1.0, 1.0))
+    temp = np.clip(temp, T[0], T[1])
+    materials = [Material.from_E_nu(10, 2e11, 0.3, thermal=dict(STEEL_THERMAL)),
+                 Material.from_E_nu(0, 2e11, 0.3)]
+    return Scene("large_scene", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
+                 np.concatenate(vols), temp, np.concatenate(mats), materials, dt, steps,
+                 info=dict(spheres=[(c.tolist(), r) for c, r in spheres], rho=rho, spacing=spacing, block=block))
+
+
+@dataclass
+class BatchScenes:
+    """Many scenes for one batch: per-scene configs (mpm_config), counts, step
+    sizes and particle arrays, plus the shared material table."""
+    scenes: list
+    materials: list
+    steps: int
+    info: dict = field(default_factory=dict)
+
+    @property
+    def counts(self):
+        return [sc.n for sc in self.scenes]
+
+    @property
+    def total(self):
+        return sum(self.counts)
+
+    def batch(self, sort_every=16, use_graph=False):
+        from .engine import Batch, make_config
+        cfgs = [make_config(sc.lower, sc.upper, sc.h, sc.n, wall_mode=sc.wall_mode, gravity=sc.gravity,
+                            gravity_on_particles=sc.gravity_on_particles) for sc in self.scenes]
+        b = Batch(cfgs, self.counts, sort_every=sort_every, use_graph=use_graph)
+        b.set_materials(self.materials)
+        for i, sc in enumerate(self.scenes):
+            b.upload(i, sc.x, sc.v, sc.mass, sc.volume, temperature=sc.temperature, material=sc.material)
+        b.set_dt([sc.dt for sc in self.scenes])
+        return b
+
+
+def batched_scenes(nscenes=256, seed=21, counts=(2_000, 200_000), grids=(32, 64, 96), steps=300, E=1e5, nu=0.3,
+                   rho=1000.0, vsigma=0.5, cfl=0.25):
+    """Independent scenes of uneven size: scene k has a particle count
+    log-uniform in `counts`, a unit domain with a grid edge drawn from
+    `grids` cells, one seeded box of particles (2 or more per axis per cell,
+    side so that the count fits, centred at random), a per-scene velocity
+    N(0, vsigma) per component, and material 0 (fixed-corotated) or 1 (thermal
+    J2 with SURROGATE_THERMAL, temperature uniform in [300, 900] K) with equal
+    odds; E = 1e5 Pa, nu = 0.3, rho = 1000, gravity -9.81, separating walls;
+    dt = cfl h / c_p per scene. 300 steps."""
+    materials = [Material.from_E_nu(0, E, nu), Material.from_E_nu(10, E, nu, thermal=dict(SURROGATE_THERMAL))]
+    lam, mu = materials[0].lam, materials[0].mu
+    cp = float(np.sqrt((lam + 2 * mu) / rho))
+    out = []
+    for k in range(nscenes):
+        n_target = float(np.exp(uniform(seed, k, 0, 0, np.log(counts[0]), np.log(counts[1]))))
+        cells = int(grids[min(int(uniform(seed, k, 0, 1, 0, len(grids))), len(grids) - 1)])
+        h = 1.0 / cells
+        per_axis = max(2, int(np.ceil(n_target ** (1 / 3) / (0.6 * cells))))
+        spacing = h / per_axis
+        side = n_target ** (1 / 3) * spacing
+        c = np.array([uniform(seed, k, 0, 2 + a, 0.1 + side / 2, 0.9 - side / 2) for a in range(3)])
+        pts, vp = geometry(c, (side, side, side), "box", spacing, seed + 1000 + k, 0.2)
+        n = len(pts)
+        mat = 1 if uniform(seed, k, 0, 5, 0, 1) < 0.5 else 0
+        vel = np.array([vsigma * normal(seed, 20_000 + k, 0, a) for a in range(3)], float)
+        temp = uniform(seed + 9, np.arange(n) + 1_000_000 * k, 0, 0, 300.0, 900.0) if mat == 1 else np.full(n, 300.0)
+        sc = Scene(f"batch_{k}", (0, 0, 0), (1, 1, 1), h, pts, np.tile(vel, (n, 1)), np.full(n, rho * vp),
+                   np.full(n, vp), temp, np.full(n, mat, np.uint8), materials, cfl * h / cp, steps,
+                   info=dict(cells=cells, per_axis=per_axis, side=side, material=mat))
+        out.append(sc)
+    return BatchScenes(out, materials, steps, info=dict(seed=seed, counts=counts, grids=grids, cp=cp))

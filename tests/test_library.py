@@ -22,7 +22,7 @@ def header_symbols():
 def test_library_exports_every_declared_symbol():
     lib = L.load()
     syms = header_symbols()
-    assert len(syms) >= 27
+    assert len(syms) >= 42
     for s in syms:
         assert hasattr(lib, s), s
     assert syms == set(L.SIGNATURES), syms ^ set(L.SIGNATURES)
@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layout_matches_c(tmp_path):
     structs = {"mpm_config": L.MpmConfig, "mpm_material": L.MpmMaterial, "mpm_particles": L.MpmParticles,
                "mpm_conservation": L.MpmConservation, "mpm_kernel_time": L.MpmKernelTime,
-               "mpm_mlp_train_config": L.MpmMlpTrainConfig, "mpm_mlp_norm": L.MpmMlpNorm}
+               "mpm_mlp_train_config": L.MpmMlpTrainConfig, "mpm_mlp_norm": L.MpmMlpNorm,
+               "mpm_batch_options": L.MpmBatchOptions, "mpm_batch_tuning": L.MpmBatchTuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_engine.h"', "int main(void){"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -91,8 +92,16 @@ def test_surrogate_entry_points_validate_on_the_host():
     n = L.MpmMlpNorm()
     assert lib.mpm_mlp_default_norm(C.byref(m), C.byref(c), C.byref(n)) == L.MPM_OK
     assert n.in_scale == pytest.approx(50.0) and n.T_center == 600.0 and n.T_scale == pytest.approx(1 / 300)
-    assert n.out_scale == pytest.approx(0.04) and n.eps_max == pytest.approx(0.08) and n.material == -1
+    assert n.out_scale == pytest.approx(0.02) and n.eps_max == pytest.approx(0.08) and n.material == -1
     assert (n.T_lo, n.T_hi) == (300.0, 900.0)
+    # the fingerprint is E-normalised: a material with another E but the same
+    # ratios matches, one with another yield ratio does not
+    assert lib.mpm_mlp_check_material(C.byref(m), C.byref(n)) == L.MPM_OK
+    m2 = L.MpmMaterial.from_buffer_copy(m)
+    m2.E, m2.mu, m2.lambda_ = m.E * 1e5, m.mu * 1e5, m.lambda_ * 1e5
+    assert lib.mpm_mlp_check_material(C.byref(m2), C.byref(n)) == L.MPM_OK
+    m2.th_A = 0.03
+    assert lib.mpm_mlp_check_material(C.byref(m2), C.byref(n)) == L.MPM_ERR_INVALID_ARGUMENT
     c.T_hi = 100.0
     assert lib.mpm_mlp_default_norm(C.byref(m), C.byref(c), C.byref(n)) == L.MPM_ERR_INVALID_ARGUMENT
     c.T_hi = 900.0
@@ -110,3 +119,21 @@ def test_surrogate_entry_points_validate_on_the_host():
     assert lib.mpm_set_surrogate(None, 0, P, C.byref(n)) == L.MPM_ERR_INVALID_ARGUMENT
     assert lib.mpm_surrogate_counts(None, None, None, None, 0) == L.MPM_ERR_INVALID_ARGUMENT
     assert lib.mpm_mlp_loss_grad(None, 0, None, None, None, None) == L.MPM_ERR_INVALID_ARGUMENT
+
+
+def test_batch_create_validates_on_the_host():
+    lib = L.load()
+    h = C.c_void_p()
+    cfg = (L.MpmConfig * 2)()
+    for c in cfg:
+        c.lower[:] = [0, 0, 0]
+        c.upper[:] = [1, 1, 1]
+        c.cell_size = 1 / 32
+    counts = (C.c_uint32 * 2)(100, 200)
+    assert lib.mpm_batch_create(cfg, counts, 0, None, C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_batch_create(None, counts, 2, None, C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    cfg[1].cell_size = 0.0  # invalid scene geometry
+    assert lib.mpm_batch_create(cfg, counts, 2, None, C.byref(h)) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_batch_destroy(None) == L.MPM_OK
+    assert lib.mpm_batch_step(None, 1) == L.MPM_ERR_INVALID_ARGUMENT
+    assert lib.mpm_batch_get_tuning(None, None) == L.MPM_ERR_INVALID_ARGUMENT
