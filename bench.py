@@ -56,6 +56,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--large", action="store_true",
+                   help="also run the large single scene (256^3 cells, ~16.5M particles), plain and surrogate")
+    p.add_argument("--large-steps", type=int, default=1000)
     p.add_argument("--batched-scenes", type=int, default=256)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                    help="first copy MB megabytes device to device (a kernel of known traffic for scaling "
@@ -150,6 +153,44 @@ def batched_leg(args):
     el_s = time.perf_counter() - t0
     out["one_by_one"] = dict(value=round(total * steps / el_s, 1), ms_per_step=round(el_s / steps * 1e3, 4))
     out["speedup_vs_one_by_one"] = round(el_s / el, 2)
+    return out
+
+
+def large_leg(args):
+    """The large single scene: plain step, then the surrogate on the J2 block."""
+    import numpy as np
+    import torch
+    from mpm_amd import scenes
+    from mpm_amd import surrogate as S
+    t = time.perf_counter()
+    sc = scenes.large_scene(steps=args.large_steps)
+    gen_s = time.perf_counter() - t
+    out = dict(workload="large_scene: 256^3 cells, J2 steel block + 32 fixed-corotated spheres, T 300-1200 K, "
+                        f"dt 1e-5, {sc.steps} steps", particles=sc.n, generation_s=round(gen_s, 1))
+    eng = sc.engine(sort_every=args.sort_every)
+    el = timed_steps(eng, sc, sc.steps, 10)
+    cons = eng.conservation()
+    assert cons["grid_mass_q"] == cons["particle_mass_q"] and cons["nonfinite"] == 0, cons
+    plain = sc.n * sc.steps / el
+    out["plain"] = dict(value=round(plain, 1), ms_per_step=round(el / sc.steps * 1e3, 4), min_J=cons["min_J"],
+                        max_speed=cons["max_speed"])
+    del eng
+    mat = sc.materials[0]
+    cfg = S.TrainConfig(strain_sigma=mat.thermal["A"], dt=sc.dt, T_lo=300.0, T_hi=1200.0)
+    P, norm, hist = S.train(mat, cfg)
+    eng = sc.engine(sort_every=args.sort_every)
+    eng.set_surrogate(0, P, norm)
+    eng.step(sc.dt, 10)
+    eng.surrogate_counts(reset=True)
+    el = timed_steps(eng, sc, sc.steps, 0)
+    counts = eng.surrogate_counts(reset=True)
+    cons = eng.conservation()
+    assert cons["grid_mass_q"] == cons["particle_mass_q"] and cons["nonfinite"] == 0, cons
+    tot = max(sum(counts.values()), 1)
+    out["surrogate"] = dict(value=round(sc.n * sc.steps / el, 1), ms_per_step=round(el / sc.steps * 1e3, 4),
+                            speedup_vs_plain=round(sc.n * sc.steps / el / plain, 3),
+                            branches={k: round(v / tot, 4) for k, v in counts.items()},
+                            training_loss_last=float(hist[-1]))
     return out
 
 
@@ -275,6 +316,7 @@ def main():
     del eng
     sur = None if args.no_surrogate else surrogate_leg(sc, args, value)
     batched = None if args.no_batched else batched_leg(args)
+    large = large_leg(args) if args.large else None
     cpu = None if args.no_cpu else cpu_baseline(sc, args.cpu_steps)
     out = {
         "metric": "particle-steps/s (P2G+grid+G2P), plain vs MLP-constitutive at <=1% stress err",
@@ -297,6 +339,7 @@ def main():
         "kernels": per_kernel,
         "surrogate": sur,
         "batched": batched,
+        "large": large,
     }
     print(json.dumps(out))
 

@@ -187,6 +187,46 @@ int mpm_synchronize(mpm_engine* e);
 /* Number of particles resident on the device. */
 int mpm_count(const mpm_engine* e, uint32_t* out);
 
+/* ---- frame wrapper ----
+ * One display frame of Simulation3.step for an all-solid scene
+ * (src/three/simulation.js:305-315) with its substeps (:330-388): while more
+ * than 1e-9 of dt remains and fewer than max_substeps were taken, a substep
+ * of h = min(remaining, stable step * time_step_scale) adds the gravity of
+ * the selected mode to unpinned particles (src/three/gravity-options.js:7-16),
+ * runs the MPM step, clamps unpinned positions to the bounds (pinned ones
+ * stay), rebuilds v = clamp((x - x_start) / h, +-velocity_limit) and, with
+ * anti_clump, applies regularizeSolids (src/three/solid-sampling.js:5-42) to
+ * particles of equal material index (the drop-in keys materials by part).
+ * Not covered: fluids, drag, the press, heat exchange. The engine's own
+ * gravity should be zero (the wrapper applies gravity itself). */
+enum { MPM_GRAVITY_NONE = 0, MPM_GRAVITY_UNIFORM = 1, MPM_GRAVITY_ATTRACT = 2, MPM_GRAVITY_REPEL = 3 };
+
+typedef struct {
+  double time_step_scale;      /* settings.timeStepScale, in [0.1, 1] */
+  int32_t max_substeps;        /* 48 in the reference */
+  double velocity_limit;       /* 160 in the reference */
+  double bounds_lo[3], bounds_hi[3]; /* s.boundsLo, s.boundsHi */
+  int32_t anti_clump;          /* settings.solidAntiClump */
+  double volume_unit;          /* particle volume in the spacing rule's unit per
+                                  engine unit (1/64: the reference's s.volume) */
+  int32_t gravity_mode;        /* MPM_GRAVITY_* */
+  double gravity_strength;     /* settings.gravityStrength */
+  double gravity_direction[3]; /* the unit vector of settings.gravityDirection */
+  double gravity_center[3];    /* settings.gravityCenter */
+  double gravity_softening;    /* settings.gravitySoftening */
+} mpm_frame_config;
+
+typedef struct {
+  int32_t substeps;
+  double advanced;             /* dt minus what remained */
+} mpm_frame_report;
+
+/* Fills c with the reference defaults: scale 1, 48 substeps, limit 160,
+ * anti-clump on, volume unit 1/64, uniform gravity down, strength 1,
+ * softening 8; bounds from the engine's domain. */
+int mpm_frame_defaults(const mpm_engine* e, mpm_frame_config* c);
+int mpm_advance_frame(mpm_engine* e, const mpm_frame_config* c, double dt, mpm_frame_report* out);
+
 /* ---- measurement and parity hooks (no reference counterpart) ---- */
 
 /* Turn per-kernel HIP-event timing on (1) or off (0); reset clears totals. */
@@ -251,6 +291,10 @@ typedef struct {
   float T_lo, T_hi;    /* T ~ U(T_lo, T_hi) */
   float dt;            /* step passed to the return map */
   float eps_max;       /* envelope half-width of every eps component; <= 0: 4 sigma */
+  float near_yield_fraction; /* share of samples drawn near the yield surface:
+                                |dev eps| uniform in [0.8, 2.5] times the yield
+                                strain at T, random direction, volumetric
+                                part N(0, sigma); the rest is N(0, sigma) */
 } mpm_mlp_train_config;
 
 typedef struct {
@@ -259,6 +303,10 @@ typedef struct {
                           the stress is output * out_scale * E */
   float eps_max;       /* envelope: |eps_ij| <= eps_max */
   float T_lo, T_hi;    /* envelope: T_lo <= T <= T_hi */
+  float min_yield_ratio; /* envelope: the network takes trial states with
+                            q >= min_yield_ratio * yield; states just past
+                            yield (where a 64-wide network is least
+                            accurate) take the plain return */
   int32_t material;    /* set by mpm_set_surrogate */
   /* the training material, in E-normalised form: a network applies to any
    * material with the same model, mu/E, lambda/E, yield ratio and thermal
@@ -268,8 +316,8 @@ typedef struct {
 } mpm_mlp_norm;
 
 /* Normalisation for material m and a training config: in_scale 1/sigma,
- * T mapped to [-1, 1], out_scale = A (the cold yield stress over E), and m's
- * E-normalised fingerprint. */
+ * T mapped to [-1, 1], out_scale = A (the cold yield stress over E),
+ * min_yield_ratio 1.2, and m's E-normalised fingerprint. */
 int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, mpm_mlp_norm* out);
 
 /* MPM_OK when material m matches norm's training fingerprint (E-normalised

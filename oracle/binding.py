@@ -62,6 +62,13 @@ THERMAL_DEFAULTS = dict(Tref=20.0, Thot=180.0, hotRatio=.08, A=.04, H=.04, rate=
 PLASTICITY_DEFAULTS = dict(A=.025, B=.08, n=.35, C=.025, rate0=1.0, Tref=20.0, Tm=1000.0, m=1.0)
 
 
+class OrcFrame(C.Structure):
+    _fields_ = [("time_step_scale", D), ("max_substeps", C.c_int32), ("velocity_limit", D),
+                ("bounds_lo", D * 3), ("bounds_hi", D * 3), ("anti_clump", C.c_int32), ("gravity_mode", C.c_int32),
+                ("gravity_strength", D), ("gravity_direction", D * 3), ("gravity_center", D * 3),
+                ("gravity_softening", D)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -106,6 +113,9 @@ def lib():
         L.orc_min_j.restype = D
         L.orc_min_j.argtypes = [C.POINTER(OrcScene)]
         L.orc_mechanics.argtypes = [C.POINTER(OrcScene), PD]
+        L.orc_advance_frame.restype = C.c_int
+        L.orc_advance_frame.argtypes = [C.c_void_p, C.POINTER(OrcFrame), D, PD]
+        L.orc_regularize.argtypes = [C.c_void_p, D, PD, PD]
         L.orc_mlp_param_count.restype = C.c_int
         L.orc_mlp_forward.argtypes = [PD, C.c_int, PD, PD]
         L.orc_mlp_loss_grad.restype = D
@@ -338,6 +348,25 @@ class Scene:
         rc = lib().orc_step(C.byref(self.s), dt)
         if rc != 0:
             raise ValueError("unsupported model in scene")
+
+    def advance_frame(self, dt, *, time_step_scale=1.0, max_substeps=48, velocity_limit=160.0, bounds=None,
+                      anti_clump=True, gravity_mode=1, gravity_strength=1.0, gravity_direction=(0, -1, 0),
+                      gravity_center=(0, 0, 0), gravity_softening=8.0):
+        """One Simulation3.step frame of an all-solid scene; returns (substeps,
+        advanced time)."""
+        f = OrcFrame()
+        f.time_step_scale, f.max_substeps, f.velocity_limit = time_step_scale, max_substeps, velocity_limit
+        lo, hi = bounds if bounds is not None else (self.lower, self.upper)
+        f.bounds_lo[:], f.bounds_hi[:] = list(map(float, lo)), list(map(float, hi))
+        f.anti_clump, f.gravity_mode, f.gravity_strength = int(anti_clump), int(gravity_mode), gravity_strength
+        f.gravity_direction[:] = list(map(float, gravity_direction))
+        f.gravity_center[:] = list(map(float, gravity_center))
+        f.gravity_softening = gravity_softening
+        adv = D(0)
+        n = lib().orc_advance_frame(C.byref(self.s), C.byref(f), dt, C.byref(adv))
+        if n < 0:
+            raise ValueError("unsupported model in scene")
+        return n, adv.value
 
     def p2g(self, dt):
         lib().orc_p2g(C.byref(self.s), dt)

@@ -816,6 +816,147 @@ void orc_mechanics(const OrcScene* s, double* out) {
   }
 }
 
+
+// ------------------------------------------------------------ frame wrapper
+// The all-solid branch of Simulation3.step / substep around MPM3.step
+// (src/three/simulation.js:305-315, 330-388) with regularizeSolids
+// (src/three/solid-sampling.js:5-42). Gravity modes as gravityAcceleration
+// (src/three/gravity-options.js:7-16): 0 none, 1 uniform, 2 attract,
+// 3 repel. No fluid, drag, press or thermal exchange.
+typedef struct {
+  double time_step_scale;
+  int32_t max_substeps;
+  double velocity_limit;
+  double bounds_lo[3], bounds_hi[3];
+  int32_t anti_clump;
+  int32_t gravity_mode;
+  double gravity_strength;
+  double gravity_direction[3];
+  double gravity_center[3];
+  double gravity_softening;
+} OrcFrame;
+
+static void gravity_at(const OrcFrame* f, double x, double y, double z, double a[3]) {
+  a[0] = a[1] = a[2] = 0;
+  if (f->gravity_mode == 1) {
+    for (int k = 0; k < 3; ++k) a[k] = f->gravity_direction[k] * f->gravity_strength * 24;
+  } else if (f->gravity_mode == 2 || f->gravity_mode == 3) {
+    const double d[3] = {f->gravity_center[0] - x, f->gravity_center[1] - y, f->gravity_center[2] - z};
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + f->gravity_softening * f->gravity_softening;
+    const double sc = (f->gravity_mode == 3 ? -1 : 1) * 24000 * f->gravity_strength / std::pow(r2, 1.5);
+    for (int k = 0; k < 3; ++k) a[k] = d[k] * sc;
+  }
+}
+
+// regularizeSolids: pairs of solid particles of one part closer than the mean
+// of their spacings 2.8 cbrt(clamp(volume det F, 1/64, 8)) are pushed apart
+// by (target - r) gain / (r (1/m_i + 1/m_j)) weighted by 1/m; the
+// displacements are scaled down together so the largest is at most 8 dt,
+// then applied to unpinned particles and clamped to the bounds. All pairs
+// are visited (the reference's neighbour cells cover every pair closer than
+// the largest spacing, 5.6).
+void orc_regularize(const OrcScene* s, double dt, const double* lo, const double* hi) {
+  const int n = s->count;
+  std::vector<double> d(3 * (size_t)n, 0.0), sp(n), im(n);
+  for (int i = 0; i < n; ++i) {
+    const double* F = s->F + 9 * (size_t)i;
+    const double J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+    sp[i] = 2.8 * std::cbrt(clampd(s->volume[i] * J, .015625, 8.0));
+    im[i] = s->pinned[i] ? 0.0 : 1.0 / s->mass[i];
+  }
+  const double gain = 1 - std::exp(-40 * dt);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      if (s->part[i] != s->part[j]) continue;
+      const double total = im[i] + im[j];
+      if (!total) continue;
+      double dx = s->x[i] - s->x[j], dy = s->y[i] - s->y[j], dz = s->z[i] - s->z[j];
+      double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+      const double target = (sp[i] + sp[j]) * .5;
+      if (r >= target) continue;
+      if (r < 1e-8) {
+        const uint32_t hash = ((uint32_t)(i + 1) * 73856093u) ^ ((uint32_t)(j + 1) * 19349663u);
+        dx = (hash & 1023) / 511.5 - 1;
+        dy = ((hash >> 10) & 1023) / 511.5 - 1;
+        dz = ((hash >> 20) & 1023) / 511.5 - 1;
+        r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const double q = target * gain / (r * total);
+        dx *= q;
+        dy *= q;
+        dz *= q;
+      } else {
+        const double q = (target - r) * gain / (r * total);
+        dx *= q;
+        dy *= q;
+        dz *= q;
+      }
+      d[3 * (size_t)i] += dx * im[i];
+      d[3 * (size_t)i + 1] += dy * im[i];
+      d[3 * (size_t)i + 2] += dz * im[i];
+      d[3 * (size_t)j] -= dx * im[j];
+      d[3 * (size_t)j + 1] -= dy * im[j];
+      d[3 * (size_t)j + 2] -= dz * im[j];
+    }
+  double scale = 1;
+  const double budget = 8 * dt;
+  for (int i = 0; i < n; ++i)
+    scale = std::max(scale, std::sqrt(d[3 * (size_t)i] * d[3 * (size_t)i] + d[3 * (size_t)i + 1] * d[3 * (size_t)i + 1] +
+                                      d[3 * (size_t)i + 2] * d[3 * (size_t)i + 2]) /
+                                budget);
+  for (int i = 0; i < n; ++i) {
+    if (s->pinned[i]) continue;
+    s->x[i] = clampd(s->x[i] + d[3 * (size_t)i] / scale, lo[0], hi[0]);
+    s->y[i] = clampd(s->y[i] + d[3 * (size_t)i + 1] / scale, lo[1], hi[1]);
+    s->z[i] = clampd(s->z[i] + d[3 * (size_t)i + 2] / scale, lo[2], hi[2]);
+  }
+}
+
+// One frame of dt: substeps h = min(remaining, stableStep * scale) while more
+// than 1e-9 remains and fewer than max_substeps were taken. Returns the
+// substep count; *advanced receives the time advanced.
+int orc_advance_frame(const OrcScene* s, const OrcFrame* f, double dt, double* advanced) {
+  const int n = s->count;
+  double remaining = dt;
+  int steps = 0;
+  std::vector<double> ox(n), oy(n), oz(n);
+  while (remaining > 1e-9 && steps < f->max_substeps) {
+    const double h = std::min(remaining, orc_stable_step(s, 0, 0) * f->time_step_scale);
+    for (int i = 0; i < n; ++i) {
+      ox[i] = s->x[i];
+      oy[i] = s->y[i];
+      oz[i] = s->z[i];
+      if (s->pinned[i]) continue;
+      double a[3];
+      gravity_at(f, s->x[i], s->y[i], s->z[i], a);
+      s->vx[i] += a[0] * h;
+      s->vy[i] += a[1] * h;
+      s->vz[i] += a[2] * h;
+    }
+    if (orc_step(s, h) != 0) return -1;
+    const double L = f->velocity_limit;
+    for (int i = 0; i < n; ++i) {
+      if (s->pinned[i]) {
+        s->x[i] = ox[i];
+        s->y[i] = oy[i];
+        s->z[i] = oz[i];
+      } else {
+        s->x[i] = clampd(s->x[i], f->bounds_lo[0], f->bounds_hi[0]);
+        s->y[i] = clampd(s->y[i], f->bounds_lo[1], f->bounds_hi[1]);
+        s->z[i] = clampd(s->z[i], f->bounds_lo[2], f->bounds_hi[2]);
+      }
+      s->vx[i] = clampd((s->x[i] - ox[i]) / h, -L, L);
+      s->vy[i] = clampd((s->y[i] - oy[i]) / h, -L, L);
+      s->vz[i] = clampd((s->z[i] - oz[i]) / h, -L, L);
+    }
+    if (f->anti_clump) orc_regularize(s, h, f->bounds_lo, f->bounds_hi);
+    remaining -= h;
+    ++steps;
+  }
+  if (advanced) *advanced = dt - remaining;
+  return steps;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------- learned surrogate

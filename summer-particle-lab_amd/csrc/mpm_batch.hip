@@ -68,6 +68,7 @@ struct mpm_batch {
   uint32_t* d_block_scene = nullptr;
   uint32_t total_n = 0, total_nodes = 0, total_blocks = 0, key_bits = 0;
   uint32_t particle_blocks = 0;
+  uint32_t grid_blocks = 1;
   int sort_every = 16;
   int steps_since_sort = 0;
   bool needs_rebin = true;
@@ -165,7 +166,8 @@ void particles(mpm_batch* b, int mode) {
 }
 
 void grid(mpm_batch* b) {
-  launch_grid_batch(b->stream, b->d, b->d_scenes, b->d_block_scene, b->total_blocks, b->total_nodes);
+  launch_compact(b->stream, b->d, b->total_nodes / 64, (uint32_t)b->S);
+  launch_grid_batch(b->stream, b->d, b->d_scenes, b->d_block_scene, b->total_nodes, b->grid_blocks);
 }
 
 // One re-bin period of P steps: the launch sequence a graph captures.
@@ -291,20 +293,23 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   const size_t hist_words = radix_hist_words(cap), part_words = radix_partial_words(cap);
   alloc((void**)&d.hist, sizeof(uint32_t) * hist_words);
   alloc((void**)&d.partials, sizeof(uint32_t) * part_words);
-  alloc((void**)&d.red, sizeof(double) * 4 * b->total_blocks);
+  alloc((void**)&d.red, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes);
+  alloc((void**)&d.piece_list, sizeof(uint32_t) * (Nt / 64));
+  alloc((void**)&d.piece_counters, sizeof(unsigned long long) * 3);
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
   alloc((void**)&b->d_scenes, sizeof(SceneDesc) * nscenes);
   alloc((void**)&b->d_block_scene, sizeof(uint32_t) * b->total_blocks);
   for (auto& bk : b->bucket) alloc((void**)&bk.work, sizeof(unsigned int));
   if (st == hipSuccess)
-    st = hipHostMalloc((void**)&b->red_host, sizeof(double) * 4 * std::max<uint32_t>(b->total_blocks, 1),
-                       hipHostMallocDefault);
+    st = hipHostMalloc((void**)&b->red_host, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes, hipHostMallocDefault);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, Nt / 64, b->stream);
-  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * b->total_blocks, b->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes, b->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.piece_counters, 0, sizeof(unsigned long long) * 3, b->stream);
+  b->grid_blocks = (uint32_t)std::max(1, std::min<int>(cus, (int)((Nt / 64 + 3) / 4)));
   if (st == hipSuccess) st = hipMemsetAsync(d.planes, 0, sizeof(float) * NPLANES * (size_t)cap, b->stream);
   for (auto& bk : b->bucket)
     if (st == hipSuccess) st = hipMemsetAsync(bk.work, 0, sizeof(unsigned int), b->stream);
@@ -342,7 +347,8 @@ int mpm_batch_destroy(mpm_batch* b) {
   DevState& d = b->d;
   void* ptrs[] = {d.planes, d.planes_alt, d.ids,   d.ids_alt,  d.meta,     d.meta_alt, d.acc_m, d.acc_p,
                   d.vel,    d.fixed,      d.pieces, d.keys,    d.keys_alt, d.vals,     d.vals_alt, d.hist,
-                  d.partials, d.red,      d.materials, b->d_scenes, b->d_bin_chunks, b->d_block_scene};
+                  d.partials, d.red,      d.materials, b->d_scenes, b->d_bin_chunks, b->d_block_scene,
+                  d.piece_list, d.piece_counters};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& bk : b->bucket) {
@@ -507,14 +513,15 @@ int mpm_batch_conservation(mpm_batch* b, int32_t scene, mpm_conservation* out) {
   launch_particle_reduce(b->stream, dd, sd.n, sd.sp.mass_scale, D, dbuf);
   std::vector<double> h((size_t)K * nb);
   hipError_t st = hipMemcpyAsync(h.data(), dbuf, sizeof(double) * K * nb, hipMemcpyDeviceToHost, b->stream);
-  const uint32_t gb1 = scene + 1 < b->S ? b->sc[scene + 1].block0 : b->total_blocks;
+  const size_t slot0 = (size_t)scene * SUM_SLOTS * 4;
   if (st == hipSuccess)
-    st = hipMemcpyAsync(b->red_host + 4 * (size_t)sd.block0, b->d.red + 4 * (size_t)sd.block0,
-                        sizeof(double) * 4 * (gb1 - sd.block0), hipMemcpyDeviceToHost, b->stream);
+    st = hipMemcpyAsync(b->red_host + slot0, b->d.red + slot0, sizeof(double) * 4 * SUM_SLOTS, hipMemcpyDeviceToHost,
+                        b->stream);
   (void)hipFreeAsync(dbuf, b->stream);
   if (st == hipSuccess) st = hipStreamSynchronize(b->stream);
   if (st != hipSuccess) return bfail(b, MPM_ERR_HIP, "conservation: %s", hipGetErrorString(st));
-  finish_conservation(h.data(), K, nb, b->red_host, sd.block0, gb1, b->mass_q[scene], sd.sp.mass_scale, sd.n, out);
+  finish_conservation(h.data(), K, nb, b->red_host + slot0, SUM_SLOTS, b->mass_q[scene], sd.sp.mass_scale, sd.n,
+                      out);
   return MPM_OK;
 }
 

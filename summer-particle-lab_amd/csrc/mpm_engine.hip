@@ -37,6 +37,12 @@ struct mpm_engine {
   int steps_since_sort = 0;
   std::string err;
   double* red_host = nullptr;  // pinned
+  uint32_t grid_blocks = 1;    // persistent workgroups of the grid update
+  // frame wrapper scratch (allocated on first use)
+  float* frame_ox = nullptr;
+  float* frame_disp = nullptr;
+  uint32_t* frame_cell_start = nullptr;
+  unsigned int* frame_max = nullptr;
   // profiling
   bool profiling = false;
   std::vector<PendingTiming> pending;
@@ -269,16 +275,23 @@ int mpm_create(const mpm_config* c, mpm_engine** out) {
   alloc((void**)&d.vals_alt, sizeof(uint32_t) * cap);
   alloc((void**)&d.hist, sizeof(uint32_t) * radix_hist_words(cap));
   alloc((void**)&d.partials, sizeof(uint32_t) * radix_partial_words(cap));
-  alloc((void**)&d.red, sizeof(double) * 4 * grid_blocks(g));
+  const uint32_t npieces = grid_pieces(g);
+  alloc((void**)&d.red, sizeof(double) * 4 * SUM_SLOTS);
+  alloc((void**)&d.piece_list, sizeof(uint32_t) * npieces);
+  alloc((void**)&d.piece_counters, sizeof(unsigned long long) * 3);
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
   if (st == hipSuccess)
-    st = hipHostMalloc((void**)&e->red_host, sizeof(double) * 4 * grid_blocks(g), hipHostMallocDefault);
+    st = hipHostMalloc((void**)&e->red_host, sizeof(double) * 4 * SUM_SLOTS, hipHostMallocDefault);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, (Nn + 63) / 64, e->stream);
-  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * grid_blocks(g), e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * SUM_SLOTS, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.piece_counters, 0, sizeof(unsigned long long) * 3, e->stream);
+  int cus = 256;
+  if (st == hipSuccess) st = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
+  e->grid_blocks = (uint32_t)std::max(1, std::min<int>(cus, (int)((npieces + 3) / 4)));
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) {
     mpm_destroy(e);
@@ -300,7 +313,8 @@ int mpm_destroy(mpm_engine* e) {
   DevState& d = e->d;
   void* ptrs[] = {d.planes, d.planes_alt, d.ids,  d.ids_alt,  d.meta,     d.meta_alt, d.acc_m,
                   d.acc_p,  d.vel,        d.fixed, d.pieces,  d.keys,     d.keys_alt, d.vals,
-                  d.vals_alt, d.hist,     d.partials, d.red,  d.materials, d.mlp, d.sur_counts};
+                  d.vals_alt, d.hist,     d.partials, d.red,  d.materials, d.mlp, d.sur_counts,
+                  d.piece_list, d.piece_counters, e->frame_ox, e->frame_disp, e->frame_cell_start, e->frame_max};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (e->red_host) hipHostFree(e->red_host);
@@ -399,8 +413,12 @@ int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
   }
   for (int s = 0; s < nsteps; ++s) {
     {
+      Timed t(e, "pieces", true);
+      launch_compact(e->stream, e->d, grid_pieces(e->g), 1, t.ev());
+    }
+    {
       Timed t(e, "grid", true);
-      launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx, t.ev());
+      launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx, e->grid_blocks, t.ev());
     }
     e->steps_since_sort++;
     const bool last = s == nsteps - 1;
@@ -467,13 +485,12 @@ int mpm_conservation_report(mpm_engine* e, mpm_conservation* out) {
   launch_particle_reduce(e->stream, e->d, e->n, (float)e->mass_scale, D, dbuf);
   std::vector<double> h((size_t)K * nb);
   hipError_t st = hipMemcpyAsync(h.data(), dbuf, sizeof(double) * K * nb, hipMemcpyDeviceToHost, e->stream);
-  const uint32_t gb = grid_blocks(e->g);
   if (st == hipSuccess)
-    st = hipMemcpyAsync(e->red_host, e->d.red, sizeof(double) * 4 * gb, hipMemcpyDeviceToHost, e->stream);
+    st = hipMemcpyAsync(e->red_host, e->d.red, sizeof(double) * 4 * SUM_SLOTS, hipMemcpyDeviceToHost, e->stream);
   hipFreeAsync(dbuf, e->stream);
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "conservation: %s", hipGetErrorString(st));
-  finish_conservation(h.data(), K, nb, e->red_host, 0, gb, e->particle_mass_q, e->mass_scale, e->n, out);
+  finish_conservation(h.data(), K, nb, e->red_host, SUM_SLOTS, e->particle_mass_q, e->mass_scale, e->n, out);
   return MPM_OK;
 }
 
@@ -654,6 +671,7 @@ int mpm_set_surrogate(mpm_engine* e, int32_t material, const float* params, cons
   nm.eps_max = norm->eps_max;
   nm.T_lo = norm->T_lo;
   nm.T_hi = norm->T_hi;
+  nm.min_yield_ratio = norm->min_yield_ratio;
   nm.material = material;
   pack_mlp(params, nm, *host);
   hipError_t st = hipSuccess;
@@ -687,6 +705,87 @@ int mpm_surrogate_counts(mpm_engine* e, uint64_t* used, uint64_t* elastic, uint6
   if (used) *used = tot[0];
   if (elastic) *elastic = tot[1];
   if (fallback) *fallback = tot[2];
+  return MPM_OK;
+}
+
+int mpm_frame_defaults(const mpm_engine* e, mpm_frame_config* c) {
+  if (!e || !c) return MPM_ERR_INVALID_ARGUMENT;
+  std::memset(c, 0, sizeof *c);
+  c->time_step_scale = 1.0;
+  c->max_substeps = 48;
+  c->velocity_limit = 160.0;
+  for (int a = 0; a < 3; ++a) {
+    c->bounds_lo[a] = e->cfg.lower[a];
+    c->bounds_hi[a] = e->cfg.upper[a];
+  }
+  c->anti_clump = 1;
+  c->volume_unit = 1.0 / 64.0;
+  c->gravity_mode = MPM_GRAVITY_UNIFORM;
+  c->gravity_strength = 1.0;
+  c->gravity_direction[1] = -1.0;
+  c->gravity_softening = 8.0;
+  return MPM_OK;
+}
+
+int mpm_advance_frame(mpm_engine* e, const mpm_frame_config* c, double dt, mpm_frame_report* out) {
+  if (!e || !c) return MPM_ERR_INVALID_ARGUMENT;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt");
+  if (!(c->time_step_scale > 0) || c->max_substeps < 1 || !(c->velocity_limit > 0) || c->gravity_mode < 0 ||
+      c->gravity_mode > 3 || !(c->volume_unit > 0))
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad frame settings");
+  for (int a = 0; a < 3; ++a)
+    if (!(c->bounds_hi[a] >= c->bounds_lo[a])) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad bounds");
+  // The spacing rule reaches 5.6 in the reference's units; the neighbour
+  // search covers it with +-1 base cells for h >= 5.6 and +-2 for h >= 2.8.
+  const double h = e->cfg.cell_size;
+  const int R = h >= 5.6 ? 1 : 2;
+  if (c->anti_clump && h < 2.8)
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "anti-clump spacing rule needs a cell size >= 2.8 (reference units)");
+  const uint32_t cap = e->d.cap;
+  const size_t Nn = (size_t)e->g.N * e->g.N * e->g.N;
+  if (!e->frame_ox) HIP_OK(e, hipMalloc((void**)&e->frame_ox, sizeof(float) * 3 * (size_t)cap));
+  if (c->anti_clump) {
+    if (!e->frame_disp) HIP_OK(e, hipMalloc((void**)&e->frame_disp, sizeof(float) * 3 * (size_t)cap));
+    if (!e->frame_cell_start) HIP_OK(e, hipMalloc((void**)&e->frame_cell_start, sizeof(uint32_t) * Nn));
+    if (!e->frame_max) HIP_OK(e, hipMalloc((void**)&e->frame_max, sizeof(unsigned int)));
+  }
+  FrameDev f;
+  for (int a = 0; a < 3; ++a) {
+    f.lo[a] = (float)c->bounds_lo[a];
+    f.hi[a] = (float)c->bounds_hi[a];
+    f.gdir[a] = (float)c->gravity_direction[a];
+    f.gcenter[a] = (float)c->gravity_center[a];
+  }
+  f.vlimit = (float)c->velocity_limit;
+  f.gmode = c->gravity_mode;
+  f.gstrength = (float)c->gravity_strength;
+  f.gsoft = (float)c->gravity_softening;
+  f.vol_unit = (float)c->volume_unit;
+  double remaining = dt;
+  int steps = 0;
+  while (remaining > 1e-9 && steps < c->max_substeps) {
+    double stable = 0;
+    int rc = mpm_stable_dt(e, 0, 0, &stable);
+    if (rc) return rc;
+    const double hs = std::min(remaining, stable * c->time_step_scale);
+    launch_frame_pre(e->stream, e->d, e->n, e->frame_ox, f, (float)hs);
+    rc = mpm_step(e, hs, 1);
+    if (rc) return rc;
+    launch_frame_post(e->stream, e->d, e->n, e->frame_ox, f, (float)hs);
+    if (c->anti_clump && e->n) {
+      rc = rebin(e);
+      if (rc) return rc;
+      launch_regularize(e->stream, e->d, e->n, e->g, R, f, (float)hs, e->frame_cell_start, e->frame_disp,
+                        e->frame_max);
+    }
+    HIP_OK(e, hipGetLastError());
+    remaining -= hs;
+    ++steps;
+  }
+  if (out) {
+    out->substeps = steps;
+    out->advanced = dt - remaining;
+  }
   return MPM_OK;
 }
 

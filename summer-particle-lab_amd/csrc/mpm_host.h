@@ -132,9 +132,9 @@ inline void unpack_particles(const float* planes, size_t stride, uint32_t slot0,
 }
 
 // Finishes a conservation report from the particle reduction records (nb
-// records of K doubles, see k_particle_reduce) and the grid blocks' records
-// [gb0, gb1) of {fixed-point mass bits, momentum x, y, z}.
-inline void finish_conservation(const double* h, int K, int nb, const double* grid, uint32_t gb0, uint32_t gb1,
+// records of K doubles, see k_particle_reduce) and the scene's grid-sum
+// slots (nslots records of {fixed-point mass bits, momentum x, y, z}).
+inline void finish_conservation(const double* h, int K, int nb, const double* grid, int nslots,
                                 int64_t particle_mass_q, double mass_scale, uint32_t n, mpm_conservation* out) {
   std::memset(out, 0, sizeof *out);
   double acc[16] = {0};
@@ -148,7 +148,7 @@ inline void finish_conservation(const double* h, int K, int nb, const double* gr
   out->particle_mass_q = particle_mass_q;
   int64_t gq = 0;
   double gp[3] = {0, 0, 0};
-  for (uint32_t b = gb0; b < gb1; ++b) {
+  for (int b = 0; b < nslots; ++b) {
     int64_t q;
     std::memcpy(&q, &grid[4 * (size_t)b], sizeof q);
     gq += q;

@@ -65,9 +65,11 @@ struct DevState {
   uint8_t* pieces;            // active 64-node pieces [ceil(N^3/64)]
   // sort scratch
   uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *partials;
-  // grid sums of the last grid update, one record per grid block:
-  // {fixed-point mass (int64 bits), momentum x, y, z}
-  double* red;
+  // grid sums of the last grid update, {fixed-point mass (int64 bits),
+  // momentum x, y, z} per slot
+  double* red;                 // grid sums: SUM_SLOTS slots of 4 per scene
+  uint32_t* piece_list;        // active pieces of this step
+  unsigned long long* piece_counters;  // {append position, step start, previous end}
   DevMaterial* materials;
   // learned constitutive surrogate (nullptr: off) and its per-branch
   // particle-step counters, SUR_SLOTS slots of {network, elastic, fallback, -}
@@ -76,6 +78,7 @@ struct DevState {
 };
 
 constexpr int SUR_SLOTS = 64;
+constexpr int SUM_SLOTS = 16;  // slots per scene of the grid sums (spreads the atomics)
 
 // One scene of a batch: its grid and step parameters, its particle slot
 // range [p0, p0 + n) and the first node / grid block of its grid in the
@@ -100,6 +103,15 @@ struct BatchArgs {
   unsigned int* work;
 };
 
+// Settings of the frame wrapper as the kernels see them.
+struct FrameDev {
+  float lo[3], hi[3];
+  float vlimit;
+  int gmode;  // 0 none, 1 uniform, 2 attract, 3 repel
+  float gstrength, gdir[3], gcenter[3], gsoft;
+  float vol_unit;  // particle volume unit of the spacing rule (1/64 for the reference's V)
+};
+
 // Optional events that a launch stamps with its own start and end (profiling
 // without barrier packets between launches).
 struct LaunchEvents {
@@ -109,21 +121,31 @@ struct LaunchEvents {
 // Kernel launchers (mpm_kernels.hip)
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
 void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
+// Grid update: launch_compact lists the pieces P2G flagged (and clears the
+// grid sums), launch_grid / launch_grid_batch update the listed pieces with
+// `blocks` persistent workgroups.
+void launch_compact(hipStream_t s, const DevState& d, uint32_t npieces, uint32_t nscenes, LaunchEvents ev = {});
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
-                 const int hi_idx[3], LaunchEvents ev = {});
+                 const int hi_idx[3], uint32_t blocks, LaunchEvents ev = {});
 // Fused particle kernel (mpm_fused.hip): mode 0 P2G from the stored state,
 // 1 G2P of this step + P2G of the next, 2 G2P writing the full state. With
 // d.mlp set, the G2P modes take the surrogate variant.
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode,
                   LaunchEvents ev = {});
-uint32_t grid_blocks(const GridDims& g);
+uint32_t grid_pieces(const GridDims& g);
 // Batched launchers (mpm_kernels.hip, mpm_fused.hip): all scenes of a batch
 // in one launch; `particle_blocks` is the persistent grid size.
 void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b);
 void launch_grid_batch(hipStream_t s, const DevState& d, const SceneDesc* scenes, const uint32_t* block_scene,
-                       uint32_t nblocks, uint32_t node_stride, LaunchEvents ev = {});
+                       uint32_t node_stride, uint32_t blocks, LaunchEvents ev = {});
 void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, uint32_t particle_blocks, int mode,
                         int fb, LaunchEvents ev = {});
+// Frame wrapper kernels (mpm_kernels.hip). `ox` and `disp` are 3 planes of
+// cap floats; cell_start is N^3 entries; maxbits one word.
+void launch_frame_pre(hipStream_t s, const DevState& d, uint32_t n, float* ox, const FrameDev& f, float dt);
+void launch_frame_post(hipStream_t s, const DevState& d, uint32_t n, const float* ox, const FrameDev& f, float dt);
+void launch_regularize(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, int R, const FrameDev& f,
+                       float dt, uint32_t* cell_start, float* disp, unsigned int* maxbits);
 // Trained parameters packed into the device record (mpm_mlp.hip).
 void pack_mlp(const float* params, const MlpNorm& norm, MlpDev& out);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);

@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "mpm_internal.h"
 
 namespace mpm {
@@ -101,63 +104,153 @@ __device__ __forceinline__ void step_stress(const M3& F, const DevMaterial& m, u
   }
 }
 
-// One wave per 64-node piece of the dense grid; waves of inactive pieces do
-// no memory work. Each block writes one record of FP64 grid sums (fixed-point
-// mass, momentum) so the sums need no same-address atomics. `blk` is the
-// block's index within its scene's grid, `out` its record; the momentum
-// planes are `pstride` floats apart.
-__device__ __forceinline__ void grid_block(uint32_t blk, uint32_t out, unsigned long long* __restrict__ acc_m,
-                                           float* __restrict__ acc_p, size_t pstride, uint8_t* __restrict__ fixed,
-                                           uint8_t* __restrict__ pieces, float4* __restrict__ vel, const GridDims& g,
-                                           const StepParams& sp, const int lo_idx[3], const int hi_idx[3],
-                                           double* __restrict__ partials) {
-  __shared__ double s_red[BLOCK / 64][4];
+// Grid update over the active pieces only. P2G flags every 64-node piece it
+// adds mass to; k_compact_pieces appends the flagged pieces to a circular
+// list (and clears the flags), and k_grid_list updates the pieces appended
+// since the previous update: v = p/m, gravity, wall rules
+// (src/three/mpm.js:89-102), clear of the accumulators. Grid sums
+// (fixed-point mass, momentum) go to SUM_SLOTS slots per scene with atomics.
+//
+// counters (64-bit): [0] running append position, [1] first position of this
+// step, [2] end position the previous update reached. Positions only grow,
+// so no counter is ever reset; list slots are positions modulo the list
+// capacity (at most every piece is appended in one step).
+
+// Compaction of the piece flags. Block 0 also starts the step's list at the
+// previous update's end and clears this step's grid sums.
+__global__ __launch_bounds__(BLOCK) void k_compact_pieces(uint8_t* __restrict__ pieces, uint32_t npieces,
+                                                          uint32_t* __restrict__ list,
+                                                          unsigned long long* __restrict__ counters,
+                                                          double* __restrict__ sums, uint32_t nsums) {
+  if (blockIdx.x == 0) {
+    for (uint32_t k = threadIdx.x; k < nsums; k += BLOCK) sums[k] = 0.0;
+    if (threadIdx.x == 0) counters[1] = counters[2];
+  }
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  bool act = false;
+  if (i < npieces) {
+    act = pieces[i] != 0;
+    if (act) pieces[i] = 0;
+  }
+  const uint64_t m = __ballot(act);
+  unsigned long long base = 0;
+  if (lane == 0 && m) base = atomicAdd(&counters[0], (unsigned long long)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (act) list[(base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))) % npieces] = i;
+}
+
+// One node of a listed piece (node index within its scene's grid).
+__device__ __forceinline__ void grid_node(size_t node, size_t Nn, unsigned long long* __restrict__ acc_m,
+                                          float* __restrict__ acc_p, size_t pstride, uint8_t* __restrict__ fixed,
+                                          float4* __restrict__ vel, const GridDims& g, const StepParams& sp,
+                                          const int lo_idx[3], const int hi_idx[3], long long& Mq, double& p0,
+                                          double& p1, double& p2) {
+  if (node >= Nn) return;
+  const unsigned long long M = acc_m[node];
+  Mq += (long long)M;
+  const float px = acc_p[node], py = acc_p[pstride + node], pz = acc_p[2 * pstride + node];
+  p0 += px;
+  p1 += py;
+  p2 += pz;
+  const float m = (float)M * sp.inv_mass_scale;
+  float v[3] = {px, py, pz};
+  if (m > g.mass_eps) {
+    const float im = 1.f / m;
+    const float gd = g.gravity_on_particles ? 0.f : sp.dt;
+    v[0] = px * im + g.gravity[0] * gd;
+    v[1] = py * im + g.gravity[1] * gd;
+    v[2] = pz * im + g.gravity[2] * gd;
+    const int N = g.N;
+    const int ix = (int)(node % N), iy = (int)((node / N) % N), iz = (int)(node / ((size_t)N * N));
+    const int idx[3] = {ix, iy, iz};
+    bool sticky = fixed[node] != 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if ((g.wall[2 * a] == MPM_WALL_STICKY && idx[a] <= lo_idx[a]) ||
+          (g.wall[2 * a + 1] == MPM_WALL_STICKY && idx[a] >= hi_idx[a]))
+        sticky = true;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool lo_hit = idx[a] <= lo_idx[a] && (g.wall[2 * a] == MPM_WALL_SLIP || v[a] < 0.f);
+      const bool hi_hit = idx[a] >= hi_idx[a] && (g.wall[2 * a + 1] == MPM_WALL_SLIP || v[a] > 0.f);
+      if (sticky || lo_hit || hi_hit) v[a] = 0.f;
+    }
+  }
+  vel[node] = make_float4(v[0], v[1], v[2], m);
+  acc_m[node] = 0ull;
+  acc_p[node] = 0.f;
+  acc_p[pstride + node] = 0.f;
+  acc_p[2 * pstride + node] = 0.f;
+  fixed[node] = 0;
+}
+
+// Adds a wave's grid sums of one scene to that scene's slot.
+__device__ __forceinline__ void flush_sums(double* __restrict__ sums, uint32_t scene, uint32_t slot, long long Mq,
+                                           double p0, double p1, double p2) {
+  double* s = sums + ((size_t)scene * SUM_SLOTS + slot) * 4;
+  atomicAdd(reinterpret_cast<unsigned long long*>(s), (unsigned long long)Mq);
+  atomicAdd(s + 1, p0);
+  atomicAdd(s + 2, p1);
+  atomicAdd(s + 3, p2);
+}
+
+// Wave w of the launch updates the step's list entries w, w + W, ... (W waves
+// in all). Each wave keeps running sums for its current scene and flushes
+// them when the scene changes; at the end the waves of a workgroup combine
+// sums of equal scenes before the atomics.
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
+                                                     uint8_t* __restrict__ fixed, float4* __restrict__ vel,
+                                                     GridDims g1, StepParams sp1, int lo0, int lo1, int lo2, int hi0,
+                                                     int hi1, int hi2, const SceneDesc* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ block_scene, uint32_t node_stride,
+                                                     const uint32_t* __restrict__ list, uint32_t npieces,
+                                                     unsigned long long* __restrict__ counters,
+                                                     double* __restrict__ sums) {
+  __shared__ double s_sum[BLOCK / 64][4];
+  __shared__ uint32_t s_scene[BLOCK / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const size_t piece = (size_t)blk * (BLOCK / 64) + wave;
-  const size_t Nn = (size_t)g.N * g.N * g.N;
-  const size_t npieces = (Nn + 63) / 64;
-  const bool active = piece < npieces && pieces[piece] != 0;
-  const size_t node = piece * 64 + lane;
+  const unsigned long long start = counters[1], end = counters[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[2] = end;
+  const uint32_t count = (uint32_t)(end - start);
+  const uint32_t W = gridDim.x * (BLOCK / 64);
+  uint32_t cur = 0xffffffffu;
   long long Mq = 0;
   double p0 = 0, p1 = 0, p2 = 0;
-  if (active && node < Nn) {
-    const unsigned long long M = acc_m[node];
-    Mq = (long long)M;
-    const float px = acc_p[node], py = acc_p[pstride + node], pz = acc_p[2 * pstride + node];
-    p0 = px;
-    p1 = py;
-    p2 = pz;
-    const float m = (float)M * sp.inv_mass_scale;
-    float v[3] = {px, py, pz};
-    if (m > g.mass_eps) {
-      const float im = 1.f / m;
-      const float gd = g.gravity_on_particles ? 0.f : sp.dt;
-      v[0] = px * im + g.gravity[0] * gd;
-      v[1] = py * im + g.gravity[1] * gd;
-      v[2] = pz * im + g.gravity[2] * gd;
-      const int N = g.N;
-      const int ix = (int)(node % N), iy = (int)((node / N) % N), iz = (int)(node / ((size_t)N * N));
-      const int idx[3] = {ix, iy, iz};
-      bool sticky = fixed[node] != 0;
+  for (uint32_t e = blockIdx.x * (BLOCK / 64) + wave; e < count; e += W) {
+    const uint32_t gp = list[(start + e) % npieces];  // piece index in the (concatenated) grid
+    if constexpr (BATCH) {
+      const uint32_t scene = block_scene[gp >> 2];
+      if (scene != cur) {
+        if (cur != 0xffffffffu) {
+          long long m = Mq;
+          double a = p0, b = p1, c = p2;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if ((g.wall[2 * a] == MPM_WALL_STICKY && idx[a] <= lo_idx[a]) ||
-            (g.wall[2 * a + 1] == MPM_WALL_STICKY && idx[a] >= hi_idx[a]))
-          sticky = true;
+          for (int d = 32; d > 0; d >>= 1) {
+            m += __shfl_down(m, d, 64);
+            a += __shfl_down(a, d, 64);
+            b += __shfl_down(b, d, 64);
+            c += __shfl_down(c, d, 64);
+          }
+          if (lane == 0) flush_sums(sums, cur, blockIdx.x % SUM_SLOTS, m, a, b, c);
+        }
+        cur = scene;
+        Mq = 0;
+        p0 = p1 = p2 = 0;
       }
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const bool lo_hit = idx[a] <= lo_idx[a] && (g.wall[2 * a] == MPM_WALL_SLIP || v[a] < 0.f);
-        const bool hi_hit = idx[a] >= hi_idx[a] && (g.wall[2 * a + 1] == MPM_WALL_SLIP || v[a] > 0.f);
-        if (sticky || lo_hit || hi_hit) v[a] = 0.f;
-      }
+      const SceneDesc* sd = scenes + scene;
+      const uint32_t node0 = sd->node0;
+      const size_t Nn = (size_t)sd->g.N * sd->g.N * sd->g.N;
+      grid_node((size_t)(gp - (node0 >> 6)) * 64 + lane, Nn, acc_m + node0, acc_p + node0, node_stride,
+                fixed + node0, vel + node0, sd->g, sd->sp, sd->lo_idx, sd->hi_idx, Mq, p0, p1, p2);
+    } else {
+      cur = 0;
+      const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+      const size_t Nn = (size_t)g1.N * g1.N * g1.N;
+      grid_node((size_t)gp * 64 + lane, Nn, acc_m, acc_p, Nn, fixed, vel, g1, sp1, lo, hi, Mq, p0, p1, p2);
     }
-    vel[node] = make_float4(v[0], v[1], v[2], m);
-    acc_m[node] = 0ull;
-    acc_p[node] = 0.f;
-    acc_p[pstride + node] = 0.f;
-    acc_p[2 * pstride + node] = 0.f;
-    fixed[node] = 0;
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
@@ -167,49 +260,193 @@ __device__ __forceinline__ void grid_block(uint32_t blk, uint32_t out, unsigned 
     p2 += __shfl_down(p2, d, 64);
   }
   if (lane == 0) {
-    s_red[wave][0] = __longlong_as_double(Mq);
-    s_red[wave][1] = p0;
-    s_red[wave][2] = p1;
-    s_red[wave][3] = p2;
-    if (active) pieces[piece] = 0;
+    s_scene[wave] = cur;
+    s_sum[wave][0] = __longlong_as_double(Mq);
+    s_sum[wave][1] = p0;
+    s_sum[wave][2] = p1;
+    s_sum[wave][3] = p2;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    // Waves of one workgroup hold neighbouring list entries: combine runs of
+    // equal scenes, one flush per run.
+    uint32_t sc = 0xffffffffu;
     long long m = 0;
-    double q[3] = {0, 0, 0};
-    for (int w = 0; w < BLOCK / 64; ++w) {
-      m += __double_as_longlong(s_red[w][0]);
-      for (int c = 0; c < 3; ++c) q[c] += s_red[w][1 + c];
+    double a = 0, b = 0, c = 0;
+    for (uint32_t w = 0; w < BLOCK / 64; ++w) {
+      if (s_scene[w] == 0xffffffffu) continue;
+      if (s_scene[w] != sc) {
+        if (sc != 0xffffffffu) flush_sums(sums, sc, blockIdx.x % SUM_SLOTS, m, a, b, c);
+        sc = s_scene[w];
+        m = 0;
+        a = b = c = 0;
+      }
+      m += __double_as_longlong(s_sum[w][0]);
+      a += s_sum[w][1];
+      b += s_sum[w][2];
+      c += s_sum[w][3];
     }
-    partials[(size_t)out * 4 + 0] = __longlong_as_double(m);
-    for (int c = 0; c < 3; ++c) partials[(size_t)out * 4 + 1 + c] = q[c];
+    if (sc != 0xffffffffu) flush_sums(sums, sc, blockIdx.x % SUM_SLOTS, m, a, b, c);
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_grid(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-                                                uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces,
-                                                float4* __restrict__ vel, GridDims g, StepParams sp,
-                                                int lo_idx0, int lo_idx1, int lo_idx2, int hi_idx0, int hi_idx1,
-                                                int hi_idx2, double* __restrict__ partials) {
-  const int lo_idx[3] = {lo_idx0, lo_idx1, lo_idx2};
-  const int hi_idx[3] = {hi_idx0, hi_idx1, hi_idx2};
-  grid_block(blockIdx.x, blockIdx.x, acc_m, acc_p, (size_t)g.N * g.N * g.N, fixed, pieces, vel, g, sp, lo_idx, hi_idx,
-             partials);
+// ------------------------------------------------------------ frame wrapper
+// Device side of Simulation3.step's all-solid substep around MPM3.step
+// (src/three/simulation.js:330-388) and of regularizeSolids
+// (src/three/solid-sampling.js:5-42).
+
+// Particle gravity of the gravity modes (src/three/gravity-options.js:7-16).
+__device__ __forceinline__ void frame_gravity(const FrameDev& f, float x, float y, float z, float a[3]) {
+  a[0] = a[1] = a[2] = 0.f;
+  if (f.gmode == 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[k] = f.gdir[k] * f.gstrength * 24.f;
+  } else if (f.gmode == 2 || f.gmode == 3) {
+    const float d[3] = {f.gcenter[0] - x, f.gcenter[1] - y, f.gcenter[2] - z};
+    const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + f.gsoft * f.gsoft;
+    const float sc = (f.gmode == 3 ? -24000.f : 24000.f) * f.gstrength / (r2 * sqrtf(r2));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[k] = d[k] * sc;
+  }
 }
 
-// Grid update of many scenes in one launch: block -> scene from a table; the
-// scenes' grids are concatenated (each starts on a block boundary).
-__global__ __launch_bounds__(BLOCK) void k_grid_batch(unsigned long long* __restrict__ acc_m,
-                                                      float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
-                                                      uint8_t* __restrict__ pieces, float4* __restrict__ vel,
-                                                      const SceneDesc* __restrict__ scenes,
-                                                      const uint32_t* __restrict__ block_scene, uint32_t node_stride,
-                                                      double* __restrict__ partials) {
-  const SceneDesc* sd = scenes + block_scene[blockIdx.x];
-  const uint32_t node0 = sd->node0;
-  grid_block(blockIdx.x - sd->block0, blockIdx.x, acc_m + node0, acc_p + node0, node_stride, fixed + node0,
-             pieces + (node0 >> 6), vel + node0, sd->g, sd->sp, sd->lo_idx, sd->hi_idx, partials);
+// Before a substep: remember x; unpinned particles take gravity into v.
+__global__ __launch_bounds__(BLOCK) void k_frame_pre(float* __restrict__ P, const uint32_t* __restrict__ meta,
+                                                     uint32_t cap, uint32_t n, float* __restrict__ ox, FrameDev f,
+                                                     float dt) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float x = plane(P, cap, P_X, i), y = plane(P, cap, P_Y, i), z = plane(P, cap, P_Z, i);
+  ox[i] = x;
+  ox[(size_t)cap + i] = y;
+  ox[2 * (size_t)cap + i] = z;
+  if ((meta[i] >> 8) & FLAG_PINNED) return;
+  float a[3];
+  frame_gravity(f, x, y, z, a);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) set_plane(P, cap, P_VX + k, i, plane(P, cap, P_VX + k, i) + a[k] * dt);
 }
+
+// After the step: unpinned x clamped to the bounds, pinned x back to its
+// anchor; v = clamp((x - x_start) / dt, +-limit).
+__global__ __launch_bounds__(BLOCK) void k_frame_post(float* __restrict__ P, const uint32_t* __restrict__ meta,
+                                                      uint32_t cap, uint32_t n, const float* __restrict__ ox,
+                                                      FrameDev f, float dt) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const bool pinned = (meta[i] >> 8) & FLAG_PINNED;
+  const float inv = 1.f / dt;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float o = ox[(size_t)k * cap + i];
+    const float x = pinned ? o : fminf(fmaxf(plane(P, cap, P_X + k, i), f.lo[k]), f.hi[k]);
+    set_plane(P, cap, P_X + k, i, x);
+    set_plane(P, cap, P_VX + k, i, fminf(fmaxf((x - o) * inv, -f.vlimit), f.vlimit));
+  }
+}
+
+// First slot of every base cell in the sorted slot order (entries of cells
+// with no particle are left stale; readers check them).
+__global__ __launch_bounds__(BLOCK) void k_cell_starts(const uint32_t* __restrict__ keys, uint32_t n,
+                                                       uint32_t* __restrict__ start) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) start[k] = i;
+}
+
+__device__ __forceinline__ float frame_spacing(const float* __restrict__ P, uint32_t cap, uint32_t i, float vol_unit) {
+  float F[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
+  const float J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                  F[2] * (F[3] * F[7] - F[4] * F[6]);
+  return 2.8f * cbrtf(fminf(fmaxf(plane(P, cap, P_VOL, i) * vol_unit * J, 0.015625f), 8.f));
+}
+
+// regularizeSolids as a gather: particle i sums the pushes of every particle
+// j of the same material closer than the mean of their spacings, from the
+// base cells within +-R of its own; a coincident pair uses the reference's
+// hashed direction, oriented from the lower to the higher original index.
+// d receives the displacement, maxbits the largest |d| (float bits).
+__global__ __launch_bounds__(BLOCK) void k_regularize(const float* __restrict__ P, const uint32_t* __restrict__ meta,
+                                                      const uint32_t* __restrict__ ids,
+                                                      const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ cstart, uint32_t cap, uint32_t n,
+                                                      int N, int R, FrameDev f, float gain, float* __restrict__ d,
+                                                      unsigned int* __restrict__ maxbits) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  float len = 0.f;
+  if (i < n) {
+    const uint32_t mi = meta[i];
+    const float imi = ((mi >> 8) & FLAG_PINNED) ? 0.f : 1.f / plane(P, cap, P_MASS, i);
+    const float xi = plane(P, cap, P_X, i), yi = plane(P, cap, P_Y, i), zi = plane(P, cap, P_Z, i);
+    const float ri = frame_spacing(P, cap, i, f.vol_unit);
+    const uint32_t oi = ids[i];
+    const uint32_t c = keys[i];
+    const int bx = (int)(c % (uint32_t)N), by = (int)((c / (uint32_t)N) % (uint32_t)N), bz = (int)(c / ((uint32_t)N * N));
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+    for (int oz = -R; oz <= R; ++oz)
+      for (int oy = -R; oy <= R; ++oy)
+        for (int oxo = -R; oxo <= R; ++oxo) {
+          const int cx = bx + oxo, cy = by + oy, cz = bz + oz;
+          if (cx < 0 || cy < 0 || cz < 0 || cx > N - 3 || cy > N - 3 || cz > N - 3) continue;
+          const uint32_t cc = ((uint32_t)cz * N + cy) * N + cx;
+          uint32_t j = cstart[cc];
+          if (j >= n || keys[j] != cc || (j > 0 && keys[j - 1] == cc)) continue;
+          for (; j < n && keys[j] == cc; ++j) {
+            if (j == i || (meta[j] & 255u) != (mi & 255u)) continue;
+            const float imj = ((meta[j] >> 8) & FLAG_PINNED) ? 0.f : 1.f / plane(P, cap, P_MASS, j);
+            const float total = imi + imj;
+            if (total == 0.f) continue;
+            float ex = xi - plane(P, cap, P_X, j), ey = yi - plane(P, cap, P_Y, j), ez = zi - plane(P, cap, P_Z, j);
+            float r = sqrtf(ex * ex + ey * ey + ez * ez);
+            const float target = 0.5f * (ri + frame_spacing(P, cap, j, f.vol_unit));
+            if (r >= target) continue;
+            float q;
+            if (r < 1e-8f) {
+              const uint32_t oj = ids[j];
+              const uint32_t lo = oi < oj ? oi : oj, hi = oi < oj ? oj : oi;
+              const uint32_t hash = ((lo + 1u) * 73856093u) ^ ((hi + 1u) * 19349663u);
+              const float sgn = oi < oj ? 1.f : -1.f;
+              ex = sgn * ((float)(hash & 1023u) / 511.5f - 1.f);
+              ey = sgn * ((float)((hash >> 10) & 1023u) / 511.5f - 1.f);
+              ez = sgn * ((float)((hash >> 20) & 1023u) / 511.5f - 1.f);
+              r = sqrtf(ex * ex + ey * ey + ez * ez);
+              q = target * gain / (r * total);
+            } else {
+              q = (target - r) * gain / (r * total);
+            }
+            dx += ex * q * imi;
+            dy += ey * q * imi;
+            dz += ez * q * imi;
+          }
+        }
+    d[i] = dx;
+    d[(size_t)cap + i] = dy;
+    d[2 * (size_t)cap + i] = dz;
+    len = sqrtf(dx * dx + dy * dy + dz * dz);
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) len = fmaxf(len, __shfl_down(len, s, 64));
+  if ((threadIdx.x & 63u) == 0 && len > 0.f) atomicMax(maxbits, __float_as_uint(len));
+}
+
+// Applies the displacements scaled so the largest is at most `budget`.
+__global__ __launch_bounds__(BLOCK) void k_regularize_apply(float* __restrict__ P, const uint32_t* __restrict__ meta,
+                                                            uint32_t cap, uint32_t n, const float* __restrict__ d,
+                                                            const unsigned int* __restrict__ maxbits, FrameDev f,
+                                                            float budget) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n || ((meta[i] >> 8) & FLAG_PINNED)) return;
+  const float scale = fmaxf(1.f, __uint_as_float(*maxbits) / budget);
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    set_plane(P, cap, P_X + k, i,
+              fminf(fmaxf(plane(P, cap, P_X + k, i) + d[(size_t)k * cap + i] / scale, f.lo[k]), f.hi[k]));
+}
+
+__global__ void k_zero_u32(unsigned int* p) { *p = 0u; }
 
 __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,10 +640,7 @@ inline unsigned blocks(size_t n, int b = BLOCK) { return (unsigned)((n + b - 1) 
 }  // namespace
 
 int particle_reduce_k() { return RED_K; }
-uint32_t grid_blocks(const GridDims& g) {
-  const size_t Nn = (size_t)g.N * g.N * g.N;
-  return (uint32_t)(((Nn + 63) / 64 + BLOCK / 64 - 1) / (BLOCK / 64));
-}
+uint32_t grid_pieces(const GridDims& g) { return (uint32_t)(((size_t)g.N * g.N * g.N + 63) / 64); }
 int particle_reduce_blocks() { return RED_BLOCKS; }
 
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g) {
@@ -423,25 +657,52 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
   std::swap(d.meta, d.meta_alt);
 }
 
+void launch_compact(hipStream_t s, const DevState& d, uint32_t npieces, uint32_t nscenes, LaunchEvents ev) {
+  hipExtLaunchKernelGGL(k_compact_pieces, dim3(std::max<uint32_t>(1, (npieces + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                        s, ev.start, ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
+                        nscenes * SUM_SLOTS * 4);
+}
+
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
-                 const int hi_idx[3], LaunchEvents ev) {
-  hipExtLaunchKernelGGL(k_grid, dim3(grid_blocks(g)), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
-                        d.fixed, d.pieces, d.vel, g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1],
-                        hi_idx[2], d.red);
+                 const int hi_idx[3], uint32_t blocks, LaunchEvents ev) {
+  hipExtLaunchKernelGGL(k_grid_list<false>, dim3(blocks), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
+                        d.fixed, d.vel, g, p, lo_idx[0], lo_idx[1], lo_idx[2], hi_idx[0], hi_idx[1], hi_idx[2],
+                        (const SceneDesc*)nullptr, (const uint32_t*)nullptr, 0u, d.piece_list, grid_pieces(g),
+                        d.piece_counters, d.red);
 }
 
 void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b) {
-  // Chunks of the batch binning pass are BLOCK-aligned slices of the fused
-  // chunks: chunk_size is the largest fused chunk (256 slots).
+  // One 256-thread block per 256-slot binning chunk.
   constexpr uint32_t chunk = 256;
   hipLaunchKernelGGL(k_bin_keys_batch, dim3(b.nchunks * (chunk / BLOCK)), dim3(BLOCK), 0, s, d.planes, d.ids, d.cap,
                      b, chunk, d.keys, d.vals);
 }
 
 void launch_grid_batch(hipStream_t s, const DevState& d, const SceneDesc* scenes, const uint32_t* block_scene,
-                       uint32_t nblocks, uint32_t node_stride, LaunchEvents ev) {
-  hipExtLaunchKernelGGL(k_grid_batch, dim3(nblocks), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
-                        d.fixed, d.pieces, d.vel, scenes, block_scene, node_stride, d.red);
+                       uint32_t node_stride, uint32_t blocks, LaunchEvents ev) {
+  hipExtLaunchKernelGGL(k_grid_list<true>, dim3(blocks), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
+                        d.fixed, d.vel, GridDims{}, StepParams{}, 0, 0, 0, 0, 0, 0, scenes, block_scene, node_stride,
+                        d.piece_list, node_stride / 64, d.piece_counters, d.red);
+}
+
+void launch_frame_pre(hipStream_t s, const DevState& d, uint32_t n, float* ox, const FrameDev& f, float dt) {
+  if (n) hipLaunchKernelGGL(k_frame_pre, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, ox, f, dt);
+}
+
+void launch_frame_post(hipStream_t s, const DevState& d, uint32_t n, const float* ox, const FrameDev& f, float dt) {
+  if (n) hipLaunchKernelGGL(k_frame_post, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, ox, f, dt);
+}
+
+void launch_regularize(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, int R, const FrameDev& f,
+                       float dt, uint32_t* cell_start, float* disp, unsigned int* maxbits) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cell_starts, dim3(blocks(n)), dim3(BLOCK), 0, s, d.keys, n, cell_start);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, maxbits);
+  const float gain = 1.f - std::exp(-40.f * dt);
+  hipLaunchKernelGGL(k_regularize, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.ids, d.keys, cell_start,
+                     d.cap, n, g.N, R, f, gain, disp, maxbits);
+  hipLaunchKernelGGL(k_regularize_apply, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, disp,
+                     maxbits, f, 8.f * dt);
 }
 
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {

@@ -38,19 +38,37 @@ __device__ __forceinline__ void sym_eig(const float e[6], float V[9], float lam[
   jacobi_eig3(e[0], e[1], e[2], e[3], e[4], e[5], V, lam);
 }
 
-// One training sample: log strain eps ~ N(0, sigma) per component, T uniform
-// in [T_lo, T_hi]; F = exp(eps) (the trial state with no rotation); the plain
-// return map of the step gives the Kirchhoff stress; the target is its
-// deviatoric part in units of out_scale * E. Inputs and targets feature-major.
+// One training sample: T uniform in [T_lo, T_hi]; log strain eps ~ N(0,
+// sigma) per component, or (for a share near of the samples) a deviatoric
+// part of norm uniform in [0.8, 2.5] times the yield strain at T in a random
+// direction plus a volumetric part N(0, sigma); F = exp(eps) (the trial
+// state with no rotation); the plain return map of the step gives the
+// Kirchhoff stress; the target is its deviatoric part in units of
+// out_scale * E. Inputs and targets feature-major.
 __global__ __launch_bounds__(TB) void k_mlp_samples(uint32_t seed, uint32_t n, float sigma, float T_lo, float T_hi,
-                                                    float dt, DevMaterial mat, MlpNorm nrm, float* __restrict__ X,
-                                                    float* __restrict__ Y) {
+                                                    float dt, float near, DevMaterial mat, MlpNorm nrm,
+                                                    float* __restrict__ X, float* __restrict__ Y) {
   const uint32_t i = blockIdx.x * TB + threadIdx.x;
   if (i >= n) return;
+  const float T = T_lo + (T_hi - T_lo) * uhash(seed, i, 100);
   float e[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) e[c] = sigma * gauss(seed, i, c);
-  const float T = T_lo + (T_hi - T_lo) * uhash(seed, i, 100);
+  for (int c = 0; c < 6; ++c) e[c] = gauss(seed, i, c);
+  if (uhash(seed, i, 101) < near) {
+    const float m = (e[0] + e[1] + e[2]) * (1.f / 3.f);
+    e[0] -= m;
+    e[1] -= m;
+    e[2] -= m;
+    const float len = sym_norm(e);
+    const float ey = surrogate_yield(T, mat) / (2.4494897427831781f * mat.G);
+    const float r = ey * (0.8f + 1.7f * uhash(seed, i, 102)) / fmaxf(len, 1e-20f);
+    const float v = sigma * gauss(seed, i, 7);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) e[c] = e[c] * r + (c < 3 ? v : 0.f);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) e[c] *= sigma;
+  }
   float V[9], lam[3];
   sym_eig(e, V, lam);
   float s[3] = {expf(lam[0]), expf(lam[1]), expf(lam[2])};
@@ -338,13 +356,14 @@ MlpNorm to_norm(const mpm_mlp_norm& n) {
   m.eps_max = n.eps_max;
   m.T_lo = n.T_lo;
   m.T_hi = n.T_hi;
+  m.min_yield_ratio = n.min_yield_ratio;
   m.material = n.material;
   return m;
 }
 
 bool train_args_ok(const mpm_mlp_train_config* c) {
   return c && c->samples >= 64 && c->batch >= 64 && c->batch <= c->samples && c->epochs >= 1 && c->lr > 0 &&
-         c->strain_sigma > 0 && c->T_hi > c->T_lo;
+         c->strain_sigma > 0 && c->T_hi > c->T_lo && c->near_yield_fraction >= 0 && c->near_yield_fraction <= 1;
 }
 }  // namespace
 
@@ -360,6 +379,7 @@ int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, m
   out->eps_max = c->eps_max > 0 ? c->eps_max : 4.f * c->strain_sigma;
   out->T_lo = c->T_lo;
   out->T_hi = c->T_hi;
+  out->min_yield_ratio = 1.2f;
   out->material = -1;
   out->model = m->model;
   out->mu_over_E = m->mu / m->E;
@@ -395,7 +415,7 @@ int mpm_mlp_samples(const mpm_material* m, const mpm_mlp_train_config* c, const 
   if (st == hipSuccess) st = hipMalloc((void**)&dY, sizeof(float) * 6 * n);
   if (st == hipSuccess) {
     hipLaunchKernelGGL(k_mlp_samples, dim3((n + TB - 1) / TB), dim3(TB), 0, 0, seed, n, c->strain_sigma, c->T_lo,
-                       c->T_hi, c->dt, dm, to_norm(*norm), dX, dY);
+                       c->T_hi, c->dt, c->near_yield_fraction, dm, to_norm(*norm), dX, dY);
     st = hipGetLastError();
   }
   std::vector<float> hx((size_t)7 * n), hy((size_t)6 * n);
@@ -431,7 +451,7 @@ int mpm_mlp_train(const mpm_material* m, const mpm_mlp_train_config* c, const mp
   if (st == hipSuccess) st = hipMemsetAsync(t.m2, 0, sizeof(float) * MLP_NPARAM, s);
   if (st == hipSuccess) {
     hipLaunchKernelGGL(k_mlp_samples, dim3((n + TB - 1) / TB), dim3(TB), 0, s, c->seed, n, c->strain_sigma, c->T_lo,
-                       c->T_hi, c->dt, dm, to_norm(*norm), dX, dY);
+                       c->T_hi, c->dt, c->near_yield_fraction, dm, to_norm(*norm), dX, dY);
     st = hipGetLastError();
   }
   const uint32_t per_epoch = n / B;

@@ -36,6 +36,7 @@ struct MlpNorm {
   float out_scale;  // output unit over E (the yield ratio A)
   float eps_max;    // envelope: |eps_ij| <= eps_max
   float T_lo, T_hi; // envelope: T_lo <= T <= T_hi
+  float min_yield_ratio;  // envelope: q >= min_yield_ratio * yield
   int32_t material; // material index the surrogate replaces (-1: none)
 };
 
@@ -265,25 +266,27 @@ enum SurBranch : int { SUR_NONE = -1, SUR_NETWORK = 0, SUR_ELASTIC = 1, SUR_FALL
 // stress tau) for a lane holding a particle of the surrogate's material
 // (mine). Inside the training envelope (where the series logarithm is also
 // accurate), trial states inside the yield surface take the closed-form
-// elastic branch and yielding states the network; states outside it return
-// SUR_FALLBACK for the caller's plain return. Wave-uniform:
+// elastic branch and states past min_yield_ratio times the yield stress the
+// network; the rest return SUR_FALLBACK for the caller's plain return. Wave-uniform:
 // all 64 lanes of the wave must call it together.
 __device__ __forceinline__ int surrogate_return(const MlpDev* __restrict__ M, bool mine, M3& F, float& ep, float T,
                                                 const DevMaterial& mat, float tau[6], unsigned char* wave_lds) {
   const MlpNorm nm = M->norm;
   float eps[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, lnJ = 0.f;
-  bool elastic = false, inenv = false;
+  bool elastic = false, inenv = false, deep = false;
   if (mine) {
     log_strain_series(F, eps, lnJ);
     const float mean = (eps[0] + eps[1] + eps[2]) * (1.f / 3.f);
     const float dv[6] = {eps[0] - mean, eps[1] - mean, eps[2] - mean, eps[3], eps[4], eps[5]};
     const float q = 2.4494897427831781f * mat.G * sym_norm(dv) * __expf(-lnJ);
-    elastic = q <= surrogate_yield(T, mat);
+    const float y = surrogate_yield(T, mat);
+    elastic = q <= y;
+    deep = q >= nm.min_yield_ratio * y;
     inenv = T >= nm.T_lo && T <= nm.T_hi;
 #pragma unroll
     for (int k = 0; k < 6; ++k) inenv = inenv && fabsf(eps[k]) <= nm.eps_max;
   }
-  const bool use = mine && !elastic && inenv;
+  const bool use = mine && deep && inenv;
   float dtau[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (__any(use)) {
     float xin[7], yout[6];
@@ -299,7 +302,7 @@ __device__ __forceinline__ int surrogate_return(const MlpDev* __restrict__ M, bo
     ep += surrogate_apply(F, eps, lnJ, dtau, mat, tau);
     return SUR_NETWORK;
   }
-  if (mine && inenv) {
+  if (mine && inenv && elastic) {
     elastic_tau(eps, mat, tau);
     return SUR_ELASTIC;
   }

@@ -216,6 +216,22 @@ class Engine:
                                             perm.ctypes.data_as(P)))
         return kin, kout, perm
 
+    def advance_frame(self, dt, **settings):
+        """One display frame of Simulation3.step for an all-solid scene
+        (mpm_advance_frame). Settings override the reference defaults by
+        field name of mpm_frame_config (bounds as bounds_lo / bounds_hi).
+        Returns (substeps, advanced time)."""
+        c = L.MpmFrameConfig()
+        self._check(self._lib.mpm_frame_defaults(self.h, C.byref(c)))
+        for k, v in settings.items():
+            if k in ("bounds_lo", "bounds_hi", "gravity_direction", "gravity_center"):
+                getattr(c, k)[:] = list(map(float, v))
+            else:
+                setattr(c, k, v)
+        r = L.MpmFrameReport()
+        self._check(self._lib.mpm_advance_frame(self.h, C.byref(c), float(dt), C.byref(r)))
+        return r.substeps, r.advanced
+
     def set_surrogate(self, material_index, params=None, norm=None):
         """Use a trained surrogate network for particles of one material
         (params None switches it off)."""

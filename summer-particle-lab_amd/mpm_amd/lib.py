@@ -69,11 +69,12 @@ class MpmConservation(C.Structure):
 class MpmMlpTrainConfig(C.Structure):
     _fields_ = [("samples", C.c_uint32), ("batch", C.c_uint32), ("epochs", C.c_uint32), ("seed", C.c_uint32),
                 ("lr", C.c_float), ("lr_final", C.c_float), ("strain_sigma", C.c_float), ("T_lo", C.c_float),
-                ("T_hi", C.c_float), ("dt", C.c_float), ("eps_max", C.c_float)]
+                ("T_hi", C.c_float), ("dt", C.c_float), ("eps_max", C.c_float), ("near_yield_fraction", C.c_float)]
 
 
 class MpmMlpNorm(C.Structure):
-    _fields_ = ([(k, C.c_float) for k in ("in_scale", "T_center", "T_scale", "out_scale", "eps_max", "T_lo", "T_hi")]
+    _fields_ = ([(k, C.c_float) for k in ("in_scale", "T_center", "T_scale", "out_scale", "eps_max", "T_lo", "T_hi",
+                                          "min_yield_ratio")]
                 + [("material", C.c_int32), ("model", C.c_int32)]
                 + [(k, C.c_float) for k in ("mu_over_E", "lambda_over_E", "A", "Tref", "Thot", "hotRatio")])
 
@@ -89,6 +90,18 @@ class MpmBatchTuning(C.Structure):
     _fields_ = [("buckets", C.c_int32), ("bucket_edge", C.c_int32 * 4), ("bucket_scenes", C.c_int32 * 4),
                 ("particles_per_group", C.c_int32 * 4), ("sort_every", C.c_int32), ("particle_blocks", C.c_int32),
                 ("ms_per_step", C.c_double), ("bucket_ms", C.c_double * 4)]
+
+
+class MpmFrameConfig(C.Structure):
+    _fields_ = [("time_step_scale", C.c_double), ("max_substeps", C.c_int32), ("velocity_limit", C.c_double),
+                ("bounds_lo", C.c_double * 3), ("bounds_hi", C.c_double * 3), ("anti_clump", C.c_int32),
+                ("volume_unit", C.c_double), ("gravity_mode", C.c_int32), ("gravity_strength", C.c_double),
+                ("gravity_direction", C.c_double * 3), ("gravity_center", C.c_double * 3),
+                ("gravity_softening", C.c_double)]
+
+
+class MpmFrameReport(C.Structure):
+    _fields_ = [("substeps", C.c_int32), ("advanced", C.c_double)]
 
 
 class MpmKernelTime(C.Structure):
@@ -110,6 +123,8 @@ SIGNATURES = {
     "mpm_refresh_diagnostics": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mpm_synchronize": (C.c_int, [_P]),
     "mpm_count": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "mpm_frame_defaults": (C.c_int, [_P, C.POINTER(MpmFrameConfig)]),
+    "mpm_advance_frame": (C.c_int, [_P, C.POINTER(MpmFrameConfig), C.c_double, C.POINTER(MpmFrameReport)]),
     "mpm_profile": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "mpm_kernel_times": (C.c_int, [_P, C.POINTER(MpmKernelTime), C.c_int32, C.POINTER(C.c_int32)]),
     "mpm_debug_bin": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),

@@ -34,11 +34,12 @@ class TrainConfig:
     T_hi: float = 900.0
     dt: float = 1e-4
     eps_max: float = 0.0  # <= 0: 4 sigma
+    near_yield_fraction: float = 0.3
 
     def to_c(self):
         c = L.MpmMlpTrainConfig()
         for k in ("samples", "batch", "epochs", "seed", "lr", "lr_final", "strain_sigma", "T_lo", "T_hi", "dt",
-                  "eps_max"):
+                  "eps_max", "near_yield_fraction"):
             setattr(c, k, getattr(self, k))
         return c
 

@@ -157,6 +157,34 @@ class MPM3 {
     if (!this.options.resident) this.download(s);
   }
 
+  // One frame of Simulation3.step for an all-solid scene, run on the device
+  // (src/three/simulation.js:305-388 with regularizeSolids,
+  // src/three/solid-sampling.js:10-42): adaptive substeps, particle gravity,
+  // bounds, velocity reconstruction and anti-clump. Reads s.settings,
+  // s.boundsLo/boundsHi and s.scene like the reference; returns
+  // {substeps, advanced} and advances s.time.
+  advanceFrame(s, dt = 1 / 60) {
+    if (!this.options.resident || !this.onDevice) this.upload(s);
+    const st = s.settings || {};
+    const DIRS = {down: [0, -1, 0], up: [0, 1, 0], left: [-1, 0, 0], right: [1, 0, 0], front: [0, 0, 1], back: [0, 0, -1]};
+    let mode = st.gravityMode === 'scene' || st.gravityMode === undefined
+      ? (s.scene === 'million' ? 'none' : 'uniform') : st.gravityMode;
+    if (st.gravity === false || st.gravityStrength === 0) mode = 'none';
+    const code = {none: 0, uniform: 1, attract: 2, repel: 3}[mode];
+    if (code === undefined) throw new Error(`gravity mode ${mode} is not supported on the device`);
+    const num = (v, d) => (Number.isFinite(v) ? v : d);
+    const settings = {timeStepScale: num(st.timeStepScale, 1), antiClump: st.solidAntiClump === false ? 0 : 1,
+      gravityMode: code, gravityStrength: num(st.gravityStrength, 1),
+      gravityDirection: DIRS[st.gravityDirection || 'down'], gravityCenter: st.gravityCenter || [0, 0, 0],
+      gravitySoftening: num(st.gravitySoftening, 8)};
+    if (s.boundsLo) settings.boundsLo = Array.from(s.boundsLo);
+    if (s.boundsHi) settings.boundsHi = Array.from(s.boundsHi);
+    const r = native.advanceFrame(this.engine, dt, settings);
+    if (!this.options.resident) this.download(s);
+    s.time = (s.time || 0) + r.advanced; s.substeps = r.substeps;
+    return r;
+  }
+
   refreshDiagnostics(s) {
     if (!this.options.resident || !this.onDevice) this.upload(s);
     const out = new Float64Array(this._ids.length);

@@ -309,6 +309,50 @@ static napi_value js_stable_dt(napi_env env, napi_callback_info info) {
   return r;
 }
 
+/* advanceFrame(engine, dt, settings) -> {substeps, advanced}: one frame of
+ * Simulation3.step for an all-solid scene (mpm_advance_frame). settings:
+ * timeStepScale, maxSubsteps, velocityLimit, boundsLo, boundsHi, antiClump,
+ * volumeUnit, gravityMode (0 none, 1 uniform, 2 attract, 3 repel),
+ * gravityStrength, gravityDirection, gravityCenter, gravitySoftening; missing
+ * fields keep the reference defaults. */
+static napi_value js_advance_frame(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  double dt = 0;
+  napi_get_value_double(env, argv[1], &dt);
+  mpm_frame_config c;
+  int rc = mpm_frame_defaults(e, &c);
+  if (rc) return throw_mpm(env, rc, e);
+  if (argc >= 3) {
+    napi_value o = argv[2];
+    c.time_step_scale = get_num(env, o, "timeStepScale", c.time_step_scale);
+    c.max_substeps = (int32_t)get_num(env, o, "maxSubsteps", c.max_substeps);
+    c.velocity_limit = get_num(env, o, "velocityLimit", c.velocity_limit);
+    get_vec(env, o, "boundsLo", c.bounds_lo, 3);
+    get_vec(env, o, "boundsHi", c.bounds_hi, 3);
+    c.anti_clump = (int32_t)get_num(env, o, "antiClump", c.anti_clump);
+    c.volume_unit = get_num(env, o, "volumeUnit", c.volume_unit);
+    c.gravity_mode = (int32_t)get_num(env, o, "gravityMode", c.gravity_mode);
+    c.gravity_strength = get_num(env, o, "gravityStrength", c.gravity_strength);
+    get_vec(env, o, "gravityDirection", c.gravity_direction, 3);
+    get_vec(env, o, "gravityCenter", c.gravity_center, 3);
+    c.gravity_softening = get_num(env, o, "gravitySoftening", c.gravity_softening);
+  }
+  mpm_frame_report r;
+  rc = mpm_advance_frame(e, &c, dt, &r);
+  if (rc) return throw_mpm(env, rc, e);
+  napi_value out, x;
+  CHECK_NAPI(env, napi_create_object(env, &out));
+  CHECK_NAPI(env, napi_create_int32(env, r.substeps, &x));
+  CHECK_NAPI(env, napi_set_named_property(env, out, "substeps", x));
+  CHECK_NAPI(env, napi_create_double(env, r.advanced, &x));
+  CHECK_NAPI(env, napi_set_named_property(env, out, "advanced", x));
+  return out;
+}
+
 static void set_num(napi_env env, napi_value obj, const char* key, double v) {
   napi_value x;
   napi_create_double(env, v, &x);
@@ -406,6 +450,7 @@ static napi_value init(napi_env env, napi_value exports) {
              {"upload", js_upload},
              {"download", js_download},
              {"step", js_step},
+             {"advanceFrame", js_advance_frame},
              {"synchronize", js_synchronize},
              {"stableDt", js_stable_dt},
              {"conservation", js_conservation},

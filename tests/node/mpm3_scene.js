@@ -18,14 +18,20 @@ for (let i = 0; i < n; i++) {
   [s.vx[i], s.vy[i], s.vz[i]] = Array.isArray(input.velocity[0]) ? input.velocity[i] : input.velocity;
   s.mass[i] = 1.1; s.volume[i] = 1; s.temperature[i] = input.temperature || 20; s.model[i] = input.models[i];
 }
+if (input.pinned) for (const i of input.pinned) s.pinned[i] = 1;
+if (input.part) for (let i = 0; i < n; i++) s.lab.part[i] = input.part[i];
+if (input.settings) Object.assign(s.settings, input.settings);
+if (input.boundsLo) { s.boundsLo = input.boundsLo; s.boundsHi = input.boundsHi; }
 const mpm = new MPM3(n, {sortEvery: 1});
 if (input.F) for (let i = 0; i < n; i++) mpm.F.set(input.F, i * 9);
 const dtStable = mpm.stableStep(s);
-for (let k = 0; k < input.steps; k++) mpm.step(s, input.dt);
+const frames = [];
+if (input.frames) for (let k = 0; k < input.frames; k++) frames.push(mpm.advanceFrame(s, input.dt));
+else for (let k = 0; k < input.steps; k++) mpm.step(s, input.dt);
 mpm.refreshDiagnostics(s);
 const out = {dtStable, x: Array.from(s.x), y: Array.from(s.y), z: Array.from(s.z), vx: Array.from(s.vx),
   vy: Array.from(s.vy), vz: Array.from(s.vz), F: Array.from(mpm.F.subarray(0, n * 9)),
   plastic: Array.from(mpm.plastic.subarray(0, n)), stress: Array.from(mpm.stress.subarray(0, n)),
-  minJ: mpm.minJ(s), conservation: mpm.conservation()};
+  minJ: mpm.minJ(s), conservation: mpm.conservation(), frames};
 mpm.destroy();
 console.log(JSON.stringify(out));

@@ -37,7 +37,7 @@ console.log(JSON.stringify({{names, code, typeErr}}));
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
     assert out["names"] == sorted(["create", "destroy", "setMaterials", "upload", "download", "step", "synchronize",
-                                   "stableDt", "conservation", "refreshDiagnostics", "lastError"])
+                                   "stableDt", "conservation", "refreshDiagnostics", "lastError", "advanceFrame"])
     assert out["code"] == "MPM1"
     assert "engine handle" in out["typeErr"]
 
@@ -79,3 +79,33 @@ def test_drop_in_mpm3_matches_oracle_on_reference_scene(gpu, tmp_path):
     assert got["conservation"]["gridMassQ"] == got["conservation"]["particleMassQ"]
     orc.refresh_diagnostics()
     np.testing.assert_allclose(np.array(got["stress"]), orc.stress, atol=3e-5, rtol=5e-3)
+
+
+@pytest.mark.gpu
+def test_drop_in_advance_frame_matches_oracle(gpu, tmp_path):
+    """MPM3.advanceFrame (Simulation3.step's solid substeps on the device) on
+    two parts with a clumped pair and a pinned particle, two frames of 1/60,
+    against the oracle's frame restatement: the same substep counts and the
+    positions within the reference's bar."""
+    pts, models = four_blocks()
+    pts = pts[:54].copy()  # two blocks
+    part = [0] * 27 + [1] * 27
+    pts[1] = pts[0] + [0.6, 0, 0]
+    inp = dict(points=pts.tolist(), velocity=[0, -30, 0], models=[0] * 54, dt=1 / 60, frames=2, pinned=[13],
+               part=part, settings=dict(gravity=True, gravityStrength=1, timeStepScale=1, solidAntiClump=True),
+               boundsLo=[-60, -2, -60], boundsHi=[60, 68, 60])
+    path = tmp_path / "frame.json"
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout)
+    pinned = np.zeros(54, np.uint8)
+    pinned[13] = 1
+    orc = o.Scene(pts, (0, -30, 0), 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8, part=part,
+                  pinned=pinned)
+    for k in range(2):
+        n, adv = orc.advance_frame(1 / 60, bounds=((-60, -2, -60), (60, 68, 60)))
+        assert got["frames"][k]["substeps"] == n
+    x = np.stack([got["x"], got["y"], got["z"]], 1)
+    np.testing.assert_allclose(x, orc.positions(), atol=.003, rtol=.001)
