@@ -56,10 +56,14 @@ enum { MPM_F64 = 0, MPM_F32 = 1 };
 
 /* Constitutive models, numbered as the reference's MODELS table
  * (src/three/constitutive.js:6-19). Supported: 0 fixed-corotated elastic,
- * 4 ideal J2, 5 linear-hardening J2, 6 Johnson-Cook, 7 simplified
- * Johnson-Cook, 10 thermal J2, 11 Norton-Arrhenius creep. */
+ * 1 Neo-Hookean, 2 snow, 3 sand (Drucker-Prager), 4 ideal J2,
+ * 5 linear-hardening J2, 6 Johnson-Cook, 7 simplified Johnson-Cook,
+ * 10 thermal J2, 11 Norton-Arrhenius creep. */
 enum {
   MPM_MODEL_FIXED_COROTATED = 0,
+  MPM_MODEL_NEO_HOOKEAN = 1,
+  MPM_MODEL_SNOW = 2,
+  MPM_MODEL_SAND = 3,
   MPM_MODEL_J2 = 4,
   MPM_MODEL_J2_HARDENING = 5,
   MPM_MODEL_JOHNSON_COOK = 6,
@@ -101,6 +105,11 @@ typedef struct {
   float th_Tref, th_Thot, th_hotRatio, th_A, th_H, th_rate, th_Q, th_n, th_stress, th_kelvin_offset;
   /* J2 / Johnson-Cook (models 4-7) */
   float pl_A, pl_B, pl_n, pl_C, pl_rate0, pl_Tref, pl_Tm, pl_m;
+  /* snow (2): stretch yield strain; sand (3): friction angle in degrees.
+   * 0 selects the reference defaults .025 and 35; other values are clamped
+   * to the reference's ranges [.005, .08] and [15, 50]
+   * (src/three/simulation.js:40). */
+  float yield_strain, friction;
 } mpm_material;
 
 /* Host view of particle state (structure of arrays). On upload every pointer
@@ -118,6 +127,10 @@ typedef struct {
   uint8_t *material;              /* index into the material table [count] */
   uint8_t *pinned;                /* 1 = pinned (MPM3 skips G2P) [count] */
   uint8_t *cache_valid;           /* optional [count] (MPM3.cacheValid) */
+  void *jp;                       /* optional [count]: snow's plastic volume
+                                     ratio MPM3.Jp (1 when NULL) */
+  void *dilation;                 /* optional [count]: sand's MPM3.dilation
+                                     (0 when NULL) */
 } mpm_particles;
 
 /* FP64 conservation report. Grid totals are those of the most recent P2G. */

@@ -375,7 +375,7 @@ int mpm_batch_set_materials(mpm_batch* b, const mpm_material* mats, int32_t coun
   std::vector<DevMaterial> dm(count);
   for (int i = 0; i < count; ++i) {
     const int m = mats[i].model;
-    if (!(m == 0 || (m >= 4 && m <= 7) || m == 10 || m == 11))
+    if (!((m >= 0 && m <= 7) || m == 10 || m == 11))
       return bfail(b, MPM_ERR_UNSUPPORTED, "material %d: model %d is not supported", i, m);
     if (!(mats[i].E > 0) || !(mats[i].mu >= 0) || !std::isfinite(mats[i].lambda))
       return bfail(b, MPM_ERR_INVALID_ARGUMENT, "material %d: E must be > 0 and mu >= 0", i);

@@ -24,7 +24,8 @@ struct DevMaterial {
   float E, mu, lambda, G, lr;
   float th_Tref, th_Thot, th_hotRatio, th_A, th_H, th_rate, th_Q, th_n, th_stress, th_kelvin;
   float pl_A, pl_B, pl_n, pl_C, pl_rate0, pl_Tref, pl_Tm, pl_m;
-  float pad[2];
+  float sn_ys;     // snow stretch yield strain
+  float sd_alpha;  // sand: sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
 };
 
 struct M3 {
@@ -262,27 +263,98 @@ __device__ __forceinline__ float return_map(float e[3], int model, float ep, flo
   return inc;
 }
 
-// Plastic projection of F (models 4-7, 10, 11): SVD, return on log stretches,
-// recompose F, and the Hencky stress on the returned principal directions,
-// 2 mu e + lambda tr(e), as the next step's stress (6-vector).
-__device__ __forceinline__ float plastic_project(M3& F, float ep, float dt, float T, const DevMaterial& m,
-                                                 float tau[6]) {
+// Snow's compaction hardening clamp(exp(10 (1 - Jp)), .25, 4)
+// (src/three/constitutive.js:48).
+__device__ __forceinline__ float snow_hardening(float jp) {
+  return fminf(fmaxf(__expf(10.f * (1.f - jp)), .25f), 4.f);
+}
+
+// Neo-Hookean Kirchhoff stress mu (F F^T - I) + lambda ln J I
+// (src/three/constitutive.js:88-93).
+__device__ __forceinline__ void neo_hookean_tau(const M3& F, float mu, float lambda, float t[6]) {
+  const float p = lambda * logf(fmaxf(det3(F), 1e-6f)) - mu;
+  t[0] = mu * (F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) + p;
+  t[1] = mu * (F[3] * F[3] + F[4] * F[4] + F[5] * F[5]) + p;
+  t[2] = mu * (F[6] * F[6] + F[7] * F[7] + F[8] * F[8]) + p;
+  t[3] = mu * (F[0] * F[3] + F[1] * F[4] + F[2] * F[5]);
+  t[4] = mu * (F[0] * F[6] + F[1] * F[7] + F[2] * F[8]);
+  t[5] = mu * (F[3] * F[6] + F[4] * F[7] + F[5] * F[8]);
+}
+
+// Plastic projection of F (models 2-7, 10, 11): SVD, return on the principal
+// stretches, recompose F, and the stress of the returned state on the
+// returned principal directions as the next step's stress (6-vector):
+// snow 2 mu (s - 1) s + lambda (J - 1) J times its hardening, the others
+// Hencky 2 mu e + lambda tr(e) (src/three/constitutive.js:52-86). jp and dil
+// carry snow's Jp and sand's dilation. Returns the plastic strain increment.
+__device__ __forceinline__ float plastic_project(M3& F, float ep, float& jp, float& dil, float dt, float T,
+                                                 const DevMaterial& m, float tau[6]) {
   float U[9], V[9], s[3];
   svd3(F, U, V, s);
-  float e[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) e[k] = logf(fmaxf(s[k], 1e-6f));
-  const float inc = return_map(e, m.model, ep, dt, T, m);
   float sp[3];
+  float inc;
+  if (m.model == 2) {
+    // Snow: clamp the stretches to [1 - ys, 1 + 0.3 ys]; Jp takes the
+    // clamped-away volume (src/three/constitutive.js:56-58).
 #pragma unroll
-  for (int k = 0; k < 3; ++k) sp[k] = expf(e[k]);
+    for (int k = 0; k < 3; ++k) sp[k] = fminf(fmaxf(s[k], 1.f - m.sn_ys), 1.f + m.sn_ys * .3f);
+    jp = fminf(fmaxf(jp * (s[0] * s[1] * s[2]) / (sp[0] * sp[1] * sp[2]), .6f), 20.f);
+  } else if (m.model == 3) {
+    // Sand: non-cohesive logarithmic Drucker-Prager with a dilation history
+    // (src/three/constitutive.js:65-73); (3 lambda + 2 mu) / (2 mu) is the
+    // reference's 2.5 at lambda = mu.
+    float e[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e[k] = logf(fmaxf(s[k], 1e-6f));
+    const float tr = e[0] + e[1] + e[2] + dil;
+    if (tr >= 0.f) {
+      sp[0] = sp[1] = sp[2] = 1.f;
+      dil = tr;
+    } else {
+      dil = 0.f;
+      float dv[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dv[k] = e[k] - tr * (1.f / 3.f);
+      const float len = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+      const float delta = fmaxf(0.f, len + (1.5f * m.lr + 1.f) * tr * m.sd_alpha);
+      const float f = delta / (len != 0.f ? len : 1.f);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sp[k] = expf(e[k] - f * dv[k]);
+    }
+  } else {
+    float e[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e[k] = logf(fmaxf(s[k], 1e-6f));
+    inc = return_map(e, m.model, ep, dt, T, m);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sp[k] = expf(e[k]);
+  }
+  if (m.model == 2 || m.model == 3) {
+    const float l0 = logf(s[0] / sp[0]), l1 = logf(s[1] / sp[1]), l2 = logf(s[2] / sp[2]);
+    inc = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+  }
   compose(U, sp, V, F);
-  const float tr = e[0] + e[1] + e[2];
   float d[3];
+  if (m.model == 2) {
+    const float J = sp[0] * sp[1] * sp[2], h = snow_hardening(jp);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) d[k] = 2.f * m.mu * e[k] + m.lambda * tr;
+    for (int k = 0; k < 3; ++k) d[k] = h * (2.f * m.mu * (sp[k] - 1.f) * sp[k] + m.lambda * (J - 1.f) * J);
+  } else {
+    const float e0 = logf(sp[0]), e1 = logf(sp[1]), e2 = logf(sp[2]);
+    const float tr = e0 + e1 + e2;
+    d[0] = 2.f * m.mu * e0 + m.lambda * tr;
+    d[1] = 2.f * m.mu * e1 + m.lambda * tr;
+    d[2] = 2.f * m.mu * e2 + m.lambda * tr;
+  }
   sym_from_principal(U, d, tau);
   return inc;
+}
+
+// Compatibility form for the models without snow or sand state.
+__device__ __forceinline__ float plastic_project(M3& F, float ep, float dt, float T, const DevMaterial& m,
+                                                 float tau[6]) {
+  float jp = 1.f, dil = 0.f;
+  return plastic_project(F, ep, jp, dil, dt, T, m, tau);
 }
 
 // Hencky stress 2 mu log(sigma) + lambda tr from F's own decomposition (the
@@ -298,6 +370,20 @@ __device__ __forceinline__ void hencky_tau(const M3& F, const DevMaterial& m, fl
 #pragma unroll
   for (int k = 0; k < 3; ++k) d[k] = 2.f * m.mu * e[k] + m.lambda * tr;
   sym_from_principal(U, d, tau);
+}
+
+// Step stress from F alone (no cached return): constitutiveStress with the
+// material's moduli, snow times its hardening (src/three/mpm.js:33-37,
+// src/three/constitutive.js:87-99).
+__device__ __forceinline__ void elastic_tau(const M3& F, const DevMaterial& m, float jp, float tau[6]) {
+  if (m.model == 1) {
+    neo_hookean_tau(F, m.mu, m.lambda, tau);
+  } else if (m.model == 0 || m.model == 2) {
+    const float h = m.model == 2 ? snow_hardening(jp) : 1.f;
+    fixed_corotated_tau(F, m.mu * h, m.lambda * h, tau);
+  } else {
+    hencky_tau(F, m, tau);
+  }
 }
 
 }  // namespace mpm

@@ -159,13 +159,20 @@ DevMaterial to_dev_material(const mpm_material& m) {
   d.pl_Tref = m.pl_Tref;
   d.pl_Tm = m.pl_Tm;
   d.pl_m = m.pl_m;
+  // Snow and sand parameters with the reference's defaults and ranges
+  // (src/three/simulation.js:36-40).
+  const float ys = m.yield_strain > 0 ? std::min(std::max(m.yield_strain, .005f), .08f) : .025f;
+  const float fr = m.friction > 0 ? std::min(std::max(m.friction, 15.f), 50.f) : 35.f;
+  const double sn = std::sin(fr * M_PI / 180.0);
+  d.sn_ys = ys;
+  d.sd_alpha = (float)(std::sqrt(2.0 / 3.0) * 2 * sn / (3 - sn));
   return d;
 }
 }  // namespace mpm
 
 namespace {
 
-bool supported_model(int m) { return m == 0 || (m >= 4 && m <= 7) || m == 10 || m == 11; }
+bool supported_model(int m) { return (m >= 0 && m <= 7) || m == 10 || m == 11; }
 
 // Bin, sort and permute the resident particles into cell order.
 int rebin(mpm_engine* e) {

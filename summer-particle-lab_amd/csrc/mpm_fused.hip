@@ -148,7 +148,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   for (int k = 0; k < 9; ++k) F[k] = ld(P, cap, P_F0 + k, ic);
   float v[3] = {0.f, 0.f, 0.f}, Cm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float tau[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  float ep = 0.f;
+  float ep = 0.f, jp = 1.f, dil = 0.f;
   uint32_t new_flags = flags;
 
   if (MODE == 0 || pinned) {
@@ -158,13 +158,11 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     v[2] = ld(P, cap, P_VZ, ic);
 #pragma unroll
     for (int k = 0; k < 9; ++k) Cm[k] = ld(P, cap, P_C0 + k, ic);
-    if (mat.model < 2) {
-      fixed_corotated_tau(F, mat.mu, mat.lambda, tau);
-    } else if (flags & FLAG_CACHE) {
+    if (mat.model >= 2 && (flags & FLAG_CACHE)) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) tau[k] = ld(P, cap, P_T0 + k, ic);
     } else {
-      hencky_tau(F, mat, tau);
+      elastic_tau(F, mat, mat.model == 2 ? ld(P, cap, P_JP, ic) : 1.f, tau);
     }
   }
 
@@ -282,10 +280,14 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     if (gather && !done) {
       if (mat.model >= 2) {
         ep = ld(P, cap, P_EP, ic);
-        ep += plastic_project(Fn, ep, dt, ld(P, cap, P_TEMP, ic), mat, tau);
+        if (mat.model <= 3) {
+          jp = mat.model == 2 ? ld(P, cap, P_JP, ic) : 1.f;
+          dil = mat.model == 3 ? ld(P, cap, P_DIL, ic) : 0.f;
+        }
+        ep += plastic_project(Fn, ep, jp, dil, dt, ld(P, cap, P_TEMP, ic), mat, tau);
         new_flags |= FLAG_CACHE;
       } else if (MODE == 1) {
-        fixed_corotated_tau(Fn, mat.mu, mat.lambda, tau);
+        elastic_tau(Fn, mat, 1.f, tau);
       }
       F = Fn;
     }
@@ -297,6 +299,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 #pragma unroll
       for (int k = 0; k < 9; ++k) st(P, cap, P_F0 + k, i, F[k]);
       if (mat.model >= 2) st(P, cap, P_EP, i, ep);
+      if (mat.model == 2) st(P, cap, P_JP, i, jp);
+      if (mat.model == 3) st(P, cap, P_DIL, i, dil);
       if (new_flags != flags) meta[i] = (mt & 255u) | (new_flags << 8);
       if (MODE == 2) {
         st(P, cap, P_VX, i, v[0]);

@@ -81,6 +81,8 @@ inline std::string pack_particles(const mpm_particles* p, size_t nmaterials, flo
     set(P_VOL, vol);
     set(P_TEMP, rdv(p->temperature, dt, i));
     set(P_EP, rdv(p->plastic, dt, i));
+    set(P_JP, p->jp ? rdv(p->jp, dt, i) : 1.0);
+    set(P_DIL, p->dilation ? rdv(p->dilation, dt, i) : 0.0);
     ids[i] = id0 + i;
     meta[i] = (uint32_t)p->material[i] | (flags << 8);
   }
@@ -120,6 +122,8 @@ inline void unpack_particles(const float* planes, size_t stride, uint32_t slot0,
     if (p->volume) wrv(p->volume, dt, i, get(P_VOL));
     if (p->temperature) wrv(p->temperature, dt, i, get(P_TEMP));
     if (p->plastic) wrv(p->plastic, dt, i, get(P_EP));
+    if (p->jp) wrv(p->jp, dt, i, get(P_JP));
+    if (p->dilation) wrv(p->dilation, dt, i, get(P_DIL));
     if (p->cached_tau) {
       const double t[6] = {get(P_T0), get(P_T0 + 1), get(P_T0 + 2), get(P_T0 + 3), get(P_T0 + 4), get(P_T0 + 5)};
       const double full[9] = {t[0], t[3], t[4], t[3], t[1], t[5], t[4], t[5], t[2]};

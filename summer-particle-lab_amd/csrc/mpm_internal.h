@@ -25,6 +25,8 @@ enum Plane : int {
   P_VOL,
   P_TEMP,
   P_EP,
+  P_JP,     // snow: plastic volume ratio Jp (src/three/mpm.js:9)
+  P_DIL,    // sand: dilation history
   NPLANES
 };
 

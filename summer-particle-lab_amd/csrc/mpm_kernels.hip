@@ -94,13 +94,11 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const float* __restrict__ P, 
 // Kirchhoff stress used by this step's P2G (6-vector).
 __device__ __forceinline__ void step_stress(const M3& F, const DevMaterial& m, uint32_t flags,
                                             const float* __restrict__ P, uint32_t cap, uint32_t i, float tau[6]) {
-  if (m.model < 2) {
-    fixed_corotated_tau(F, m.mu, m.lambda, tau);
-  } else if (flags & FLAG_CACHE) {
+  if (m.model >= 2 && (flags & FLAG_CACHE)) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) tau[k] = plane(P, cap, P_T0 + k, i);
   } else {
-    hencky_tau(F, m, tau);
+    elastic_tau(F, m, m.model == 2 ? plane(P, cap, P_JP, i) : 1.f, tau);
   }
 }
 
@@ -557,10 +555,19 @@ __global__ __launch_bounds__(BLOCK) void k_stable(const float* __restrict__ P, c
       col = fmaxf(col, c);
     }
     const float rho = plane(P, cap, P_MASS, i) / plane(P, cap, P_VOL, i);
-    const float c2 = (mat.lambda + 2.f * mat.mu) / rho;
+    const float hard = mat.model == 2 ? snow_hardening(plane(P, cap, P_JP, i)) : 1.f;
+    const float c2 = (mat.lambda + 2.f * mat.mu) * hard / rho;
+    // Neo-Hookean stiffens in compression: 1 / max(.05, J).
+    float comp = 1.f;
+    if (mat.model == 1) {
+      M3 F;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
+      comp = 1.f / fmaxf(.05f, det3(F));
+    }
     const float vx = plane(P, cap, P_VX, i), vy = plane(P, cap, P_VY, i), vz = plane(P, cap, P_VZ, i);
     const float speed = sqrtf(vx * vx + vy * vy + vz * vz);
-    mx = fmax(mx, (double)(speed + sqrtf(c2) * fmaxf(1.f, row * col)));
+    mx = fmax(mx, (double)(speed + sqrtf(c2) * fmaxf(fmaxf(1.f, row * col), comp)));
     grad = fmax(grad, (double)gsum);
   }
   const double a = -block_min(-mx, s);
@@ -606,9 +613,9 @@ __global__ __launch_bounds__(BLOCK) void k_return_batch(DevMaterial mat, uint32_
   float tau[6];
   float e = ep[i];
   if (mat.model >= 2) {
-    e += plastic_project(f, e, dt, T[i], mat, tau);
+    e += plastic_project(f, e, dt, T[i], mat, tau);  // snow Jp = 1, sand dilation = 0
   } else {
-    fixed_corotated_tau(f, mat.mu, mat.lambda, tau);
+    elastic_tau(f, mat, 1.f, tau);
   }
 #pragma unroll
   for (int k = 0; k < 9; ++k) F_out[(size_t)i * 9 + k] = f[k];
@@ -627,10 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_stress_batch(DevMaterial mat, uint32_
 #pragma unroll
   for (int k = 0; k < 9; ++k) f[k] = F[(size_t)i * 9 + k];
   float tau[6];
-  if (mat.model < 2)
-    fixed_corotated_tau(f, mat.mu, mat.lambda, tau);
-  else
-    hencky_tau(f, mat, tau);
+  elastic_tau(f, mat, 1.f, tau);
 #pragma unroll
   for (int k = 0; k < 6; ++k) tau_out[(size_t)i * 6 + k] = tau[k];
 }

@@ -253,7 +253,7 @@ __device__ __forceinline__ float surrogate_apply(M3& F, const float eps[6], floa
 }
 
 // Elastic trial state: tau = 2 mu eps + lambda tr(eps) I, F unchanged.
-__device__ __forceinline__ void elastic_tau(const float eps[6], const DevMaterial& m, float tau[6]) {
+__device__ __forceinline__ void hencky_tau_of_strain(const float eps[6], const DevMaterial& m, float tau[6]) {
   const float l = m.lambda * (eps[0] + eps[1] + eps[2]);
 #pragma unroll
   for (int k = 0; k < 6; ++k) tau[k] = 2.f * m.mu * eps[k] + (k < 3 ? l : 0.f);
@@ -303,7 +303,7 @@ __device__ __forceinline__ int surrogate_return(const MlpDev* __restrict__ M, bo
     return SUR_NETWORK;
   }
   if (mine && inenv && elastic) {
-    elastic_tau(eps, mat, tau);
+    hencky_tau_of_strain(eps, mat, tau);
     return SUR_ELASTIC;
   }
   return mine ? SUR_FALLBACK : SUR_NONE;

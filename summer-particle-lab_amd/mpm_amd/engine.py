@@ -26,6 +26,8 @@ class Material:
     lam: float = 0.4
     thermal: dict = field(default_factory=dict)
     plastic: dict = field(default_factory=dict)
+    yield_strain: float = 0.0  # snow; 0: the reference default .025
+    friction: float = 0.0      # sand, degrees; 0: the reference default 35
 
     @staticmethod
     def from_E_nu(model, E, nu, **kw):
@@ -42,6 +44,7 @@ class Material:
         m = L.MpmMaterial()
         m.model = self.model
         m.E, m.mu, m.lambda_ = self.E, self.mu, self.lam
+        m.yield_strain, m.friction = self.yield_strain, self.friction
         for k, v in th.items():
             setattr(m, "th_" + k, v)
         for k, v in pl.items():
@@ -70,7 +73,7 @@ def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, grav
 
 
 def particle_view(x, v, mass, volume, *, F=None, Cmat=None, temperature=None, plastic=None, material=None,
-                  pinned=None, cached_tau=None, cache_valid=None, dtype=np.float64):
+                  pinned=None, cached_tau=None, cache_valid=None, jp=None, dilation=None, dtype=np.float64):
     """An mpm_particles view of host arrays; returns (view, arrays to keep alive)."""
     x = np.asarray(x, np.float64)
     n = len(x)
@@ -93,6 +96,10 @@ def particle_view(x, v, mass, volume, *, F=None, Cmat=None, temperature=None, pl
     if cached_tau is not None:
         keep["cached_tau"] = np.ascontiguousarray(np.asarray(cached_tau).reshape(n * 9), f)
         keep["cache_valid"] = np.ascontiguousarray(np.broadcast_to(np.asarray(cache_valid, np.uint8), (n,)))
+    if jp is not None:
+        keep["jp"] = arr(jp, 1.0)
+    if dilation is not None:
+        keep["dilation"] = arr(dilation, 0.0)
     p = L.MpmParticles()
     p.count = n
     p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
@@ -108,7 +115,7 @@ def download_view(n, dtype=np.float64):
                vz=np.zeros(n, f), F=np.zeros(9 * n, f), C=np.zeros(9 * n, f), mass=np.zeros(n, f),
                volume=np.zeros(n, f), temperature=np.zeros(n, f), plastic=np.zeros(n, f),
                cached_tau=np.zeros(9 * n, f), material=np.zeros(n, np.uint8), pinned=np.zeros(n, np.uint8),
-               cache_valid=np.zeros(n, np.uint8))
+               cache_valid=np.zeros(n, np.uint8), jp=np.zeros(n, f), dilation=np.zeros(n, f))
     p = L.MpmParticles()
     p.count = n
     p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32

@@ -37,7 +37,8 @@ class MpmConfig(C.Structure):
 _MAT_FLOATS = ["E", "mu", "lambda_",
                "th_Tref", "th_Thot", "th_hotRatio", "th_A", "th_H", "th_rate", "th_Q", "th_n", "th_stress",
                "th_kelvin_offset",
-               "pl_A", "pl_B", "pl_n", "pl_C", "pl_rate0", "pl_Tref", "pl_Tm", "pl_m"]
+               "pl_A", "pl_B", "pl_n", "pl_C", "pl_rate0", "pl_Tref", "pl_Tm", "pl_m",
+               "yield_strain", "friction"]
 
 
 class MpmMaterial(C.Structure):
@@ -53,6 +54,7 @@ class MpmParticles(C.Structure):
         ("mass", C.c_void_p), ("volume", C.c_void_p), ("temperature", C.c_void_p),
         ("plastic", C.c_void_p), ("cached_tau", C.c_void_p),
         ("material", C.c_void_p), ("pinned", C.c_void_p), ("cache_valid", C.c_void_p),
+        ("jp", C.c_void_p), ("dilation", C.c_void_p),
     ]
 
 

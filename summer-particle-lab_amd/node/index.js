@@ -23,7 +23,7 @@ const native = require('./mpm_napi.node');
 const DEFAULT_DOMAIN = Object.freeze({lower: [-64, -48, -64], upper: [64, 72, 64], cellSize: 8, particlesPerAxis: 2});
 const THERMAL_KEYS = ['Tref', 'Thot', 'hotRatio', 'A', 'H', 'rate', 'Q', 'n', 'stress'];
 const PLASTIC_KEYS = ['A', 'B', 'n', 'C', 'rate0', 'Tref', 'Tm', 'm'];
-const SUPPORTED = new Set([0, 4, 5, 6, 7, 10, 11]);
+const SUPPORTED = new Set([0, 1, 2, 3, 4, 5, 6, 7, 10, 11]);
 
 function identity() { return [1, 0, 0, 0, 1, 0, 0, 0, 1]; }
 function det(a) {
@@ -77,7 +77,10 @@ class MPM3 {
       if (!SUPPORTED.has(model)) throw new Error(`model ${model} is not supported by the MI355X engine`);
       const part = s.lab && s.lab.part ? s.lab.part[p] : 0;
       const E = s.mass[p] / (64 * s.volume[p]) * 16000 * ((s.settings && s.settings.stiffness) || 1);
-      const key = `${model}|${part}|${E}`;
+      // Snow's yield strain and sand's friction angle are per particle in the
+      // reference (src/three/simulation.js:40) and constant within a part.
+      const ys = s.yieldStrain ? s.yieldStrain[p] : 0.025, fr = s.friction ? s.friction[p] : 35;
+      const key = `${model}|${part}|${E}|${ys}|${fr}`;
       let m = keys.get(key);
       if (m === undefined) {
         m = materials.length;
@@ -88,7 +91,7 @@ class MPM3 {
         const thermal = {}, plastic = {};
         for (const k of THERMAL_KEYS) if (Number.isFinite(th[k])) thermal[k] = th[k];
         for (const k of PLASTIC_KEYS) if (Number.isFinite(pl[k])) plastic[k] = pl[k];
-        materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic});
+        materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic, yieldStrain: ys, friction: fr});
       }
       material[ids.length] = m;
       ids.push(p);
@@ -98,7 +101,7 @@ class MPM3 {
 
   _view(n) {
     const v = {count: n};
-    for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature', 'plastic']) v[k] = new Float64Array(n);
+    for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature', 'plastic', 'jp', 'dilation']) v[k] = new Float64Array(n);
     v.F = new Float64Array(n * 9); v.C = new Float64Array(n * 9); v.cachedTau = new Float64Array(n * 9);
     v.pinned = new Uint8Array(n); v.cacheValid = new Uint8Array(n); v.material = new Uint8Array(n);
     return v;
@@ -109,16 +112,20 @@ class MPM3 {
     const {ids, materials, material} = this._gather(s);
     const n = ids.length, v = this._view(n);
     this._scale = new Float64Array(n);
+    this._snow = new Float64Array(n);
     for (let k = 0; k < n; k++) {
       const p = ids[k];
       v.x[k] = s.x[p]; v.y[k] = s.y[p]; v.z[k] = s.z[p]; v.vx[k] = s.vx[p]; v.vy[k] = s.vy[p]; v.vz[k] = s.vz[p];
       v.mass[k] = s.mass[p]; v.volume[k] = 64 * s.volume[p];
       v.temperature[k] = s.temperature ? s.temperature[p] : 20; v.plastic[k] = this.plastic[p];
+      v.jp[k] = this.Jp[p]; v.dilation[k] = this.dilation[p];
       v.pinned[k] = s.pinned ? s.pinned[p] : 0; v.material[k] = material[k];
       v.cacheValid[k] = this.cacheValid[p];
-      // Cached return stress: unit-modulus in the reference, absolute here.
-      const E = materials[material[k]].E;
-      this._scale[k] = 0.4 * E;
+      // Cached return stress: unit-modulus in the reference, absolute here
+      // (and for snow times its hardening, src/three/mpm.js:35-37).
+      const mat = materials[material[k]];
+      this._snow[k] = mat.model === 2 ? Math.min(4, Math.max(.25, Math.exp(10 * (1 - this.Jp[p])))) : 0;
+      this._scale[k] = 0.4 * mat.E * (this._snow[k] || 1);
       for (let j = 0; j < 9; j++) {
         v.F[k * 9 + j] = this.F[p * 9 + j]; v.C[k * 9 + j] = this.C[p * 9 + j];
         v.cachedTau[k * 9 + j] = this.cachedTau[p * 9 + j] * this._scale[k];
@@ -138,9 +145,13 @@ class MPM3 {
       const p = ids[k];
       s.x[p] = v.x[k]; s.y[p] = v.y[k]; s.z[p] = v.z[k]; s.vx[p] = v.vx[k]; s.vy[p] = v.vy[k]; s.vz[p] = v.vz[k];
       this.plastic[p] = v.plastic[k]; this.cacheValid[p] = v.cacheValid[k];
+      this.Jp[p] = v.jp[k]; this.dilation[p] = v.dilation[k];
+      // The device's cached snow stress carries the hardening of the new Jp.
+      const scale = this._snow && this._snow[k] ? this._scale[k] / this._snow[k] *
+        Math.min(4, Math.max(.25, Math.exp(10 * (1 - v.jp[k])))) : this._scale[k];
       for (let j = 0; j < 9; j++) {
         this.F[p * 9 + j] = v.F[k * 9 + j]; this.C[p * 9 + j] = v.C[k * 9 + j];
-        if (v.cacheValid[k]) this.cachedTau[p * 9 + j] = v.cachedTau[k * 9 + j] / this._scale[k];
+        if (v.cacheValid[k]) this.cachedTau[p * 9 + j] = v.cachedTau[k * 9 + j] / scale;
       }
     }
   }

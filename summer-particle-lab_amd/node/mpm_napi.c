@@ -151,6 +151,8 @@ static napi_value js_set_materials(napi_env env, napi_callback_info info) {
     o->E = (float)get_num(env, m, "E", 1);
     o->mu = (float)get_num(env, m, "mu", .4);
     o->lambda = (float)get_num(env, m, "lambda", .4);
+    o->yield_strain = (float)get_num(env, m, "yieldStrain", 0);
+    o->friction = (float)get_num(env, m, "friction", 0);
     /* Reference defaults: src/three/thermal-plasticity.js:3, metal-plasticity.js:3 */
     o->th_Tref = 20; o->th_Thot = 180; o->th_hotRatio = .08f; o->th_A = .04f; o->th_H = .04f;
     o->th_rate = .0005f; o->th_Q = 35000; o->th_n = 3; o->th_stress = .04f; o->th_kelvin_offset = 273.15f;
@@ -201,7 +203,8 @@ static int particle_view(napi_env env, napi_value obj, mpm_particles* p) {
                 {"vz", &p->vz, 0}, {"F", &p->F, 0}, {"C", &p->C, 0}, {"mass", &p->mass, 0},
                 {"volume", &p->volume, 0}, {"temperature", &p->temperature, 0}, {"plastic", &p->plastic, 0},
                 {"cachedTau", &p->cached_tau, 0}, {"material", (void**)&p->material, 1},
-                {"pinned", (void**)&p->pinned, 1}, {"cacheValid", (void**)&p->cache_valid, 1}};
+                {"pinned", (void**)&p->pinned, 1}, {"cacheValid", (void**)&p->cache_valid, 1},
+                {"jp", &p->jp, 0}, {"dilation", &p->dilation, 0}};
   for (size_t k = 0; k < sizeof fields / sizeof fields[0]; ++k) {
     bool has = false;
     napi_value v;

@@ -72,7 +72,7 @@ def test_invalid_arguments_rejected_before_any_device_call():
     assert lib.mpm_step(None, 1e-3, 1) == L.MPM_ERR_INVALID_ARGUMENT
     assert lib.mpm_destroy(None) == L.MPM_OK
     m = L.MpmMaterial()
-    m.model = 3  # sand: not supported by the engine
+    m.model = 8  # pressure plasticity: not supported by the engine
     assert lib.mpm_return_map_batch(C.byref(m), 0, 0.0, None, None, None, None, None, None) == L.MPM_ERR_UNSUPPORTED
 
 
