@@ -303,7 +303,12 @@ class Scene:
         self.tau = np.zeros(9 * n)
         self.cachedTau = np.zeros(9 * n)
         self.cacheValid = np.zeros(n, np.uint8)
-        parts = parts or [make_part()]
+        need = int(self.part.max()) + 1 if n else 1
+        if self.part.min(initial=0) < 0:
+            raise ValueError("negative part index")
+        parts = parts or [make_part() for _ in range(need)]
+        if len(parts) < need:
+            raise ValueError(f"part index {need - 1} without a part record ({len(parts)} given)")
         self._parts = (OrcPart * len(parts))(*parts)
         d = domain(lower, upper, h, ppa, strict=strict_domain)
         self.origin = d["origin"]
@@ -347,7 +352,7 @@ class Scene:
             lib().orc_apply_particle_gravity(C.byref(self.s), _p(g), dt)
         rc = lib().orc_step(C.byref(self.s), dt)
         if rc != 0:
-            raise ValueError("unsupported model in scene")
+            raise ValueError("unsupported model or part index in scene")
 
     def advance_frame(self, dt, *, time_step_scale=1.0, max_substeps=48, velocity_limit=160.0, bounds=None,
                       anti_clump=True, gravity_mode=1, gravity_strength=1.0, gravity_direction=(0, -1, 0),
@@ -365,7 +370,7 @@ class Scene:
         adv = D(0)
         n = lib().orc_advance_frame(C.byref(self.s), C.byref(f), dt, C.byref(adv))
         if n < 0:
-            raise ValueError("unsupported model in scene")
+            raise ValueError("unsupported model or part index in scene")
         return n, adv.value
 
     def p2g(self, dt):

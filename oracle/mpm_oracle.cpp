@@ -620,7 +620,14 @@ void orc_apply_particle_gravity(const OrcScene* s, const double* g, double dt) {
 // Every particle in the scene is a solid (no fluid, no fracture, no separate
 // part grids, no platen). Returns -1 for an unsupported model.
 // src/three/mpm.js:56-134
+static bool parts_valid(const OrcScene* s) {
+  for (int p = 0; p < s->count; ++p)
+    if (s->part[p] < 0 || s->part[p] >= s->nparts) return false;
+  return true;
+}
+
 int orc_step(const OrcScene* s, double dt) {
+  if (!parts_valid(s)) return -1;
   const double H = s->H;
   const int N = s->N;
   const double* O = s->origin;
@@ -916,6 +923,7 @@ void orc_regularize(const OrcScene* s, double dt, const double* lo, const double
 // than 1e-9 remains and fewer than max_substeps were taken. Returns the
 // substep count; *advanced receives the time advanced.
 int orc_advance_frame(const OrcScene* s, const OrcFrame* f, double dt, double* advanced) {
+  if (!parts_valid(s)) return -1;
   const int n = s->count;
   double remaining = dt;
   int steps = 0;

@@ -32,6 +32,7 @@ mpm.refreshDiagnostics(s);
 const out = {dtStable, x: Array.from(s.x), y: Array.from(s.y), z: Array.from(s.z), vx: Array.from(s.vx),
   vy: Array.from(s.vy), vz: Array.from(s.vz), F: Array.from(mpm.F.subarray(0, n * 9)),
   plastic: Array.from(mpm.plastic.subarray(0, n)), stress: Array.from(mpm.stress.subarray(0, n)),
+  jp: Array.from(mpm.Jp.subarray(0, n)), dilation: Array.from(mpm.dilation.subarray(0, n)),
   minJ: mpm.minJ(s), conservation: mpm.conservation(), frames};
 mpm.destroy();
 console.log(JSON.stringify(out));

@@ -1,9 +1,9 @@
 """The Node.js drop-in: the N-API addon loads under node, exports the C ABI
 surface and rejects bad input before touching a device (CPU); and the JS
 MPM3 class steps the reference's own 4-block parity scene on the GPU
-(gpu/reference.mjs:1-9, models 0 and 10 instead of the unsupported 1-3) to
-the reference's CPU-GPU bar: positions atol .003 rtol .001 after 12 steps
-(gpu/test_solver.py:42)."""
+(gpu/reference.mjs:1-10: models 0-3, twelve Simulation3 frames), and a
+variant with models 0 and 10 over twelve plain steps, to the reference's
+CPU-GPU bar: positions atol .003 rtol .001 (gpu/test_solver.py:42)."""
 import json
 import os
 import shutil
@@ -109,3 +109,34 @@ def test_drop_in_advance_frame_matches_oracle(gpu, tmp_path):
         assert got["frames"][k]["substeps"] == n
     x = np.stack([got["x"], got["y"], got["z"]], 1)
     np.testing.assert_allclose(x, orc.positions(), atol=.003, rtol=.001)
+
+
+@pytest.mark.gpu
+def test_drop_in_reference_parity_scene(gpu, tmp_path):
+    """The reference's own parity scene (gpu/reference.mjs:1-10): four 3^3
+    blocks, one per model 0-3 and one part each, F preset, twelve
+    Simulation3.step frames of 1/60 with the default settings (uniform
+    gravity, solid anti-clump, bounds [-63,-47,-63] to [63,71,63]). Positions
+    and Jp (the fields the reference's script prints) against the oracle."""
+    pts, _ = four_blocks()
+    models = np.repeat([0, 1, 2, 3], 27)
+    F = [.98, .1, 0, 0, .95, 0, 0, 0, 1.02]
+    lo, hi = [-63, -47, -63], [63, 71, 63]
+    inp = dict(points=pts.tolist(), velocity=[0, 0, 0], models=models.tolist(), F=F, dt=1 / 60, frames=12,
+               part=models.tolist(), boundsLo=lo, boundsHi=hi)
+    path = tmp_path / "reference.json"
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout)
+    orc = o.Scene(pts, 0, 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8, model=models, part=models)
+    orc.F[:] = np.tile(F, len(pts))
+    for k in range(12):
+        n, adv = orc.advance_frame(1 / 60, bounds=(lo, hi))
+        assert got["frames"][k]["substeps"] == n, k
+    x = np.stack([got["x"], got["y"], got["z"]], 1)
+    np.testing.assert_allclose(x, orc.positions(), atol=.003, rtol=.001)
+    np.testing.assert_allclose(np.array(got["jp"]), orc.Jp, atol=2e-5, rtol=1e-4)
+    assert np.abs(orc.Jp[54:81] - 1).min() > 1e-3  # the snow block yielded under the preset F
+    assert got["conservation"]["gridMassQ"] == got["conservation"]["particleMassQ"]
