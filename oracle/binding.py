@@ -34,8 +34,18 @@ class OrcPlastic(C.Structure):
     _fields_ = [(k, D) for k in ("A", "B", "n", "C", "rate0", "Tref", "Tm", "m")]
 
 
+class OrcPressure(C.Structure):
+    _fields_ = [(k, D) for k in ("cohesion", "slope", "dilatancy", "tension", "alpha", "plateau", "gamma", "densify",
+                                 "beta", "epsD")]
+
+
+class OrcFracture(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("onset", D), ("failure", D)]
+
+
 class OrcPart(C.Structure):
-    _fields_ = [("E", D), ("G", D), ("L", D), ("th", OrcThermal), ("pl", OrcPlastic)]
+    _fields_ = [("E", D), ("G", D), ("L", D), ("th", OrcThermal), ("pl", OrcPlastic), ("pr", OrcPressure),
+                ("fr", OrcFracture)]
 
 
 class OrcScene(C.Structure):
@@ -60,6 +70,10 @@ class OrcScene(C.Structure):
 THERMAL_DEFAULTS = dict(Tref=20.0, Thot=180.0, hotRatio=.08, A=.04, H=.04, rate=.0005, Q=35000.0, n=3.0,
                         stress=.04, kelvin_offset=273.15)
 PLASTICITY_DEFAULTS = dict(A=.025, B=.08, n=.35, C=.025, rate0=1.0, Tref=20.0, Tm=1000.0, m=1.0)
+# src/three/pressure-plasticity.js:2, src/three/fracture.js:7
+PRESSURE_DEFAULTS = dict(cohesion=.025, slope=.6, dilatancy=.15, tension=.01, alpha=1.5, plateau=.015, gamma=.01,
+                         densify=.02, beta=2.0, epsD=.7)
+FRACTURE_DEFAULTS = dict(enabled=0, onset=.08, failure=.16)
 
 
 class OrcFrame(C.Structure):
@@ -98,6 +112,10 @@ def lib():
         L.orc_principal_stress.argtypes = [PD, PD, C.c_int, D, PD]
         L.orc_von_mises.restype = D
         L.orc_von_mises.argtypes = [PD, D]
+        L.orc_foam_strength.restype = D
+        L.orc_foam_strength.argtypes = [D, C.POINTER(OrcPressure)]
+        L.orc_pressure_return.argtypes = [PD, C.c_int, D, D, C.POINTER(OrcPressure), D, D, PD, PD, PD]
+        L.orc_update_damage.argtypes = [C.POINTER(OrcScene), C.c_int]
         L.orc_project_plastic.restype = C.c_int
         L.orc_project_plastic.argtypes = [PD, C.c_int, D, D, D, D, D, D, D, C.POINTER(OrcPart),
                                           PD, PD, PD, PD, PD, PD, PD]
@@ -207,6 +225,31 @@ def plastic_params(**kw):
     return OrcPlastic(**p)
 
 
+def pressure_params(**kw):
+    p = dict(PRESSURE_DEFAULTS)
+    p.update(kw)
+    return OrcPressure(**p)
+
+
+def fracture_params(**kw):
+    p = dict(FRACTURE_DEFAULTS)
+    p.update(kw)
+    p["enabled"] = int(p["enabled"])
+    return OrcFracture(**p)
+
+
+def foam_strength(ep, p=None):
+    return lib().orc_foam_strength(ep, C.byref(p or pressure_params()))
+
+
+def pressure_return(sigma, model, jp, ep, p=None, G=.4, L=.4):
+    """(stretches, jp, increment) of the pressure-dependent return."""
+    out, jo, inc = np.zeros(3), D(0), D(0)
+    lib().orc_pressure_return(_p(_f64(sigma)), model, jp, ep, C.byref(p or pressure_params()), G, L, _p(out),
+                              C.byref(jo), C.byref(inc))
+    return out, jo.value, inc.value
+
+
 def thermal_strength(ep, T, p=None):
     return lib().orc_thermal_strength(ep, T, C.byref(p or thermal_params()))
 
@@ -252,8 +295,9 @@ def von_mises(t, J=1.0):
     return lib().orc_von_mises(_p(t), J)
 
 
-def make_part(E=0.0, G=.4, L=.4, thermal=None, plastic=None):
-    return OrcPart(E=E, G=G, L=L, th=thermal or thermal_params(), pl=plastic or plastic_params())
+def make_part(E=0.0, G=.4, L=.4, thermal=None, plastic=None, pressure=None, fracture=None):
+    return OrcPart(E=E, G=G, L=L, th=thermal or thermal_params(), pl=plastic or plastic_params(),
+                   pr=pressure or pressure_params(), fr=fracture or fracture_params())
 
 
 def project_plastic(F, model, jp=1.0, dilation=0.0, yield_strain=.025, friction=35.0, ep=0.0, dt=1 / 240,
@@ -298,6 +342,7 @@ class Scene:
         self.C = np.zeros(9 * n)
         self.Jp = np.ones(n)
         self.dilation = np.zeros(n)
+        self.damage = np.zeros(n)
         self.stress = np.zeros(n)
         self.plastic = np.zeros(n)
         self.tau = np.zeros(9 * n)
@@ -321,9 +366,8 @@ class Scene:
         s = OrcScene()
         s.count = n
         for k in ("x", "y", "z", "vx", "vy", "vz", "mass", "volume", "temperature", "F", "C", "Jp", "dilation",
-                  "stress", "plastic", "tau", "cachedTau", "gmass", "gv"):
+                  "stress", "plastic", "tau", "cachedTau", "gmass", "gv", "damage"):
             setattr(s, k, _p(getattr(self, k)))
-        s.damage = None
         s.yield_strain = None
         s.friction = None
         s.model = _p(self.model, PI32)

@@ -182,10 +182,24 @@ typedef struct {
 // part; E == 0 takes the reference's per-particle E = m/(64 V)*16000*stiffness
 // (src/three/mpm.js:34). `G`, `L` are mu/E and lambda/E; the reference
 // hard-codes G = L = 0.4 (src/three/mpm.js:37, thermal-plasticity.js:24).
+// Pressure-dependent record (models 8, 9): Cauchy strengths / E, tension
+// positive. src/three/pressure-plasticity.js:2-3
+typedef struct {
+  double cohesion, slope, dilatancy, tension, alpha, plateau, gamma, densify, beta, epsD;
+} OrcPressure;
+
+// Tensile-strain damage record. src/three/fracture.js:6-9
+typedef struct {
+  int32_t enabled;
+  double onset, failure;
+} OrcFracture;
+
 typedef struct {
   double E, G, L;
   OrcThermal th;
   OrcPlastic pl;
+  OrcPressure pr;
+  OrcFracture fr;
 } OrcPart;
 
 // Caller-owned scene (all arrays FP64, per-particle 3x3 tensors row-major,
@@ -194,9 +208,10 @@ typedef struct {
   int count;
   double *x, *y, *z, *vx, *vy, *vz;
   const double *mass, *volume, *temperature;
-  const double* damage;                 // may be NULL
+  double* damage;                       // may be NULL (no fracture)
   const double *yield_strain, *friction;  // may be NULL (defaults .025, 35)
-  const int32_t *model, *part;
+  int32_t* model;                       // a failed particle becomes sand (3)
+  const int32_t* part;
   const uint8_t* pinned;
   double *F, *C, *Jp, *dilation, *stress, *plastic, *tau, *cachedTau;
   uint8_t* cacheValid;
@@ -417,6 +432,85 @@ void orc_metal_return(const double* sigma, int model, double ep, double dt, doub
   *out_increment = increment;
 }
 
+// Crushable-foam hardening curve; 1e20 at and beyond full densification.
+// src/three/pressure-plasticity.js:11-14
+double orc_foam_strength(double ep, const OrcPressure* p) {
+  const double r = ep / p->epsD;
+  if (r >= 1) return 1e20;
+  return p->plateau + p->gamma * r - p->densify * std::log(std::max(1e-30, 1 - std::pow(r, p->beta)));
+}
+
+// Return map of the pressure-dependent models on principal stretches:
+// 8 cohesive Drucker-Prager with a tension cut-off and non-associated
+// dilatancy, 9 Deshpande-Fleck crushable foam (associated, bisection on the
+// multiplier). Hencky strains, Cauchy measures with J = exp(tr e) * jp.
+// G, L are mu/E and lambda/E of the part (the reference's .4 and .4 give its
+// G = .4/J, K = (2/3)/J). Outputs the returned stretches, the new jp (total J
+// kept) and the equivalent plastic strain increment.
+// src/three/pressure-plasticity.js:15-45
+void orc_pressure_return(const double* sigma, int model, double jp, double ep, const OrcPressure* p, double G0,
+                         double L0, double* out_sigma, double* out_jp, double* out_increment) {
+  double e[3], dev[3];
+  for (int k = 0; k < 3; ++k) e[k] = std::log(std::max(sigma[k], 1e-6));
+  const double trace = e[0] + e[1] + e[2];
+  for (int k = 0; k < 3; ++k) dev[k] = e[k] - trace / 3;
+  const double length = std::sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const double J = std::max(1e-6, std::exp(trace) * jp);
+  const double G = G0 / J, K = (L0 + 2 * G0 / 3) / J;
+  const double mean = K * trace, q = std::sqrt(6.0) * G * length;
+  double meanNew = mean, qNew = q, increment = 0;
+  if (model == 8) {
+    const double t = q / std::sqrt(3.0), c = p->cohesion, a = p->slope, b = p->dilatancy, T = p->tension;
+    if (mean > T && t + a * T <= c) {
+      meanNew = T;
+    } else if (t + a * mean > c) {
+      const double dl = (t + a * mean - c) / (G + a * K * b);
+      meanNew = mean - K * b * dl;
+      qNew = std::sqrt(3.0) * std::max(0.0, t - G * dl);
+      if (meanNew > T) {
+        meanNew = T;
+        qNew = std::sqrt(3.0) * std::max(0.0, c - a * T);
+      }
+    }
+  } else {
+    const double a2 = p->alpha * p->alpha, B = 1 + a2 / 9;
+    const double trial = std::sqrt((q * q + a2 * mean * mean) / B);
+    if (trial > orc_foam_strength(ep, p)) {
+      auto state = [&](double eta, double* qn, double* mn, double* dl) {
+        *qn = q / (1 + 3 * G * eta);
+        *mn = mean / (1 + K * a2 * eta);
+        const double hat = std::sqrt((*qn * *qn + a2 * *mn * *mn) / B);
+        *dl = eta * B * hat;
+        return hat - orc_foam_strength(ep + *dl, p);
+      };
+      double qn, mn, dl, lo = 0, hi = 1 / G;
+      for (int k = 0; k < 32 && state(hi, &qn, &mn, &dl) > 0; ++k) hi *= 2;
+      for (int k = 0; k < 28; ++k) {
+        const double mid = (lo + hi) / 2;
+        if (state(mid, &qn, &mn, &dl) > 0)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      state((lo + hi) / 2, &qn, &mn, &dl);
+      qNew = qn;
+      meanNew = mn;
+      increment = dl;
+    }
+  }
+  const double traceNew = meanNew / K, scale = q > 1e-20 ? qNew / q : 1;
+  double ne[3];
+  for (int k = 0; k < 3; ++k) ne[k] = traceNew / 3 + scale * dev[k];
+  if (model == 8) {
+    double d2 = 0;
+    for (int k = 0; k < 3; ++k) d2 += (e[k] - ne[k]) * (e[k] - ne[k]);
+    increment = std::sqrt(2.0 / 3) * std::sqrt(d2);
+  }
+  for (int k = 0; k < 3; ++k) out_sigma[k] = std::exp(ne[k]);
+  *out_jp = jp * std::exp(trace - traceNew);
+  *out_increment = increment;
+}
+
 // Kirchhoff stress of a law with unit shear modulus: mu = 1, lambda = lr
 // (lr = lambda/mu; 1 in the reference). src/three/constitutive.js:92-106
 void orc_constitutive_stress(const double* F, int model, double mu, double lambda, double jp, double* tau) {
@@ -468,8 +562,7 @@ double orc_von_mises(const double* t, double J) {
          std::max(1e-6, J);
 }
 
-// Plastic projection for models 2-7, 10, 11 (8/9 are not restated). Returns
-// -1 for an unsupported model. Outputs F, jp, dilation, increment and the
+// Plastic projection for models 2-11. Returns -1 for an unknown model. Outputs F, jp, dilation, increment and the
 // returned decomposition (u, v, projected sigma). src/three/constitutive.js:52-78
 int orc_project_plastic(const double* F, int model, double jp, double dilation, double yieldStrain,
                         double friction, double ep, double dt, double T, const OrcPart* part,
@@ -482,7 +575,7 @@ int orc_project_plastic(const double* F, int model, double jp, double dilation, 
     *out_increment = 0;
     return 0;
   }
-  if (model == 8 || model == 9 || model > 11) return -1;
+  if (model > 11) return -1;
   Svd d = svd3(F);
   const double* s = d.sigma;
   double projected[3];
@@ -492,6 +585,8 @@ int orc_project_plastic(const double* F, int model, double jp, double dilation, 
     jp = clampd(jp * (s[0] * s[1] * s[2]) / (projected[0] * projected[1] * projected[2]), .6, 20);
   } else if (model >= 10) {
     orc_thermal_return(s, model, ep, dt, T, &part->th, part->G, projected, &metal);
+  } else if (model >= 8) {
+    orc_pressure_return(s, model, jp, ep, &part->pr, part->G, part->L, projected, &jp, &metal);
   } else if (model >= 4) {
     orc_metal_return(s, model, ep, dt, T, &part->pl, part->G, projected, &metal);
   } else {
@@ -535,6 +630,28 @@ static inline double particle_E(const OrcScene* s, int p) {
   const OrcPart& part = s->parts[s->part[p]];
   if (part.E > 0) return part.E;
   return s->mass[p] / (64 * s->volume[p]) * 16000 * s->stiffness;  // src/three/mpm.js:34
+}
+
+// Tensile-strain damage after the F update: the largest principal log
+// stretch between onset and failure maps to damage in [0, 1], which never
+// decreases; at 1 the particle becomes cohesionless sand with F = I, Jp = 1,
+// no dilation and no cached stress. src/three/fracture.js:11-18
+void orc_update_damage(const OrcScene* s, int p) {
+  if (!s->damage) return;
+  const OrcFracture& f = s->parts[s->part[p]].fr;
+  if (!f.enabled || s->damage[p] >= 1) return;
+  const Svd d = svd3(s->F + 9 * (size_t)p);
+  double strain = 0;
+  for (int k = 0; k < 3; ++k) strain = std::max(strain, std::log(std::max(d.sigma[k], 1e-8)));
+  s->damage[p] = std::max(s->damage[p], std::min(1.0, (strain - f.onset) / (f.failure - f.onset)));
+  if (s->damage[p] >= 1) {
+    s->model[p] = 3;
+    double* F = s->F + 9 * (size_t)p;
+    for (int k = 0; k < 9; ++k) F[k] = (k % 4 == 0) ? 1 : 0;
+    s->Jp[p] = 1;
+    s->dilation[p] = 0;
+    s->cacheValid[p] = 0;
+  }
 }
 
 // Stress for this step from F (or the cached return stress for models >= 2)
@@ -721,6 +838,7 @@ int orc_step(const OrcScene* s, double dt) {
         for (int k = 0; k < 3; ++k) value += dt * C[a * 3 + k] * F[k * 3 + b];
         s->F[p * 9 + a * 3 + b] = value;
       }
+    orc_update_damage(s, p);
     std::memcpy(s->C + p * 9, C, sizeof C);
     s->vx[p] = vx;
     s->vy[p] = vy;
