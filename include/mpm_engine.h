@@ -55,9 +55,10 @@ enum { MPM_WALL_SEPARATE = 0, MPM_WALL_STICKY = 1, MPM_WALL_SLIP = 2 };
 enum { MPM_F64 = 0, MPM_F32 = 1 };
 
 /* Constitutive models, numbered as the reference's MODELS table
- * (src/three/constitutive.js:6-19). Supported: 0 fixed-corotated elastic,
+ * (src/three/constitutive.js:6-19): 0 fixed-corotated elastic,
  * 1 Neo-Hookean, 2 snow, 3 sand (Drucker-Prager), 4 ideal J2,
  * 5 linear-hardening J2, 6 Johnson-Cook, 7 simplified Johnson-Cook,
+ * 8 cohesive Drucker-Prager soil, 9 Deshpande-Fleck crushable foam,
  * 10 thermal J2, 11 Norton-Arrhenius creep. */
 enum {
   MPM_MODEL_FIXED_COROTATED = 0,
@@ -68,6 +69,8 @@ enum {
   MPM_MODEL_J2_HARDENING = 5,
   MPM_MODEL_JOHNSON_COOK = 6,
   MPM_MODEL_JOHNSON_COOK_SIMPLE = 7,
+  MPM_MODEL_COHESIVE_DP = 8,
+  MPM_MODEL_FOAM = 9,
   MPM_MODEL_THERMAL_J2 = 10,
   MPM_MODEL_CREEP = 11
 };
@@ -110,6 +113,19 @@ typedef struct {
    * to the reference's ranges [.005, .08] and [15, 50]
    * (src/three/simulation.js:40). */
   float yield_strain, friction;
+  /* cohesive Drucker-Prager (8) and crushable foam (9): Cauchy strengths / E,
+   * tension positive (src/three/pressure-plasticity.js:2-3). Checked against
+   * the reference's ranges and constraints when the model is 8 or 9;
+   * mpm_material_defaults fills the reference defaults. */
+  float pr_cohesion, pr_slope, pr_dilatancy, pr_tension, pr_alpha, pr_plateau, pr_gamma, pr_densify, pr_beta,
+      pr_epsD;
+  /* tensile-strain damage (src/three/fracture.js:6-18): when fracture != 0,
+   * damage = (max principal log stretch - onset) / (failure - onset), never
+   * decreasing; the step stress carries (1 - damage), and at damage 1 the
+   * particle steps on as sand (model 3) with F = I. Requires
+   * .005 <= onset < failure <= 2. */
+  int32_t fracture;
+  float fr_onset, fr_failure;
 } mpm_material;
 
 /* Host view of particle state (structure of arrays). On upload every pointer
@@ -131,6 +147,9 @@ typedef struct {
                                      ratio MPM3.Jp (1 when NULL) */
   void *dilation;                 /* optional [count]: sand's MPM3.dilation
                                      (0 when NULL) */
+  void *damage;                   /* optional [count]: tensile damage in
+                                     [0, 1] (the scene's s.damage; 0 when
+                                     NULL). A particle at 1 is failed. */
 } mpm_particles;
 
 /* FP64 conservation report. Grid totals are those of the most recent P2G. */
@@ -165,8 +184,27 @@ int mpm_destroy(mpm_engine* e);
 const char* mpm_last_error(const mpm_engine* e);
 
 /* Material table (up to 64 entries); replaces the per-part records the
- * reference passes to projectPlastic (src/three/mpm.js:127). */
+ * reference passes to projectPlastic (src/three/mpm.js:127). Every entry
+ * must pass mpm_material_check. */
 int mpm_set_materials(mpm_engine* e, const mpm_material* materials, int32_t count);
+
+/* A material of the given model with mu = lambda = 0.4 E and every other
+ * field at the reference's defaults: thermal record
+ * (src/three/thermal-plasticity.js:3), Johnson-Cook record
+ * (src/three/metal-plasticity.js:3), pressure record
+ * (src/three/pressure-plasticity.js:2), yield strain .025, friction 35,
+ * fracture off with onset .08, failure .16 (src/three/fracture.js:7). Host
+ * only. */
+int mpm_material_defaults(int32_t model, float E, mpm_material* out);
+
+/* MPM_OK when the record is usable; MPM_ERR_UNSUPPORTED for a model number
+ * outside 0-11; else MPM_ERR_INVALID_ARGUMENT (non-finite or non-positive E,
+ * negative mu, non-finite lambda; for models 8 and 9
+ * the pressure record outside the ranges of pressureParameters or violating
+ * dilatancy <= slope and slope * tension <= cohesion,
+ * src/three/pressure-plasticity.js:3-10; with fracture on, onset and failure
+ * outside .005 <= onset < failure <= 2, src/three/fracture.js:8). Host only. */
+int mpm_material_check(const mpm_material* m);
 
 /* Upload particle state (replaces reading s.x.. and MPM3's own arrays at the
  * start of MPM3.step). Resets the fixed-point mass scale and re-bins. */
@@ -279,6 +317,13 @@ int mpm_return_map_batch(const mpm_material* m, uint32_t n, float dt, const floa
  * src/three/mpm.js:33-39): fixed-corotated for model 0, Hencky from F's own
  * decomposition for plastic models. tau_out [6n]. */
 int mpm_stress_batch(const mpm_material* m, uint32_t n, const float* F, float* tau_out);
+
+/* mpm_return_map_batch with the history variables in and out: jp [n] (snow's
+ * Jp; the pressure models' plastic volume ratio) and dilation [n] (sand).
+ * jp_out and dilation_out may alias jp and dilation. */
+int mpm_return_map_state_batch(const mpm_material* m, uint32_t n, float dt, const float* F, const float* ep,
+                               const float* T, const float* jp, const float* dilation, float* F_out, float* ep_out,
+                               float* jp_out, float* dilation_out, float* tau_out);
 
 /* ---- learned constitutive surrogate (no reference counterpart) ----
  *

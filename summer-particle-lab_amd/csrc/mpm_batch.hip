@@ -331,10 +331,8 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
     mpm_batch_destroy(b);
     return MPM_ERR_HIP;
   }
-  mpm_material m0{};
-  m0.model = MPM_MODEL_FIXED_COROTATED;
-  m0.E = 1.f;
-  m0.mu = m0.lambda = .4f;
+  mpm_material m0;
+  mpm_material_defaults(MPM_MODEL_FIXED_COROTATED, 1.f, &m0);
   mpm_batch_set_materials(b, &m0, 1);
   *out = b;
   return MPM_OK;
@@ -374,14 +372,18 @@ int mpm_batch_set_materials(mpm_batch* b, const mpm_material* mats, int32_t coun
   if (!b || !mats || count < 1 || count > 64) return bfail(b, MPM_ERR_INVALID_ARGUMENT, "1..64 materials required");
   std::vector<DevMaterial> dm(count);
   for (int i = 0; i < count; ++i) {
-    const int m = mats[i].model;
-    if (!((m >= 0 && m <= 7) || m == 10 || m == 11))
-      return bfail(b, MPM_ERR_UNSUPPORTED, "material %d: model %d is not supported", i, m);
-    if (!(mats[i].E > 0) || !(mats[i].mu >= 0) || !std::isfinite(mats[i].lambda))
-      return bfail(b, MPM_ERR_INVALID_ARGUMENT, "material %d: E must be > 0 and mu >= 0", i);
+    const int rc = mpm_material_check(&mats[i]);
+    if (rc == MPM_ERR_UNSUPPORTED)
+      return bfail(b, rc, "material %d: model %d is not supported", i, mats[i].model);
+    if (rc != MPM_OK)
+      return bfail(b, rc, "material %d: E, mu, lambda or the model's parameters are out of range", i);
     dm[i] = to_dev_material(mats[i]);
   }
   b->materials.assign(mats, mats + count);
+  bool ext = false;
+  for (int i = 0; i < count; ++i) ext |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
+  if (ext != b->d.ext_laws) drop_graphs(b);  // captured launches name the kernel variant
+  b->d.ext_laws = ext;
   BHIP(b, hipMemcpyAsync(b->d.materials, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   return MPM_OK;

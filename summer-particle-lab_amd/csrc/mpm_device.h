@@ -26,7 +26,18 @@ struct DevMaterial {
   float pl_A, pl_B, pl_n, pl_C, pl_rate0, pl_Tref, pl_Tm, pl_m;
   float sn_ys;     // snow stretch yield strain
   float sd_alpha;  // sand: sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
+  // cohesive Drucker-Prager (8) / crushable foam (9), strengths / E
+  float pr_c, pr_slope, pr_dil, pr_T, pr_alpha, pr_plateau, pr_gamma, pr_densify, pr_beta, pr_epsD;
+  int32_t fr_on;   // tensile damage on
+  float fr_onset;  // damage onset (log stretch)
+  float fr_inv;    // 1 / (failure - onset)
 };
+
+// The model a particle steps with: a failed particle (damage 1) is
+// cohesionless sand (src/three/fracture.js:17).
+__device__ __forceinline__ int effective_model(const DevMaterial& m, float dmg) {
+  return (m.fr_on && dmg >= 1.f) ? 3 : m.model;
+}
 
 struct M3 {
   float a[9];
@@ -281,25 +292,108 @@ __device__ __forceinline__ void neo_hookean_tau(const M3& F, float mu, float lam
   t[5] = mu * (F[3] * F[6] + F[4] * F[7] + F[5] * F[8]);
 }
 
-// Plastic projection of F (models 2-7, 10, 11): SVD, return on the principal
+// Crushable-foam hardening plateau + gamma r - densify ln(1 - r^beta),
+// r = ep / epsD, locked (1e20) at full densification
+// (src/three/pressure-plasticity.js:11-14).
+__device__ __forceinline__ float foam_strength(float ep, const DevMaterial& m) {
+  const float r = ep / m.pr_epsD;
+  if (r >= 1.f) return 1e20f;
+  return m.pr_plateau + m.pr_gamma * r - m.pr_densify * logf(fmaxf(1e-30f, 1.f - powf(r, m.pr_beta)));
+}
+
+// Return of the pressure-dependent models on the principal log stretches e
+// (in place): 8 cohesive Drucker-Prager with tension cut-off and
+// non-associated dilatancy, 9 Deshpande-Fleck foam (associated; bisection on
+// the multiplier, 32 doublings then 28 halvings). Cauchy measures with
+// J = exp(tr e) jp; G = (mu/E)/J, K = (lambda/E + 2/3 mu/E)/J, which are the
+// reference's .4/J and (2/3)/J at mu = lambda = .4 E. jp takes the plastic
+// volume change. Returns the equivalent plastic strain increment.
+// src/three/pressure-plasticity.js:15-45
+__device__ __forceinline__ float pressure_return(float e[3], int model, float& jp, float ep, const DevMaterial& m) {
+  const float tr = e[0] + e[1] + e[2];
+  float dev[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dev[k] = e[k] - tr * (1.f / 3.f);
+  const float len = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const float J = fmaxf(1e-6f, expf(tr) * jp);
+  const float G = m.G / J, K = m.G * (m.lr + 2.f / 3.f) / J;
+  const float mean = K * tr, q = 2.4494897427831781f * G * len;
+  float mn = mean, qn = q, inc = 0.f;
+  if (model == 8) {
+    const float t = q * 0.57735026918962576f, c = m.pr_c, a = m.pr_slope, b = m.pr_dil, T = m.pr_T;
+    if (mean > T && t + a * T <= c) {
+      mn = T;
+    } else if (t + a * mean > c) {
+      const float dl = (t + a * mean - c) / (G + a * K * b);
+      mn = mean - K * b * dl;
+      qn = 1.7320508075688772f * fmaxf(0.f, t - G * dl);
+      if (mn > T) {
+        mn = T;
+        qn = 1.7320508075688772f * fmaxf(0.f, c - a * T);
+      }
+    }
+  } else {
+    const float a2 = m.pr_alpha * m.pr_alpha, B = 1.f + a2 * (1.f / 9.f), iB = 1.f / B;
+    if (sqrtf((q * q + a2 * mean * mean) * iB) > foam_strength(ep, m)) {
+      // residual(eta) = hat(eta) - strength(ep + eta B hat(eta))
+      auto residual = [&](float eta, float& qe, float& me, float& dl) {
+        qe = q / (1.f + 3.f * G * eta);
+        me = mean / (1.f + K * a2 * eta);
+        const float hat = sqrtf((qe * qe + a2 * me * me) * iB);
+        dl = eta * B * hat;
+        return hat - foam_strength(ep + dl, m);
+      };
+      float qe, me, dl, lo = 0.f, hi = 1.f / G;
+      for (int k = 0; k < 32 && residual(hi, qe, me, dl) > 0.f; ++k) hi *= 2.f;
+      for (int k = 0; k < 28; ++k) {
+        const float mid = 0.5f * (lo + hi);
+        if (residual(mid, qe, me, dl) > 0.f)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      residual(0.5f * (lo + hi), qe, me, dl);
+      qn = qe;
+      mn = me;
+      inc = dl;
+    }
+  }
+  const float trn = mn / K, scale = q > 1e-20f ? qn / q : 1.f;
+  float d2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float ne = trn * (1.f / 3.f) + scale * dev[k];
+    d2 += (e[k] - ne) * (e[k] - ne);
+    e[k] = ne;
+  }
+  if (model == 8) inc = 0.81649658092772603f * sqrtf(d2);
+  jp *= expf(tr - trn);
+  return inc;
+}
+
+// Plastic projection of F (models 2-11): SVD, return on the principal
 // stretches, recompose F, and the stress of the returned state on the
 // returned principal directions as the next step's stress (6-vector):
 // snow 2 mu (s - 1) s + lambda (J - 1) J times its hardening, the others
-// Hencky 2 mu e + lambda tr(e) (src/three/constitutive.js:52-86). jp and dil
-// carry snow's Jp and sand's dilation. Returns the plastic strain increment.
-__device__ __forceinline__ float plastic_project(M3& F, float ep, float& jp, float& dil, float dt, float T,
-                                                 const DevMaterial& m, float tau[6]) {
+// Hencky 2 mu e + lambda tr(e) (src/three/constitutive.js:52-86). jp carries
+// snow's Jp and the pressure models' plastic volume ratio, dil sand's
+// dilation. `model` is the particle's effective model. PRESSURE = false
+// compiles the pressure models (8, 9) out, for kernels launched only when no
+// material uses them. Returns the plastic strain increment.
+template <bool PRESSURE = true>
+__device__ __forceinline__ float plastic_project(M3& F, int model, float ep, float& jp, float& dil, float dt,
+                                                 float T, const DevMaterial& m, float tau[6]) {
   float U[9], V[9], s[3];
   svd3(F, U, V, s);
   float sp[3];
   float inc;
-  if (m.model == 2) {
+  if (model == 2) {
     // Snow: clamp the stretches to [1 - ys, 1 + 0.3 ys]; Jp takes the
     // clamped-away volume (src/three/constitutive.js:56-58).
 #pragma unroll
     for (int k = 0; k < 3; ++k) sp[k] = fminf(fmaxf(s[k], 1.f - m.sn_ys), 1.f + m.sn_ys * .3f);
     jp = fminf(fmaxf(jp * (s[0] * s[1] * s[2]) / (sp[0] * sp[1] * sp[2]), .6f), 20.f);
-  } else if (m.model == 3) {
+  } else if (model == 3) {
     // Sand: non-cohesive logarithmic Drucker-Prager with a dilation history
     // (src/three/constitutive.js:65-73); (3 lambda + 2 mu) / (2 mu) is the
     // reference's 2.5 at lambda = mu.
@@ -325,17 +419,20 @@ __device__ __forceinline__ float plastic_project(M3& F, float ep, float& jp, flo
     float e[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) e[k] = logf(fmaxf(s[k], 1e-6f));
-    inc = return_map(e, m.model, ep, dt, T, m);
+    if (PRESSURE && (model == 8 || model == 9))
+      inc = pressure_return(e, model, jp, ep, m);
+    else
+      inc = return_map(e, model, ep, dt, T, m);
 #pragma unroll
     for (int k = 0; k < 3; ++k) sp[k] = expf(e[k]);
   }
-  if (m.model == 2 || m.model == 3) {
+  if (model == 2 || model == 3) {
     const float l0 = logf(s[0] / sp[0]), l1 = logf(s[1] / sp[1]), l2 = logf(s[2] / sp[2]);
     inc = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
   }
   compose(U, sp, V, F);
   float d[3];
-  if (m.model == 2) {
+  if (model == 2) {
     const float J = sp[0] * sp[1] * sp[2], h = snow_hardening(jp);
 #pragma unroll
     for (int k = 0; k < 3; ++k) d[k] = h * (2.f * m.mu * (sp[k] - 1.f) * sp[k] + m.lambda * (J - 1.f) * J);
@@ -350,11 +447,12 @@ __device__ __forceinline__ float plastic_project(M3& F, float ep, float& jp, flo
   return inc;
 }
 
-// Compatibility form for the models without snow or sand state.
+// Compatibility form for the models without history state (Jp = 1, no
+// dilation).
 __device__ __forceinline__ float plastic_project(M3& F, float ep, float dt, float T, const DevMaterial& m,
                                                  float tau[6]) {
   float jp = 1.f, dil = 0.f;
-  return plastic_project(F, ep, jp, dil, dt, T, m, tau);
+  return plastic_project(F, m.model, ep, jp, dil, dt, T, m, tau);
 }
 
 // Hencky stress 2 mu log(sigma) + lambda tr from F's own decomposition (the
@@ -375,15 +473,35 @@ __device__ __forceinline__ void hencky_tau(const M3& F, const DevMaterial& m, fl
 // Step stress from F alone (no cached return): constitutiveStress with the
 // material's moduli, snow times its hardening (src/three/mpm.js:33-37,
 // src/three/constitutive.js:87-99).
-__device__ __forceinline__ void elastic_tau(const M3& F, const DevMaterial& m, float jp, float tau[6]) {
-  if (m.model == 1) {
+__device__ __forceinline__ void elastic_tau(const M3& F, int model, const DevMaterial& m, float jp, float tau[6]) {
+  if (model == 1) {
     neo_hookean_tau(F, m.mu, m.lambda, tau);
-  } else if (m.model == 0 || m.model == 2) {
-    const float h = m.model == 2 ? snow_hardening(jp) : 1.f;
+  } else if (model == 0 || model == 2) {
+    const float h = model == 2 ? snow_hardening(jp) : 1.f;
     fixed_corotated_tau(F, m.mu * h, m.lambda * h, tau);
   } else {
     hencky_tau(F, m, tau);
   }
+}
+
+__device__ __forceinline__ void elastic_tau(const M3& F, const DevMaterial& m, float jp, float tau[6]) {
+  elastic_tau(F, m.model, m, jp, tau);
+}
+
+// Tensile-strain damage after the F update: the largest principal log
+// stretch between onset and failure raises the damage (never lowers it); at
+// 1 the particle fails and F resets to the identity. Returns true on failure
+// in this call. src/three/fracture.js:11-18
+__device__ __forceinline__ bool update_damage(M3& F, float& dmg, const DevMaterial& m) {
+  if (!m.fr_on || dmg >= 1.f) return false;
+  float U[9], V[9], s[3];
+  svd3(F, U, V, s);
+  const float strain = fmaxf(0.f, logf(fmaxf(fmaxf(fmaxf(s[0], s[1]), s[2]), 1e-8f)));
+  dmg = fmaxf(dmg, fminf(1.f, (strain - m.fr_onset) * m.fr_inv));
+  if (dmg < 1.f) return false;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F[k] = (k % 4 == 0) ? 1.f : 0.f;
+  return true;
 }
 
 }  // namespace mpm

@@ -166,13 +166,102 @@ DevMaterial to_dev_material(const mpm_material& m) {
   const double sn = std::sin(fr * M_PI / 180.0);
   d.sn_ys = ys;
   d.sd_alpha = (float)(std::sqrt(2.0 / 3.0) * 2 * sn / (3 - sn));
+  d.pr_c = m.pr_cohesion;
+  d.pr_slope = m.pr_slope;
+  d.pr_dil = m.pr_dilatancy;
+  d.pr_T = m.pr_tension;
+  d.pr_alpha = m.pr_alpha;
+  d.pr_plateau = m.pr_plateau;
+  d.pr_gamma = m.pr_gamma;
+  d.pr_densify = m.pr_densify;
+  d.pr_beta = m.pr_beta;
+  d.pr_epsD = m.pr_epsD;
+  d.fr_on = m.fracture != 0;
+  d.fr_onset = m.fr_onset;
+  d.fr_inv = m.fracture ? (float)(1.0 / ((double)m.fr_failure - m.fr_onset)) : 0.f;
   return d;
 }
 }  // namespace mpm
 
+extern "C" int mpm_material_defaults(int32_t model, float E, mpm_material* out) {
+  if (!out) return MPM_ERR_INVALID_ARGUMENT;
+  mpm_material m{};
+  m.model = model;
+  m.E = E;
+  m.mu = m.lambda = .4f * E;
+  // src/three/thermal-plasticity.js:3
+  m.th_Tref = 20.f;
+  m.th_Thot = 180.f;
+  m.th_hotRatio = .08f;
+  m.th_A = .04f;
+  m.th_H = .04f;
+  m.th_rate = .0005f;
+  m.th_Q = 35000.f;
+  m.th_n = 3.f;
+  m.th_stress = .04f;
+  m.th_kelvin_offset = 273.15f;
+  // src/three/metal-plasticity.js:3
+  m.pl_A = .025f;
+  m.pl_B = .08f;
+  m.pl_n = .35f;
+  m.pl_C = .025f;
+  m.pl_rate0 = 1.f;
+  m.pl_Tref = 20.f;
+  m.pl_Tm = 1000.f;
+  m.pl_m = 1.f;
+  m.yield_strain = .025f;
+  m.friction = 35.f;
+  // src/three/pressure-plasticity.js:2
+  m.pr_cohesion = .025f;
+  m.pr_slope = .6f;
+  m.pr_dilatancy = .15f;
+  m.pr_tension = .01f;
+  m.pr_alpha = 1.5f;
+  m.pr_plateau = .015f;
+  m.pr_gamma = .01f;
+  m.pr_densify = .02f;
+  m.pr_beta = 2.f;
+  m.pr_epsD = .7f;
+  // src/three/fracture.js:7
+  m.fracture = 0;
+  m.fr_onset = .08f;
+  m.fr_failure = .16f;
+  *out = m;
+  return MPM_OK;
+}
+
+extern "C" int mpm_material_check(const mpm_material* m) {
+  if (!m) return MPM_ERR_INVALID_ARGUMENT;
+  if (m->model < 0 || m->model > 11) return MPM_ERR_UNSUPPORTED;
+  if (!(m->E > 0) || !std::isfinite(m->E) || !(m->mu >= 0) || !std::isfinite(m->mu) || !std::isfinite(m->lambda))
+    return MPM_ERR_INVALID_ARGUMENT;
+  if (m->model == 8 || m->model == 9) {
+    // Ranges and constraints of pressureParameters (src/three/pressure-plasticity.js:3-10),
+    // with a relative 1e-6 allowance for the FP32 fields.
+    const struct {
+      float v;
+      double lo, hi;
+    } r[] = {{m->pr_cohesion, 0, .5},       {m->pr_slope, 0, 2},           {m->pr_dilatancy, 0, 2},
+             {m->pr_tension, 0, .5},        {m->pr_alpha, .05, std::sqrt(4.5)}, {m->pr_plateau, .0001, .5},
+             {m->pr_gamma, 0, 1},           {m->pr_densify, .0001, 1},     {m->pr_beta, 1, 5},
+             {m->pr_epsD, .05, 3}};
+    for (const auto& x : r)
+      if (!std::isfinite(x.v) || x.v < x.lo * (1 - 1e-6) || x.v > x.hi * (1 + 1e-6)) return MPM_ERR_INVALID_ARGUMENT;
+    if (!((double)m->pr_dilatancy <= (double)m->pr_slope * (1 + 1e-6)) ||
+        !((double)m->pr_slope * m->pr_tension <= (double)m->pr_cohesion + 1e-7))
+      return MPM_ERR_INVALID_ARGUMENT;
+  }
+  if (m->fracture) {  // src/three/fracture.js:8
+    if (!std::isfinite(m->fr_onset) || !std::isfinite(m->fr_failure) || m->fr_onset < .005f ||
+        !(m->fr_failure > m->fr_onset) || m->fr_failure > 2.f)
+      return MPM_ERR_INVALID_ARGUMENT;
+  }
+  return MPM_OK;
+}
+
 namespace {
 
-bool supported_model(int m) { return (m >= 0 && m <= 7) || m == 10 || m == 11; }
+bool supported_model(int m) { return m >= 0 && m <= 11; }
 
 // Bin, sort and permute the resident particles into cell order.
 int rebin(mpm_engine* e) {
@@ -339,13 +428,17 @@ int mpm_set_materials(mpm_engine* e, const mpm_material* mats, int32_t count) {
   if (!e || !mats || count < 1 || count > 64) return fail(e, MPM_ERR_INVALID_ARGUMENT, "1..64 materials required");
   std::vector<DevMaterial> dm(count);
   for (int i = 0; i < count; ++i) {
-    if (!supported_model(mats[i].model))
-      return fail(e, MPM_ERR_UNSUPPORTED, "material %d: model %d is not supported", i, mats[i].model);
-    if (!(mats[i].E > 0) || !(mats[i].mu >= 0) || !std::isfinite(mats[i].lambda))
-      return fail(e, MPM_ERR_INVALID_ARGUMENT, "material %d: E must be > 0 and mu >= 0", i);
+    const int rc = mpm_material_check(&mats[i]);
+    if (rc == MPM_ERR_UNSUPPORTED)
+      return fail(e, rc, "material %d: model %d is not supported", i, mats[i].model);
+    if (rc != MPM_OK)
+      return fail(e, rc, "material %d: E, mu, lambda or the model's parameters are out of range", i);
     dm[i] = to_dev_material(mats[i]);
   }
   e->materials.assign(mats, mats + count);
+  e->d.ext_laws = false;
+  for (int i = 0; i < count; ++i)
+    e->d.ext_laws |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
   if (e->d.mlp) {  // a new table switches the surrogate off
     HIP_OK(e, hipStreamSynchronize(e->stream));
     HIP_OK(e, hipFree(e->d.mlp));
@@ -600,11 +693,27 @@ int mpm_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t bits, uint32_t* ke
 int mpm_return_map_batch(const mpm_material* m, uint32_t n, float dt, const float* F, const float* ep,
                          const float* T, float* F_out, float* ep_out, float* tau_out) {
   if (!m || (n && (!F || !ep || !T || !F_out || !ep_out || !tau_out))) return MPM_ERR_INVALID_ARGUMENT;
-  if (!supported_model(m->model)) return MPM_ERR_UNSUPPORTED;
+  if (const int rc = mpm_material_check(m)) return rc;
   const DevMaterial dm = to_dev_material(*m);
   return batch_call(n, {{F, 9}, {ep, 1}, {T, 1}}, {{F_out, 9}, {ep_out, 1}, {tau_out, 6}},
                     [&](hipStream_t s, std::vector<float*>& in, std::vector<float*>& out) {
-                      launch_return_batch(s, dm, n, dt, in[0], in[1], in[2], out[0], out[1], out[2]);
+                      launch_return_batch(s, dm, n, dt, in[0], in[1], in[2], nullptr, nullptr, out[0], out[1],
+                                          nullptr, nullptr, out[2]);
+                    });
+}
+
+int mpm_return_map_state_batch(const mpm_material* m, uint32_t n, float dt, const float* F, const float* ep,
+                               const float* T, const float* jp, const float* dilation, float* F_out, float* ep_out,
+                               float* jp_out, float* dilation_out, float* tau_out) {
+  if (!m || (n && (!F || !ep || !T || !jp || !dilation || !F_out || !ep_out || !jp_out || !dilation_out || !tau_out)))
+    return MPM_ERR_INVALID_ARGUMENT;
+  if (const int rc = mpm_material_check(m)) return rc;
+  const DevMaterial dm = to_dev_material(*m);
+  return batch_call(n, {{F, 9}, {ep, 1}, {T, 1}, {jp, 1}, {dilation, 1}},
+                    {{F_out, 9}, {ep_out, 1}, {jp_out, 1}, {dilation_out, 1}, {tau_out, 6}},
+                    [&](hipStream_t s, std::vector<float*>& in, std::vector<float*>& out) {
+                      launch_return_batch(s, dm, n, dt, in[0], in[1], in[2], in[3], in[4], out[0], out[1], out[2],
+                                          out[3], out[4]);
                     });
 }
 

@@ -119,7 +119,7 @@ __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restri
 // The particle work of one chunk of FBK cell-sorted particles in slots
 // [first, min(first + FBK, end)). Grid pointers are those of the chunk's
 // scene; momentum planes are `pstride` floats apart.
-template <int MODE, bool SUR, int FBK>
+template <int MODE, bool SUR, int FBK, bool EXT>
 __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
                                             uint32_t first, uint32_t end, const DevMaterial* __restrict__ mats,
                                             const GridDims& g, const StepParams& sp, size_t pstride,
@@ -150,6 +150,15 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   float tau[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float ep = 0.f, jp = 1.f, dil = 0.f;
   uint32_t new_flags = flags;
+  // Damage and the model it implies (a failed particle steps as sand). EXT
+  // variants carry damage and the pressure models; the others are launched
+  // only when no material uses them.
+  float dmg = 0.f;
+  int model = mat.model;
+  if constexpr (EXT) {
+    dmg = mat.fr_on ? ld(P, cap, P_DMG, ic) : 0.f;
+    model = effective_model(mat, dmg);
+  }
 
   if (MODE == 0 || pinned) {
     // State as stored: stress of this step from F (or the cached return).
@@ -158,11 +167,11 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     v[2] = ld(P, cap, P_VZ, ic);
 #pragma unroll
     for (int k = 0; k < 9; ++k) Cm[k] = ld(P, cap, P_C0 + k, ic);
-    if (mat.model >= 2 && (flags & FLAG_CACHE)) {
+    if (model >= 2 && (flags & FLAG_CACHE)) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) tau[k] = ld(P, cap, P_T0 + k, ic);
     } else {
-      elastic_tau(F, mat, mat.model == 2 ? ld(P, cap, P_JP, ic) : 1.f, tau);
+      elastic_tau(F, model, mat, model == 2 ? ld(P, cap, P_JP, ic) : 1.f, tau);
     }
   }
 
@@ -253,10 +262,18 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     }
     __syncthreads();  // the LDS box is free: the surrogate and the scatter reuse it
 
+    bool failed = false;
+    if constexpr (EXT) {
+      if (gather && mat.fr_on) {
+        failed = update_damage(Fn, dmg, mat);
+        if (failed) model = 3;
+      }
+    }
+
     bool done = false;
     if constexpr (SUR) {
       // Learned return for the surrogate's material (J2 without hardening).
-      const bool mine = gather && (int)(mt & 255u) == mlp->norm.material && (mat.model == 4 || mat.model == 10);
+      const bool mine = gather && (int)(mt & 255u) == mlp->norm.material && (model == 4 || model == 10);
       float Tp = 0.f;
       if (mine) {
         Tp = ld(P, cap, P_TEMP, ic);
@@ -278,16 +295,16 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       }
     }
     if (gather && !done) {
-      if (mat.model >= 2) {
+      if (model >= 2) {
         ep = ld(P, cap, P_EP, ic);
-        if (mat.model <= 3) {
-          jp = mat.model == 2 ? ld(P, cap, P_JP, ic) : 1.f;
-          dil = mat.model == 3 ? ld(P, cap, P_DIL, ic) : 0.f;
+        if (!failed) {
+          if (model == 2 || model == 8 || model == 9) jp = ld(P, cap, P_JP, ic);
+          if (model == 3) dil = ld(P, cap, P_DIL, ic);
         }
-        ep += plastic_project(Fn, ep, jp, dil, dt, ld(P, cap, P_TEMP, ic), mat, tau);
+        ep += plastic_project<EXT>(Fn, model, ep, jp, dil, dt, ld(P, cap, P_TEMP, ic), mat, tau);
         new_flags |= FLAG_CACHE;
       } else if (MODE == 1) {
-        elastic_tau(Fn, mat, 1.f, tau);
+        elastic_tau(Fn, model, mat, 1.f, tau);
       }
       F = Fn;
     }
@@ -298,9 +315,10 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       st(P, cap, P_Z, i, x[2]);
 #pragma unroll
       for (int k = 0; k < 9; ++k) st(P, cap, P_F0 + k, i, F[k]);
-      if (mat.model >= 2) st(P, cap, P_EP, i, ep);
-      if (mat.model == 2) st(P, cap, P_JP, i, jp);
-      if (mat.model == 3) st(P, cap, P_DIL, i, dil);
+      if (model >= 2) st(P, cap, P_EP, i, ep);
+      if (model == 2 || model == 8 || model == 9 || failed) st(P, cap, P_JP, i, jp);
+      if (model == 3) st(P, cap, P_DIL, i, dil);
+      if (EXT && mat.fr_on) st(P, cap, P_DMG, i, dmg);
       if (new_flags != flags) meta[i] = (mt & 255u) | (new_flags << 8);
       if (MODE == 2) {
         st(P, cap, P_VX, i, v[0]);
@@ -308,7 +326,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         st(P, cap, P_VZ, i, v[2]);
 #pragma unroll
         for (int k = 0; k < 9; ++k) st(P, cap, P_C0 + k, i, Cm[k]);
-        if (mat.model >= 2) {
+        if (model >= 2) {
 #pragma unroll
           for (int k = 0; k < 6; ++k) st(P, cap, P_T0 + k, i, tau[k]);
         }
@@ -324,7 +342,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] += g.gravity[a] * dt;
   }
-  const float k4 = dt * vol * 4.f * invH * invH;
+  // The step stress carries (1 - damage) below failure (src/three/mpm.js:38).
+  const float k4 = dt * vol * 4.f * invH * invH * ((EXT && mat.fr_on && dmg < 1.f) ? 1.f - dmg : 1.f);
   const float T9[9] = {tau[0], tau[3], tau[4], tau[3], tau[1], tau[5], tau[4], tau[5], tau[2]};
   float A[9];
 #pragma unroll
@@ -492,7 +511,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 // per CU is the LDS limit (3 x 51.8 KB of 160 KB); the register budget is
 // held to the same occupancy so the surrogate variant's MFMA operands do not
 // cost a wave.
-template <int MODE, bool SUR>
+template <int MODE, bool SUR, bool EXT>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, uint32_t n,
     const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
@@ -500,15 +519,15 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB>::LDS_BYTES];
   __shared__ int s_box[6 * Shape<FB>::WAVES];
-  fused_chunk<MODE, SUR, FB>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel, acc_m,
-                             acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box);
+  fused_chunk<MODE, SUR, FB, EXT>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
+                                  acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box);
 }
 
 // Many scenes in one persistent launch: each workgroup takes chunks from a
 // shared counter until the list is exhausted (every workgroup reaches the
 // exit); the workgroup that takes the last ticket resets the counter for the
 // next launch, so no clearing operation sits between launches.
-template <int MODE, int FBK>
+template <int MODE, int FBK, bool EXT>
 __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_batch(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
@@ -529,7 +548,7 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const uint2 ch = b.chunks[t];
     const SceneDesc* sd = b.scenes + ch.x;
     const uint32_t node0 = sd->node0;
-    fused_chunk<MODE, false, FBK>(P, meta, cap, ch.y, sd->p0 + sd->n, mats, sd->g, sd->sp, b.node_stride,
+    fused_chunk<MODE, false, FBK, EXT>(P, meta, cap, ch.y, sd->p0 + sd->n, mats, sd->g, sd->sp, b.node_stride,
                                   vel + node0, acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> 6),
                                   nullptr, nullptr, s_raw, s_box);
   }
@@ -543,20 +562,27 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
                   LaunchEvents ev) {
   if (!n) return;
   const dim3 grid(fblocks(n)), block(FB);
-#define MPM_FUSED(M, S)                                                                                          \
-  hipExtLaunchKernelGGL((k_fused<M, S>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n,   \
+#define MPM_FUSED(M, S, X)                                                                                      \
+  hipExtLaunchKernelGGL((k_fused<M, S, X>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n, \
                         d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts)
+#define MPM_FUSED_X(M, S) \
+  if (d.ext_laws)         \
+    MPM_FUSED(M, S, true); \
+  else                    \
+    MPM_FUSED(M, S, false)
   const bool sur = d.mlp != nullptr;
-  if (mode == 0)
-    MPM_FUSED(0, false);
-  else if (mode == 1 && sur)
-    MPM_FUSED(1, true);
-  else if (mode == 1)
-    MPM_FUSED(1, false);
-  else if (sur)
-    MPM_FUSED(2, true);
-  else
-    MPM_FUSED(2, false);
+  if (mode == 0) {
+    MPM_FUSED_X(0, false);
+  } else if (mode == 1 && sur) {
+    MPM_FUSED_X(1, true);
+  } else if (mode == 1) {
+    MPM_FUSED_X(1, false);
+  } else if (sur) {
+    MPM_FUSED_X(2, true);
+  } else {
+    MPM_FUSED_X(2, false);
+  }
+#undef MPM_FUSED_X
 #undef MPM_FUSED
 }
 
@@ -564,16 +590,22 @@ void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, ui
                         int fb, LaunchEvents ev) {
   if (!b.nchunks) return;
   const dim3 grid(std::min(particle_blocks, b.nchunks));
-#define MPM_FUSED_B(M, K)                                                                                           \
-  hipExtLaunchKernelGGL((k_fused_batch<M, K>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, \
-                        d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces)
-#define MPM_FUSED_M(M)        \
+#define MPM_FUSED_B(M, K, X)                                                                                       \
+  hipExtLaunchKernelGGL((k_fused_batch<M, K, X>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta,     \
+                        d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces)
+#define MPM_FUSED_K(M, X)     \
   if (fb == 64)               \
-    MPM_FUSED_B(M, 64);       \
+    MPM_FUSED_B(M, 64, X);    \
   else if (fb == 128)         \
-    MPM_FUSED_B(M, 128);      \
+    MPM_FUSED_B(M, 128, X);   \
   else                        \
-    MPM_FUSED_B(M, 256)
+    MPM_FUSED_B(M, 256, X)
+#define MPM_FUSED_M(M)        \
+  if (d.ext_laws) {           \
+    MPM_FUSED_K(M, true);     \
+  } else {                    \
+    MPM_FUSED_K(M, false);    \
+  }
   if (mode == 0) {
     MPM_FUSED_M(0);
   } else if (mode == 1) {
@@ -582,6 +614,7 @@ void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, ui
     MPM_FUSED_M(2);
   }
 #undef MPM_FUSED_M
+#undef MPM_FUSED_K
 #undef MPM_FUSED_B
 }
 

@@ -83,6 +83,7 @@ inline std::string pack_particles(const mpm_particles* p, size_t nmaterials, flo
     set(P_EP, rdv(p->plastic, dt, i));
     set(P_JP, p->jp ? rdv(p->jp, dt, i) : 1.0);
     set(P_DIL, p->dilation ? rdv(p->dilation, dt, i) : 0.0);
+    set(P_DMG, p->damage ? std::min(std::max(rdv(p->damage, dt, i), 0.0), 1.0) : 0.0);
     ids[i] = id0 + i;
     meta[i] = (uint32_t)p->material[i] | (flags << 8);
   }
@@ -124,6 +125,7 @@ inline void unpack_particles(const float* planes, size_t stride, uint32_t slot0,
     if (p->plastic) wrv(p->plastic, dt, i, get(P_EP));
     if (p->jp) wrv(p->jp, dt, i, get(P_JP));
     if (p->dilation) wrv(p->dilation, dt, i, get(P_DIL));
+    if (p->damage) wrv(p->damage, dt, i, get(P_DMG));
     if (p->cached_tau) {
       const double t[6] = {get(P_T0), get(P_T0 + 1), get(P_T0 + 2), get(P_T0 + 3), get(P_T0 + 4), get(P_T0 + 5)};
       const double full[9] = {t[0], t[3], t[4], t[3], t[1], t[5], t[4], t[5], t[2]};

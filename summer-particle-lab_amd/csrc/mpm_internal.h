@@ -25,8 +25,9 @@ enum Plane : int {
   P_VOL,
   P_TEMP,
   P_EP,
-  P_JP,     // snow: plastic volume ratio Jp (src/three/mpm.js:9)
+  P_JP,     // snow, pressure models: plastic volume ratio Jp (src/three/mpm.js:9)
   P_DIL,    // sand: dilation history
+  P_DMG,    // tensile damage in [0, 1] (the scene's s.damage)
   NPLANES
 };
 
@@ -53,6 +54,7 @@ struct StepParams {
 // Device pointers of one engine.
 struct DevState {
   float* planes;      // NPLANES * cap
+  bool ext_laws;      // some material uses damage or a pressure model (8, 9)
   float* planes_alt;  // permutation target
   uint32_t* ids;      // original id per slot
   uint32_t* ids_alt;
@@ -160,7 +162,8 @@ void launch_particle_reduce(hipStream_t s, const DevState& d, uint32_t n, float 
 void launch_stable(hipStream_t s, const DevState& d, uint32_t n, float* out);
 void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out);
 void launch_return_batch(hipStream_t s, const DevMaterial& m, uint32_t n, float dt, const float* F, const float* ep,
-                         const float* T, float* F_out, float* ep_out, float* tau_out);
+                         const float* T, const float* jp, const float* dil, float* F_out, float* ep_out,
+                         float* jp_out, float* dil_out, float* tau_out);
 void launch_stress_batch(hipStream_t s, const DevMaterial& m, uint32_t n, const float* F, float* tau_out);
 DevMaterial to_dev_material(const mpm_material& m);
 

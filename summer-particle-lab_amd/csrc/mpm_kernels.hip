@@ -91,14 +91,15 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const float* __restrict__ P, 
   meta_out[i] = meta[src];
 }
 
-// Kirchhoff stress used by this step's P2G (6-vector).
-__device__ __forceinline__ void step_stress(const M3& F, const DevMaterial& m, uint32_t flags,
+// Kirchhoff stress of this step before the damage factor (6-vector), for
+// the particle's effective model.
+__device__ __forceinline__ void step_stress(const M3& F, int model, const DevMaterial& m, uint32_t flags,
                                             const float* __restrict__ P, uint32_t cap, uint32_t i, float tau[6]) {
-  if (m.model >= 2 && (flags & FLAG_CACHE)) {
+  if (model >= 2 && (flags & FLAG_CACHE)) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) tau[k] = plane(P, cap, P_T0 + k, i);
   } else {
-    elastic_tau(F, m, m.model == 2 ? plane(P, cap, P_JP, i) : 1.f, tau);
+    elastic_tau(F, model, m, model == 2 ? plane(P, cap, P_JP, i) : 1.f, tau);
   }
 }
 
@@ -555,15 +556,17 @@ __global__ __launch_bounds__(BLOCK) void k_stable(const float* __restrict__ P, c
       col = fmaxf(col, c);
     }
     const float rho = plane(P, cap, P_MASS, i) / plane(P, cap, P_VOL, i);
-    const float hard = mat.model == 2 ? snow_hardening(plane(P, cap, P_JP, i)) : 1.f;
+    const int model = effective_model(mat, mat.fr_on ? plane(P, cap, P_DMG, i) : 0.f);
+    const float hard = model == 2 ? snow_hardening(plane(P, cap, P_JP, i)) : 1.f;
     const float c2 = (mat.lambda + 2.f * mat.mu) * hard / rho;
-    // Neo-Hookean stiffens in compression: 1 / max(.05, J).
+    // Neo-Hookean and the pressure models stiffen in compression:
+    // 1 / max(.05, J), with J det F Jp for the pressure models.
     float comp = 1.f;
-    if (mat.model == 1) {
+    if (model == 1 || model == 8 || model == 9) {
       M3 F;
 #pragma unroll
       for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
-      comp = 1.f / fmaxf(.05f, det3(F));
+      comp = 1.f / fmaxf(.05f, det3(F) * (model == 1 ? 1.f : plane(P, cap, P_JP, i)));
     }
     const float vx = plane(P, cap, P_VX, i), vy = plane(P, cap, P_VY, i), vz = plane(P, cap, P_VZ, i);
     const float speed = sqrtf(vx * vx + vy * vy + vz * vz);
@@ -591,20 +594,25 @@ __global__ __launch_bounds__(BLOCK) void k_von_mises(const float* __restrict__ P
 #pragma unroll
   for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
   float t[6];
-  step_stress(F, mat, mt >> 8, P, cap, i, t);
+  const int model = effective_model(mat, mat.fr_on ? plane(P, cap, P_DMG, i) : 0.f);
+  step_stress(F, model, mat, mt >> 8, P, cap, i, t);
   const float vm = sqrtf(fmaxf(0.f, 0.5f * ((t[0] - t[1]) * (t[0] - t[1]) + (t[1] - t[2]) * (t[1] - t[2]) +
                                             (t[2] - t[0]) * (t[2] - t[0])) +
                                          3.f * (t[3] * t[3] + t[4] * t[4] + t[5] * t[5])));
-  out[i] = vm / fmaxf(1e-6f, det3(F)) / mat.E;
+  const float J = det3(F) * ((model == 8 || model == 9) ? plane(P, cap, P_JP, i) : 1.f);
+  out[i] = vm / fmaxf(1e-6f, J) / mat.E;
 }
 
 // Plain constitutive update of a batch of deformation gradients: the return
-// map the step runs after G2P (models >= 2) or, for model 0, the
-// fixed-corotated stress; tau out as the symmetric 6-vector.
+// map the step runs after G2P (models >= 2) or, for models 0 and 1, the
+// elastic stress; tau out as the symmetric 6-vector. jp / dil (Jp and
+// dilation in and out) may be null: Jp = 1, no dilation.
 __global__ __launch_bounds__(BLOCK) void k_return_batch(DevMaterial mat, uint32_t n, float dt,
                                                         const float* __restrict__ F, const float* __restrict__ ep,
-                                                        const float* __restrict__ T, float* __restrict__ F_out,
-                                                        float* __restrict__ ep_out, float* __restrict__ tau_out) {
+                                                        const float* __restrict__ T, const float* jp_in,
+                                                        const float* dil_in, float* __restrict__ F_out,
+                                                        float* __restrict__ ep_out, float* jp_out, float* dil_out,
+                                                        float* __restrict__ tau_out) {
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   M3 f;
@@ -612,11 +620,14 @@ __global__ __launch_bounds__(BLOCK) void k_return_batch(DevMaterial mat, uint32_
   for (int k = 0; k < 9; ++k) f[k] = F[(size_t)i * 9 + k];
   float tau[6];
   float e = ep[i];
+  float jp = jp_in ? jp_in[i] : 1.f, dil = dil_in ? dil_in[i] : 0.f;
   if (mat.model >= 2) {
-    e += plastic_project(f, e, dt, T[i], mat, tau);  // snow Jp = 1, sand dilation = 0
+    e += plastic_project(f, mat.model, e, jp, dil, dt, T[i], mat, tau);
   } else {
     elastic_tau(f, mat, 1.f, tau);
   }
+  if (jp_out) jp_out[i] = jp;
+  if (dil_out) dil_out[i] = dil;
 #pragma unroll
   for (int k = 0; k < 9; ++k) F_out[(size_t)i * 9 + k] = f[k];
 #pragma unroll
@@ -723,9 +734,11 @@ void launch_stable(hipStream_t s, const DevState& d, uint32_t n, float* out) {
 }
 
 void launch_return_batch(hipStream_t s, const DevMaterial& m, uint32_t n, float dt, const float* F, const float* ep,
-                         const float* T, float* F_out, float* ep_out, float* tau_out) {
+                         const float* T, const float* jp, const float* dil, float* F_out, float* ep_out,
+                         float* jp_out, float* dil_out, float* tau_out) {
   if (!n) return;
-  hipLaunchKernelGGL(k_return_batch, dim3(blocks(n)), dim3(BLOCK), 0, s, m, n, dt, F, ep, T, F_out, ep_out, tau_out);
+  hipLaunchKernelGGL(k_return_batch, dim3(blocks(n)), dim3(BLOCK), 0, s, m, n, dt, F, ep, T, jp, dil, F_out, ep_out,
+                     jp_out, dil_out, tau_out);
 }
 
 void launch_stress_batch(hipStream_t s, const DevMaterial& m, uint32_t n, const float* F, float* tau_out) {

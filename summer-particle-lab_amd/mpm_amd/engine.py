@@ -28,6 +28,8 @@ class Material:
     plastic: dict = field(default_factory=dict)
     yield_strain: float = 0.0  # snow; 0: the reference default .025
     friction: float = 0.0      # sand, degrees; 0: the reference default 35
+    pressure: dict = field(default_factory=dict)  # models 8, 9 (src/three/pressure-plasticity.js:2)
+    fracture: dict = field(default_factory=dict)  # enabled, onset, failure (src/three/fracture.js:7)
 
     @staticmethod
     def from_E_nu(model, E, nu, **kw):
@@ -49,6 +51,14 @@ class Material:
             setattr(m, "th_" + k, v)
         for k, v in pl.items():
             setattr(m, "pl_" + k, v)
+        pr = dict(cohesion=.025, slope=.6, dilatancy=.15, tension=.01, alpha=1.5, plateau=.015, gamma=.01,
+                  densify=.02, beta=2.0, epsD=.7)
+        pr.update(self.pressure)
+        for k, v in pr.items():
+            setattr(m, "pr_" + k, v)
+        fr = dict(enabled=False, onset=.08, failure=.16)
+        fr.update(self.fracture)
+        m.fracture, m.fr_onset, m.fr_failure = int(bool(fr["enabled"])), fr["onset"], fr["failure"]
         return m
 
 
@@ -73,7 +83,8 @@ def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, grav
 
 
 def particle_view(x, v, mass, volume, *, F=None, Cmat=None, temperature=None, plastic=None, material=None,
-                  pinned=None, cached_tau=None, cache_valid=None, jp=None, dilation=None, dtype=np.float64):
+                  pinned=None, cached_tau=None, cache_valid=None, jp=None, dilation=None, damage=None,
+                  dtype=np.float64):
     """An mpm_particles view of host arrays; returns (view, arrays to keep alive)."""
     x = np.asarray(x, np.float64)
     n = len(x)
@@ -100,6 +111,8 @@ def particle_view(x, v, mass, volume, *, F=None, Cmat=None, temperature=None, pl
         keep["jp"] = arr(jp, 1.0)
     if dilation is not None:
         keep["dilation"] = arr(dilation, 0.0)
+    if damage is not None:
+        keep["damage"] = arr(damage, 0.0)
     p = L.MpmParticles()
     p.count = n
     p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
@@ -115,7 +128,8 @@ def download_view(n, dtype=np.float64):
                vz=np.zeros(n, f), F=np.zeros(9 * n, f), C=np.zeros(9 * n, f), mass=np.zeros(n, f),
                volume=np.zeros(n, f), temperature=np.zeros(n, f), plastic=np.zeros(n, f),
                cached_tau=np.zeros(9 * n, f), material=np.zeros(n, np.uint8), pinned=np.zeros(n, np.uint8),
-               cache_valid=np.zeros(n, np.uint8), jp=np.zeros(n, f), dilation=np.zeros(n, f))
+               cache_valid=np.zeros(n, np.uint8), jp=np.zeros(n, f), dilation=np.zeros(n, f),
+               damage=np.zeros(n, f))
     p = L.MpmParticles()
     p.count = n
     p.dtype = L.MPM_F64 if f == np.float64 else L.MPM_F32
@@ -294,6 +308,24 @@ def return_map_batch(material, F, ep, T, dt):
     L.check(lib.mpm_return_map_batch(C.byref(m), n, float(dt), _f32p(F), _f32p(ep), _f32p(T), _f32p(Fo), _f32p(epo),
                                      _f32p(to)))
     return Fo, epo, to
+
+
+def return_map_state_batch(material, F, ep, T, jp, dilation, dt):
+    """Device plain return with the history variables (mpm_return_map_state_batch):
+    (F_out [n,9], ep_out [n], jp_out [n], dilation_out [n], tau_out [n,6])."""
+    lib = L.load()
+    F = np.ascontiguousarray(np.asarray(F, np.float32).reshape(-1, 9))
+    n = len(F)
+    col = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), (n,)))  # noqa: E731
+    ep, T, jp, dilation = col(ep), col(T), col(jp), col(dilation)
+    Fo, epo, jpo, dilo = np.zeros((n, 9), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32), \
+        np.zeros(n, np.float32)
+    to = np.zeros((n, 6), np.float32)
+    m = material.to_c()
+    L.check(lib.mpm_return_map_state_batch(C.byref(m), n, float(dt), _f32p(F), _f32p(ep), _f32p(T), _f32p(jp),
+                                           _f32p(dilation), _f32p(Fo), _f32p(epo), _f32p(jpo), _f32p(dilo),
+                                           _f32p(to)))
+    return Fo, epo, jpo, dilo, to
 
 
 def stress_batch(material, F):

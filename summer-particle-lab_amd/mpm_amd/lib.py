@@ -38,11 +38,14 @@ _MAT_FLOATS = ["E", "mu", "lambda_",
                "th_Tref", "th_Thot", "th_hotRatio", "th_A", "th_H", "th_rate", "th_Q", "th_n", "th_stress",
                "th_kelvin_offset",
                "pl_A", "pl_B", "pl_n", "pl_C", "pl_rate0", "pl_Tref", "pl_Tm", "pl_m",
-               "yield_strain", "friction"]
+               "yield_strain", "friction",
+               "pr_cohesion", "pr_slope", "pr_dilatancy", "pr_tension", "pr_alpha", "pr_plateau", "pr_gamma",
+               "pr_densify", "pr_beta", "pr_epsD"]
 
 
 class MpmMaterial(C.Structure):
-    _fields_ = [("model", C.c_int32)] + [(k, C.c_float) for k in _MAT_FLOATS]
+    _fields_ = ([("model", C.c_int32)] + [(k, C.c_float) for k in _MAT_FLOATS] +
+                [("fracture", C.c_int32), ("fr_onset", C.c_float), ("fr_failure", C.c_float)])
 
 
 class MpmParticles(C.Structure):
@@ -54,7 +57,7 @@ class MpmParticles(C.Structure):
         ("mass", C.c_void_p), ("volume", C.c_void_p), ("temperature", C.c_void_p),
         ("plastic", C.c_void_p), ("cached_tau", C.c_void_p),
         ("material", C.c_void_p), ("pinned", C.c_void_p), ("cache_valid", C.c_void_p),
-        ("jp", C.c_void_p), ("dilation", C.c_void_p),
+        ("jp", C.c_void_p), ("dilation", C.c_void_p), ("damage", C.c_void_p),
     ]
 
 
@@ -137,6 +140,10 @@ SIGNATURES = {
     "mpm_return_map_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.c_float] + [C.POINTER(C.c_float)] * 6),
     "mpm_stress_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.POINTER(C.c_float),
                                    C.POINTER(C.c_float)]),
+    "mpm_return_map_state_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.c_float] +
+                                   [C.POINTER(C.c_float)] * 10),
+    "mpm_material_defaults": (C.c_int, [C.c_int32, C.c_float, C.POINTER(MpmMaterial)]),
+    "mpm_material_check": (C.c_int, [C.POINTER(MpmMaterial)]),
     "mpm_mlp_default_norm": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpTrainConfig),
                                        C.POINTER(MpmMlpNorm)]),
     "mpm_mlp_check_material": (C.c_int, [C.POINTER(MpmMaterial), C.POINTER(MpmMlpNorm)]),

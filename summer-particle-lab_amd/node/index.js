@@ -23,7 +23,9 @@ const native = require('./mpm_napi.node');
 const DEFAULT_DOMAIN = Object.freeze({lower: [-64, -48, -64], upper: [64, 72, 64], cellSize: 8, particlesPerAxis: 2});
 const THERMAL_KEYS = ['Tref', 'Thot', 'hotRatio', 'A', 'H', 'rate', 'Q', 'n', 'stress'];
 const PLASTIC_KEYS = ['A', 'B', 'n', 'C', 'rate0', 'Tref', 'Tm', 'm'];
-const SUPPORTED = new Set([0, 1, 2, 3, 4, 5, 6, 7, 10, 11]);
+const PRESSURE_KEYS = ['cohesion', 'slope', 'dilatancy', 'tension', 'alpha', 'plateau', 'gamma', 'densify', 'beta',
+  'epsD'];
+const SUPPORTED = new Set([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11]);
 
 function identity() { return [1, 0, 0, 0, 1, 0, 0, 0, 1]; }
 function det(a) {
@@ -88,10 +90,18 @@ class MPM3 {
         keys.set(key, m);
         const th = (s.thermalMaterials && s.thermalMaterials[part]) || {};
         const pl = (s.plasticity && s.plasticity[part]) || {};
-        const thermal = {}, plastic = {};
+        const pr = (s.pressureMaterials && s.pressureMaterials[part]) || {};
+        const fm = (s.fractureMaterials && s.fractureMaterials[part]) || {};
+        const thermal = {}, plastic = {}, pressure = {};
         for (const k of THERMAL_KEYS) if (Number.isFinite(th[k])) thermal[k] = th[k];
         for (const k of PLASTIC_KEYS) if (Number.isFinite(pl[k])) plastic[k] = pl[k];
-        materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic, yieldStrain: ys, friction: fr});
+        for (const k of PRESSURE_KEYS) if (Number.isFinite(pr[k])) pressure[k] = pr[k];
+        // Damage is the scene's (s.damage, src/three/fracture.js:11-18).
+        const fracture = {enabled: fm.enabled === true && !!s.damage};
+        if (Number.isFinite(fm.onset)) fracture.onset = fm.onset;
+        if (Number.isFinite(fm.failure)) fracture.failure = fm.failure;
+        materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic, pressure, fracture,
+          yieldStrain: ys, friction: fr});
       }
       material[ids.length] = m;
       ids.push(p);
@@ -101,7 +111,7 @@ class MPM3 {
 
   _view(n) {
     const v = {count: n};
-    for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature', 'plastic', 'jp', 'dilation']) v[k] = new Float64Array(n);
+    for (const k of ['x', 'y', 'z', 'vx', 'vy', 'vz', 'mass', 'volume', 'temperature', 'plastic', 'jp', 'dilation', 'damage']) v[k] = new Float64Array(n);
     v.F = new Float64Array(n * 9); v.C = new Float64Array(n * 9); v.cachedTau = new Float64Array(n * 9);
     v.pinned = new Uint8Array(n); v.cacheValid = new Uint8Array(n); v.material = new Uint8Array(n);
     return v;
@@ -118,7 +128,7 @@ class MPM3 {
       v.x[k] = s.x[p]; v.y[k] = s.y[p]; v.z[k] = s.z[p]; v.vx[k] = s.vx[p]; v.vy[k] = s.vy[p]; v.vz[k] = s.vz[p];
       v.mass[k] = s.mass[p]; v.volume[k] = 64 * s.volume[p];
       v.temperature[k] = s.temperature ? s.temperature[p] : 20; v.plastic[k] = this.plastic[p];
-      v.jp[k] = this.Jp[p]; v.dilation[k] = this.dilation[p];
+      v.jp[k] = this.Jp[p]; v.dilation[k] = this.dilation[p]; v.damage[k] = s.damage ? s.damage[p] : 0;
       v.pinned[k] = s.pinned ? s.pinned[p] : 0; v.material[k] = material[k];
       v.cacheValid[k] = this.cacheValid[p];
       // Cached return stress: unit-modulus in the reference, absolute here
@@ -146,9 +156,13 @@ class MPM3 {
       s.x[p] = v.x[k]; s.y[p] = v.y[k]; s.z[p] = v.z[k]; s.vx[p] = v.vx[k]; s.vy[p] = v.vy[k]; s.vz[p] = v.vz[k];
       this.plastic[p] = v.plastic[k]; this.cacheValid[p] = v.cacheValid[k];
       this.Jp[p] = v.jp[k]; this.dilation[p] = v.dilation[k];
+      // A failed particle steps on as sand (src/three/fracture.js:17).
+      const failed = v.damage[k] >= 1;
+      if (s.damage) s.damage[p] = v.damage[k];
+      if (failed && s.model) s.model[p] = 3;
       // The device's cached snow stress carries the hardening of the new Jp.
-      const scale = this._snow && this._snow[k] ? this._scale[k] / this._snow[k] *
-        Math.min(4, Math.max(.25, Math.exp(10 * (1 - v.jp[k])))) : this._scale[k];
+      const scale = this._snow && this._snow[k] && !failed ? this._scale[k] / this._snow[k] *
+        Math.min(4, Math.max(.25, Math.exp(10 * (1 - v.jp[k])))) : this._scale[k] / (this._snow[k] || 1);
       for (let j = 0; j < 9; j++) {
         this.F[p * 9 + j] = v.F[k * 9 + j]; this.C[p * 9 + j] = v.C[k * 9 + j];
         if (v.cacheValid[k]) this.cachedTau[p * 9 + j] = v.cachedTau[k * 9 + j] / scale;

@@ -127,7 +127,8 @@ static napi_value js_destroy(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* setMaterials(h, [{model, E, mu, lambda, thermal:{...}, plastic:{...}}]) */
+/* setMaterials(h, [{model, E, mu, lambda, yieldStrain, friction, thermal:{...},
+ * plastic:{...}, pressure:{...}, fracture:{enabled, onset, failure}}]) */
 static napi_value js_set_materials(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -143,21 +144,38 @@ static napi_value js_set_materials(napi_env env, napi_callback_info info) {
   mpm_material mats[64];
   memset(mats, 0, sizeof mats);
   for (uint32_t i = 0; i < n; ++i) {
-    napi_value m, th, pl;
+    napi_value m, th, pl, pr, fr;
     bool has;
     CHECK_NAPI(env, napi_get_element(env, argv[1], i, &m));
     mpm_material* o = &mats[i];
-    o->model = (int32_t)get_num(env, m, "model", 0);
-    o->E = (float)get_num(env, m, "E", 1);
-    o->mu = (float)get_num(env, m, "mu", .4);
-    o->lambda = (float)get_num(env, m, "lambda", .4);
-    o->yield_strain = (float)get_num(env, m, "yieldStrain", 0);
-    o->friction = (float)get_num(env, m, "friction", 0);
-    /* Reference defaults: src/three/thermal-plasticity.js:3, metal-plasticity.js:3 */
-    o->th_Tref = 20; o->th_Thot = 180; o->th_hotRatio = .08f; o->th_A = .04f; o->th_H = .04f;
-    o->th_rate = .0005f; o->th_Q = 35000; o->th_n = 3; o->th_stress = .04f; o->th_kelvin_offset = 273.15f;
-    o->pl_A = .025f; o->pl_B = .08f; o->pl_n = .35f; o->pl_C = .025f; o->pl_rate0 = 1; o->pl_Tref = 20;
-    o->pl_Tm = 1000; o->pl_m = 1;
+    /* the reference's defaults for every record, then the given fields */
+    mpm_material_defaults((int32_t)get_num(env, m, "model", 0), (float)get_num(env, m, "E", 1), o);
+    o->mu = (float)get_num(env, m, "mu", o->mu);
+    o->lambda = (float)get_num(env, m, "lambda", o->lambda);
+    o->yield_strain = (float)get_num(env, m, "yieldStrain", o->yield_strain);
+    o->friction = (float)get_num(env, m, "friction", o->friction);
+    if (napi_has_named_property(env, m, "pressure", &has) == napi_ok && has &&
+        napi_get_named_property(env, m, "pressure", &pr) == napi_ok) {
+      o->pr_cohesion = (float)get_num(env, pr, "cohesion", o->pr_cohesion);
+      o->pr_slope = (float)get_num(env, pr, "slope", o->pr_slope);
+      o->pr_dilatancy = (float)get_num(env, pr, "dilatancy", o->pr_dilatancy);
+      o->pr_tension = (float)get_num(env, pr, "tension", o->pr_tension);
+      o->pr_alpha = (float)get_num(env, pr, "alpha", o->pr_alpha);
+      o->pr_plateau = (float)get_num(env, pr, "plateau", o->pr_plateau);
+      o->pr_gamma = (float)get_num(env, pr, "gamma", o->pr_gamma);
+      o->pr_densify = (float)get_num(env, pr, "densify", o->pr_densify);
+      o->pr_beta = (float)get_num(env, pr, "beta", o->pr_beta);
+      o->pr_epsD = (float)get_num(env, pr, "epsD", o->pr_epsD);
+    }
+    if (napi_has_named_property(env, m, "fracture", &has) == napi_ok && has &&
+        napi_get_named_property(env, m, "fracture", &fr) == napi_ok) {
+      bool on = false;
+      napi_value v;
+      if (napi_get_named_property(env, fr, "enabled", &v) == napi_ok) napi_get_value_bool(env, v, &on);
+      o->fracture = on ? 1 : 0;
+      o->fr_onset = (float)get_num(env, fr, "onset", o->fr_onset);
+      o->fr_failure = (float)get_num(env, fr, "failure", o->fr_failure);
+    }
     if (napi_has_named_property(env, m, "thermal", &has) == napi_ok && has &&
         napi_get_named_property(env, m, "thermal", &th) == napi_ok) {
       o->th_Tref = (float)get_num(env, th, "Tref", o->th_Tref);
@@ -204,7 +222,7 @@ static int particle_view(napi_env env, napi_value obj, mpm_particles* p) {
                 {"volume", &p->volume, 0}, {"temperature", &p->temperature, 0}, {"plastic", &p->plastic, 0},
                 {"cachedTau", &p->cached_tau, 0}, {"material", (void**)&p->material, 1},
                 {"pinned", (void**)&p->pinned, 1}, {"cacheValid", (void**)&p->cache_valid, 1},
-                {"jp", &p->jp, 0}, {"dilation", &p->dilation, 0}};
+                {"jp", &p->jp, 0}, {"dilation", &p->dilation, 0}, {"damage", &p->damage, 0}};
   for (size_t k = 0; k < sizeof fields / sizeof fields[0]; ++k) {
     bool has = false;
     napi_value v;

@@ -72,8 +72,65 @@ def test_invalid_arguments_rejected_before_any_device_call():
     assert lib.mpm_step(None, 1e-3, 1) == L.MPM_ERR_INVALID_ARGUMENT
     assert lib.mpm_destroy(None) == L.MPM_OK
     m = L.MpmMaterial()
-    m.model = 8  # pressure plasticity: not supported by the engine
+    m.model = 12  # past the reference's MODELS table
     assert lib.mpm_return_map_batch(C.byref(m), 0, 0.0, None, None, None, None, None, None) == L.MPM_ERR_UNSUPPORTED
+
+
+def defaults(model, E=1.0):
+    m = L.MpmMaterial()
+    assert L.load().mpm_material_defaults(model, E, C.byref(m)) == L.MPM_OK
+    return m
+
+
+def test_material_defaults_are_the_reference_records():
+    m = defaults(9, 2.0)
+    assert (m.model, m.E, m.mu, m.lambda_) == (9, 2.0, pytest.approx(.8), pytest.approx(.8))
+    want = dict(cohesion=.025, slope=.6, dilatancy=.15, tension=.01, alpha=1.5, plateau=.015, gamma=.01,
+                densify=.02, beta=2.0, epsD=.7)  # src/three/pressure-plasticity.js:2
+    for k, v in want.items():
+        assert getattr(m, "pr_" + k) == pytest.approx(v)
+    assert (m.fracture, m.fr_onset, m.fr_failure) == (0, pytest.approx(.08), pytest.approx(.16))
+    assert (m.yield_strain, m.friction, m.th_Thot, m.pl_Tm) == (pytest.approx(.025), 35.0, 180.0, 1000.0)
+    from mpm_amd import Material
+    c = Material(9, 2.0, .8, .8).to_c()  # the Python record fills the same defaults
+    for name, _ in L.MpmMaterial._fields_:
+        if name in ("yield_strain", "friction"):  # 0 there selects the same defaults on the device
+            assert getattr(c, name) == 0
+            continue
+        assert getattr(c, name) == pytest.approx(getattr(m, name)), name
+
+
+def test_material_check_follows_the_reference_parameter_rules():
+    lib = L.load()
+    for model in range(12):
+        assert lib.mpm_material_check(C.byref(defaults(model))) == L.MPM_OK
+    assert lib.mpm_material_check(C.byref(defaults(-1))) == L.MPM_ERR_UNSUPPORTED
+    # tests/pressure-plasticity.test.mjs:51: each of these pressure records is rejected
+    for bad in (dict(alpha=0), dict(epsD=0), dict(densify=0), dict(dilatancy=.8), dict(tension=.2),
+                dict(cohesion=float("nan")), dict(beta=float("inf"))):
+        for model in (8, 9):
+            m = defaults(model)
+            for k, v in bad.items():
+                setattr(m, "pr_" + k, v)
+            assert lib.mpm_material_check(C.byref(m)) == L.MPM_ERR_INVALID_ARGUMENT, (model, bad)
+    m = defaults(0)
+    m.pr_alpha = 0  # the pressure record is only checked for models 8 and 9
+    assert lib.mpm_material_check(C.byref(m)) == L.MPM_OK
+    for ok in (dict(alpha=.5), dict(alpha=2.0), dict(alpha=4.5 ** .5)):
+        m = defaults(9)
+        for k, v in ok.items():
+            setattr(m, "pr_" + k, v)
+        assert lib.mpm_material_check(C.byref(m)) == L.MPM_OK, ok
+    # src/three/fracture.js:8: .005 <= onset < failure <= 2
+    for onset, failure, rc in ((.04, .1, L.MPM_OK), (.004, .1, L.MPM_ERR_INVALID_ARGUMENT),
+                               (.1, .1, L.MPM_ERR_INVALID_ARGUMENT), (.1, 2.5, L.MPM_ERR_INVALID_ARGUMENT),
+                               (.005, 2.0, L.MPM_OK)):
+        m = defaults(0)
+        m.fracture, m.fr_onset, m.fr_failure = 1, onset, failure
+        assert lib.mpm_material_check(C.byref(m)) == rc, (onset, failure)
+    m = defaults(0)
+    m.E = 0
+    assert lib.mpm_material_check(C.byref(m)) == L.MPM_ERR_INVALID_ARGUMENT
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
