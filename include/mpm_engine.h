@@ -227,6 +227,51 @@ int mpm_stable_dt(mpm_engine* e, double cap, double cfl, double* out);
  * min_J replaces MPM3.minJ(s) (src/three/mpm.js:135). Synchronising. */
 int mpm_conservation_report(mpm_engine* e, mpm_conservation* out);
 
+/* ---- frame snapshot and wire packet (the GPU service protocol) ----
+ *
+ * The reference's GPU service sends one binary packet per frame
+ * (gpu/server.py:17-20): a u32 little-endian header length, the header (a
+ * compact UTF-8 JSON object), spaces up to a multiple of 4 bytes, then 8
+ * little-endian float32 per previewed particle: x, y, z, material code, T,
+ * pinned, stress, plastic (gpu/solver.py:986-1032). The browser worker
+ * checks that the sizes agree (src/three/gpu-worker.js:36-42). */
+
+/* Totals of a frame (the header fields of gpu/solver.py:1024-1030). All
+ * particles of the engine are MPM solids, so liquids is 0. */
+typedef struct {
+  uint32_t count;          /* particles */
+  uint32_t render_count;   /* rows: ceil(count / sample_stride) */
+  uint32_t sample_stride;  /* max(1, ceil(count / preview_limit)) */
+  int64_t solids, liquids;
+  int64_t models[12];      /* particles per effective model (a failed particle counts as 3) */
+  double mass;             /* sum m */
+  double min_temperature, max_temperature; /* 0 and 0 for an empty engine */
+  double min_J;            /* min(1, min det F) */
+  double max_stress;       /* max of the von Mises diagnostic, from 0 */
+  double max_plastic;      /* max plastic strain, from 0 */
+  int64_t nonfinite;       /* particles with |x|^2 or |v|^2 not below 1e20 */
+} mpm_snapshot_stats;
+
+/* Stride and row count of a preview of `count` particles limited to
+ * `preview_limit` rows (the service uses 100000). Host only. */
+int mpm_preview_layout(uint32_t count, uint32_t preview_limit, uint32_t* sample_stride, uint32_t* render_count);
+
+/* The frame snapshot of the resident state: one row per block of
+ * sample_stride consecutive particle ids, holding the particle the service's
+ * hash picks in that block (gpu/solver.py:990-996); data is [render_count *
+ * 8] host floats, or NULL for the totals only. codes [ncodes] gives each
+ * material's code (11, the MPM solid, shows as 11 + model); NULL gives 11
+ * for every material. Replaces Solver.snapshot and Simulation3.snapshot's
+ * packing (src/three/simulation.js:390-400). */
+int mpm_snapshot(mpm_engine* e, uint32_t preview_limit, const int32_t* codes, int32_t ncodes, float* data,
+                 mpm_snapshot_stats* stats);
+
+/* Builds one frame packet into out [capacity]: *length receives the packet
+ * size (4 + json_len + padding + rows * 32). With out NULL only *length is
+ * set; a capacity below it gives MPM_ERR_CAPACITY. Host only. */
+int mpm_frame_packet(const char* json, uint32_t json_len, const float* data, uint64_t rows, uint8_t* out,
+                     uint64_t capacity, uint64_t* length);
+
 /* Per-particle von Mises diagnostic (Cauchy von Mises / E), in the caller's
  * particle order; replaces MPM3.refreshDiagnostics (src/three/mpm.js:40-42).
  * stress is [count] FP64. */

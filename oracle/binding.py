@@ -76,6 +76,13 @@ PRESSURE_DEFAULTS = dict(cohesion=.025, slope=.6, dilatancy=.15, tension=.01, al
 FRACTURE_DEFAULTS = dict(enabled=0, onset=.08, failure=.16)
 
 
+class OrcSnapshot(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("render_count", C.c_uint32), ("sample_stride", C.c_uint32),
+                ("solids", C.c_int64), ("liquids", C.c_int64), ("models", C.c_int64 * 12),
+                ("mass", D), ("min_temperature", D), ("max_temperature", D), ("min_J", D), ("max_stress", D),
+                ("max_plastic", D), ("nonfinite", C.c_int64)]
+
+
 class OrcFrame(C.Structure):
     _fields_ = [("time_step_scale", D), ("max_substeps", C.c_int32), ("velocity_limit", D),
                 ("bounds_lo", D * 3), ("bounds_hi", D * 3), ("anti_clump", C.c_int32), ("gravity_mode", C.c_int32),
@@ -112,6 +119,10 @@ def lib():
         L.orc_principal_stress.argtypes = [PD, PD, C.c_int, D, PD]
         L.orc_von_mises.restype = D
         L.orc_von_mises.argtypes = [PD, D]
+        L.orc_preview_pick.restype = C.c_uint32
+        L.orc_preview_pick.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_snapshot.argtypes = [C.POINTER(OrcScene), C.c_uint32, PI32, C.POINTER(C.c_float),
+                                   C.POINTER(OrcSnapshot)]
         L.orc_foam_strength.restype = D
         L.orc_foam_strength.argtypes = [D, C.POINTER(OrcPressure)]
         L.orc_pressure_return.argtypes = [PD, C.c_int, D, D, C.POINTER(OrcPressure), D, D, PD, PD, PD]
@@ -416,6 +427,19 @@ class Scene:
         if n < 0:
             raise ValueError("unsupported model or part index in scene")
         return n, adv.value
+
+    def snapshot(self, limit=100_000, codes=None):
+        """(stats dict, data [rows, 8] float32) of the service's frame pack."""
+        lib().orc_refresh_diagnostics(C.byref(self.s))
+        st = OrcSnapshot()
+        lib().orc_snapshot(C.byref(self.s), limit, None, None, C.byref(st))
+        data = np.zeros((st.render_count, 8), np.float32)
+        c = None if codes is None else np.ascontiguousarray(codes, np.int32)
+        lib().orc_snapshot(C.byref(self.s), limit, None if c is None else _p(c, PI32),
+                           data.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in OrcSnapshot._fields_ if k != "models"}
+        stats["models"] = list(st.models)
+        return stats, data
 
     def p2g(self, dt):
         lib().orc_p2g(C.byref(self.s), dt)

@@ -1178,4 +1178,70 @@ double orc_mlp_loss_grad(const double* P, int n, const double* x, const double* 
   }
   return loss / n;
 }
+
+// ------------------------------------------------------------ frame snapshot
+// Preview pick of the service's pack kernel: one sample per block of
+// `stride` consecutive particles, chosen by an integer hash of the block
+// index so the preview shows no lattice stripes. gpu/solver.py:990-996
+uint32_t orc_preview_pick(uint32_t row, uint32_t stride, uint32_t n) {
+  uint32_t h = row * 747796405u + 2891336453u;
+  h = ((h >> ((h >> 28) + 4u)) ^ h) * 277803737u;
+  h = (h >> 22) ^ h;
+  const uint32_t span = std::min(stride, n - row * stride);
+  return h % span;
+}
+
+typedef struct {
+  uint32_t count, render_count, sample_stride;
+  int64_t solids, liquids;
+  int64_t models[12];
+  double mass, min_temperature, max_temperature, min_J, max_stress, max_plastic;
+  int64_t nonfinite;
+} OrcSnapshot;
+
+// Frame snapshot of an all-solid scene: stride = max(1, ceil(n / limit)),
+// rows = ceil(n / stride); row r holds x, y, z, material code, T, pinned,
+// stress, plastic of its picked particle (code 11 + model for the MPM solid
+// material 11, else the code itself; `codes` per part, NULL: 11). Totals:
+// mass, per-model counts, min/max T, min det F (from 1), max stress and
+// plastic strain (from 0), particles with |x|^2 or |v|^2 not below 1e20 or
+// non-finite. The stress is the scene's diagnostic (orc_refresh_diagnostics).
+// gpu/solver.py:986-1032, src/three/simulation.js:390-400
+void orc_snapshot(const OrcScene* s, uint32_t limit, const int32_t* codes, float* data, OrcSnapshot* out) {
+  const uint32_t n = (uint32_t)s->count;
+  OrcSnapshot r{};
+  r.count = n;
+  r.sample_stride = std::max<uint32_t>(1, (uint32_t)((n + (uint64_t)limit - 1) / limit));
+  r.render_count = (uint32_t)((n + (uint64_t)r.sample_stride - 1) / r.sample_stride);
+  r.min_temperature = 1e30;
+  r.max_temperature = -1e30;
+  r.min_J = 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t row = i / r.sample_stride;
+    if (data && i % r.sample_stride == orc_preview_pick(row, r.sample_stride, n)) {
+      float* d = data + 8 * (size_t)row;
+      const int32_t c = codes ? codes[s->part[i]] : 11;
+      d[0] = (float)s->x[i];
+      d[1] = (float)s->y[i];
+      d[2] = (float)s->z[i];
+      d[3] = (float)(c == 11 ? 11 + s->model[i] : c);
+      d[4] = (float)s->temperature[i];
+      d[5] = (float)s->pinned[i];
+      d[6] = (float)s->stress[i];
+      d[7] = (float)s->plastic[i];
+    }
+    const double x2 = s->x[i] * s->x[i] + s->y[i] * s->y[i] + s->z[i] * s->z[i];
+    const double v2 = s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i] + s->vz[i] * s->vz[i];
+    if (!(x2 < 1e20 && v2 < 1e20)) ++r.nonfinite;
+    r.mass += s->mass[i];
+    ++r.solids;
+    if (s->model[i] >= 0 && s->model[i] < 12) ++r.models[s->model[i]];
+    r.min_J = std::min(r.min_J, det3(s->F + 9 * (size_t)i));
+    r.min_temperature = std::min(r.min_temperature, s->temperature[i]);
+    r.max_temperature = std::max(r.max_temperature, s->temperature[i]);
+    r.max_stress = std::max(r.max_stress, s->stress[i]);
+    r.max_plastic = std::max(r.max_plastic, s->plastic[i]);
+  }
+  *out = r;
+}
 }  // extern "C"

@@ -612,6 +612,94 @@ int mpm_refresh_diagnostics(mpm_engine* e, double* stress) {
   return MPM_OK;
 }
 
+int mpm_preview_layout(uint32_t count, uint32_t limit, uint32_t* stride, uint32_t* rows) {
+  if (!stride || !rows || limit == 0) return MPM_ERR_INVALID_ARGUMENT;
+  *stride = std::max<uint32_t>(1, (uint32_t)((count + (uint64_t)limit - 1) / limit));
+  *rows = (uint32_t)((count + (uint64_t)*stride - 1) / *stride);
+  return MPM_OK;
+}
+
+int mpm_snapshot(mpm_engine* e, uint32_t limit, const int32_t* codes, int32_t ncodes, float* data,
+                 mpm_snapshot_stats* st) {
+  if (!e || !st) return MPM_ERR_INVALID_ARGUMENT;
+  const uint32_t n = e->n;
+  uint32_t stride, rows;
+  if (mpm_preview_layout(n, limit, &stride, &rows)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "preview limit 0");
+  if (codes && ncodes < (int32_t)e->materials.size())
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "%d material codes for %zu materials", ncodes, e->materials.size());
+  *st = mpm_snapshot_stats{};
+  st->count = n;
+  st->sample_stride = stride;
+  st->render_count = rows;
+  st->min_J = 1;
+  if (!n) return MPM_OK;
+  int32_t table[64];
+  for (int k = 0; k < 64; ++k) table[k] = codes && k < ncodes ? codes[k] : 11;
+  const uint32_t nb = snapshot_blocks(n);
+  int32_t* dcodes = nullptr;
+  float* ddata = nullptr;
+  double* dpart = nullptr;
+  unsigned long long* dcounts = nullptr;
+  hipError_t err = hipMallocAsync((void**)&dcodes, sizeof table, e->stream);
+  if (err == hipSuccess) err = hipMallocAsync((void**)&dpart, sizeof(double) * 6 * nb, e->stream);
+  if (err == hipSuccess) err = hipMallocAsync((void**)&dcounts, sizeof(unsigned long long) * 13, e->stream);
+  if (err == hipSuccess && data) err = hipMallocAsync((void**)&ddata, sizeof(float) * 8 * (size_t)rows, e->stream);
+  if (err == hipSuccess) err = hipMemcpyAsync(dcodes, table, sizeof table, hipMemcpyHostToDevice, e->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(dcounts, 0, sizeof(unsigned long long) * 13, e->stream);
+  std::vector<double> part(6 * (size_t)nb);
+  unsigned long long counts[13];
+  if (err == hipSuccess) {
+    launch_snapshot(e->stream, e->d, n, dcodes, stride, ddata, dpart, dcounts);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(part.data(), dpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess) err = hipMemcpyAsync(counts, dcounts, sizeof counts, hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess && data)
+    err = hipMemcpyAsync(data, ddata, sizeof(float) * 8 * (size_t)rows, hipMemcpyDeviceToHost, e->stream);
+  for (void* p : {(void*)dcodes, (void*)ddata, (void*)dpart, (void*)dcounts})
+    if (p) (void)hipFreeAsync(p, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return fail(e, MPM_ERR_HIP, "snapshot: %s", hipGetErrorString(err));
+  st->min_temperature = 1e30;
+  st->max_temperature = -1e30;
+  for (uint32_t b = 0; b < nb; ++b) {
+    const double* r = &part[6 * (size_t)b];
+    st->mass += r[0];
+    st->min_temperature = std::min(st->min_temperature, r[1]);
+    st->max_temperature = std::max(st->max_temperature, r[2]);
+    st->min_J = std::min(st->min_J, r[3]);
+    st->max_stress = std::max(st->max_stress, r[4]);
+    st->max_plastic = std::max(st->max_plastic, r[5]);
+  }
+  for (int k = 0; k < 12; ++k) {
+    st->models[k] = (int64_t)counts[k];
+    st->solids += (int64_t)counts[k];
+  }
+  st->nonfinite = (int64_t)counts[12];
+  return MPM_OK;
+}
+
+int mpm_frame_packet(const char* json, uint32_t json_len, const float* data, uint64_t rows, uint8_t* out,
+                     uint64_t capacity, uint64_t* length) {
+  if (!length || (json_len && !json) || (rows && !data)) return MPM_ERR_INVALID_ARGUMENT;
+  const uint64_t pad = (4 - json_len % 4) % 4;
+  const uint64_t need = 4 + json_len + pad + 32 * rows;
+  *length = need;
+  if (!out) return MPM_OK;
+  if (capacity < need) return MPM_ERR_CAPACITY;
+  for (int k = 0; k < 4; ++k) out[k] = (uint8_t)(json_len >> (8 * k));  // little-endian u32
+  if (json_len) std::memcpy(out + 4, json, json_len);
+  std::memset(out + 4 + json_len, ' ', pad);
+  uint8_t* q = out + 4 + json_len + pad;
+  for (uint64_t i = 0; i < 8 * rows; ++i) {
+    uint32_t bits;
+    std::memcpy(&bits, data + i, 4);
+    for (int k = 0; k < 4; ++k) q[4 * i + k] = (uint8_t)(bits >> (8 * k));  // little-endian float32
+  }
+  return MPM_OK;
+}
+
 int mpm_profile(mpm_engine* e, int32_t enable, int32_t reset) {
   if (!e) return MPM_ERR_INVALID_ARGUMENT;
   collect_times(e);

@@ -161,6 +161,12 @@ void launch_particle_reduce(hipStream_t s, const DevState& d, uint32_t n, float 
 // Per block {max wave speed, max sum |C|}; particle_reduce_blocks() records.
 void launch_stable(hipStream_t s, const DevState& d, uint32_t n, float* out);
 void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out);
+// Frame snapshot: rows of `data` [ceil(n / stride) x 8] (may be null), FP64
+// block totals `partial` [snapshot_blocks(n) x 6] and counters `counts`
+// [13] (per model, then non-finite), which the caller zeroes.
+uint32_t snapshot_blocks(uint32_t n);
+void launch_snapshot(hipStream_t s, const DevState& d, uint32_t n, const int32_t* codes, uint32_t stride, float* data,
+                     double* partial, unsigned long long* counts);
 void launch_return_batch(hipStream_t s, const DevMaterial& m, uint32_t n, float dt, const float* F, const float* ep,
                          const float* T, const float* jp, const float* dil, float* F_out, float* ep_out,
                          float* jp_out, float* dil_out, float* tau_out);

@@ -581,8 +581,20 @@ __global__ __launch_bounds__(BLOCK) void k_stable(const float* __restrict__ P, c
   }
 }
 
-// Von Mises of this step's Kirchhoff stress over J and E
-// (src/three/mpm.js:38), per slot.
+// Von Mises of this step's Kirchhoff stress over J (J Jp for the pressure
+// models) and E: the diagnostic MPM3.stress (src/three/mpm.js:38). F is the
+// slot's deformation gradient, `model` its effective model.
+__device__ __forceinline__ float diag_stress(const float* __restrict__ P, uint32_t cap, uint32_t i, uint32_t flags,
+                                             const DevMaterial& mat, int model, const M3& F) {
+  float t[6];
+  step_stress(F, model, mat, flags, P, cap, i, t);
+  const float vm = sqrtf(fmaxf(0.f, 0.5f * ((t[0] - t[1]) * (t[0] - t[1]) + (t[1] - t[2]) * (t[1] - t[2]) +
+                                            (t[2] - t[0]) * (t[2] - t[0])) +
+                                         3.f * (t[3] * t[3] + t[4] * t[4] + t[5] * t[5])));
+  const float J = det3(F) * ((model == 8 || model == 9) ? plane(P, cap, P_JP, i) : 1.f);
+  return vm / fmaxf(1e-6f, J) / mat.E;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_von_mises(const float* __restrict__ P, const uint32_t* __restrict__ meta,
                                                      uint32_t cap, uint32_t n, const DevMaterial* __restrict__ mats,
                                                      float* __restrict__ out) {
@@ -593,14 +605,88 @@ __global__ __launch_bounds__(BLOCK) void k_von_mises(const float* __restrict__ P
   M3 F;
 #pragma unroll
   for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
-  float t[6];
   const int model = effective_model(mat, mat.fr_on ? plane(P, cap, P_DMG, i) : 0.f);
-  step_stress(F, model, mat, mt >> 8, P, cap, i, t);
-  const float vm = sqrtf(fmaxf(0.f, 0.5f * ((t[0] - t[1]) * (t[0] - t[1]) + (t[1] - t[2]) * (t[1] - t[2]) +
-                                            (t[2] - t[0]) * (t[2] - t[0])) +
-                                         3.f * (t[3] * t[3] + t[4] * t[4] + t[5] * t[5])));
-  const float J = det3(F) * ((model == 8 || model == 9) ? plane(P, cap, P_JP, i) : 1.f);
-  out[i] = vm / fmaxf(1e-6f, J) / mat.E;
+  out[i] = diag_stress(P, cap, i, mt >> 8, mat, model, F);
+}
+
+// Preview pick of one particle per block of `stride` consecutive original
+// ids: the PCG RXS-M-XS output of one LCG step of the block index, modulo the
+// block's length (gpu/solver.py:990-996).
+__device__ __forceinline__ uint32_t preview_pick(uint32_t row, uint32_t stride, uint32_t n) {
+  uint32_t h = row * 747796405u + 2891336453u;
+  h = ((h >> ((h >> 28) + 4u)) ^ h) * 277803737u;
+  h = (h >> 22) ^ h;
+  const uint32_t span = min(stride, n - row * stride);
+  return h % span;
+}
+
+constexpr int SNAP_K = 6;  // mass, min T, max T, min J, max stress, max plastic
+
+// Frame snapshot (gpu/solver.py:986-1016): every slot whose original id is
+// its block's pick writes its row (x, y, z, material code, T, pinned,
+// diagnostic stress, plastic strain) -- one writer per row; every slot adds
+// to the block totals (FP64) and to the per-model and non-finite counters.
+__global__ __launch_bounds__(BLOCK) void k_snapshot(const float* __restrict__ P, const uint32_t* __restrict__ meta,
+                                                    const uint32_t* __restrict__ ids, uint32_t cap, uint32_t n,
+                                                    const DevMaterial* __restrict__ mats,
+                                                    const int32_t* __restrict__ codes, uint32_t stride,
+                                                    float* __restrict__ data, double* __restrict__ partial,
+                                                    unsigned long long* __restrict__ counts) {
+  __shared__ double s[BLOCK / 64];
+  __shared__ unsigned int s_counts[13];
+  if (threadIdx.x < 13) s_counts[threadIdx.x] = 0u;
+  __syncthreads();
+  double mass = 0, tmin = 1e30, tmax = -1e30, jmin = 1, smax = 0, pmax = 0;
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    const uint32_t mt = meta[i];
+    const uint32_t mi = mt & 255u;
+    const DevMaterial mat = mats[mi];
+    const int model = effective_model(mat, mat.fr_on ? plane(P, cap, P_DMG, i) : 0.f);
+    M3 F;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
+    const float stress = diag_stress(P, cap, i, mt >> 8, mat, model, F);
+    const float x = plane(P, cap, P_X, i), y = plane(P, cap, P_Y, i), z = plane(P, cap, P_Z, i);
+    const float vx = plane(P, cap, P_VX, i), vy = plane(P, cap, P_VY, i), vz = plane(P, cap, P_VZ, i);
+    const float T = plane(P, cap, P_TEMP, i), ep = plane(P, cap, P_EP, i);
+    const uint32_t id = ids[i];
+    const uint32_t row = id / stride;
+    if (data && id - row * stride == preview_pick(row, stride, n)) {
+      const int32_t c = codes[mi];
+      float* d = data + 8 * (size_t)row;
+      d[0] = x;
+      d[1] = y;
+      d[2] = z;
+      d[3] = (float)(c == 11 ? 11 + model : c);
+      d[4] = T;
+      d[5] = ((mt >> 8) & FLAG_PINNED) ? 1.f : 0.f;
+      d[6] = stress;
+      d[7] = ep;
+    }
+    const double x2 = (double)x * x + (double)y * y + (double)z * z;
+    const double v2 = (double)vx * vx + (double)vy * vy + (double)vz * vz;
+    if (!(x2 < 1e20 && v2 < 1e20)) atomicAdd(&s_counts[12], 1u);
+    atomicAdd(&s_counts[model], 1u);
+    mass += plane(P, cap, P_MASS, i);
+    tmin = fmin(tmin, (double)T);
+    tmax = fmax(tmax, (double)T);
+    jmin = fmin(jmin, (double)det3(F));
+    smax = fmax(smax, (double)stress);
+    pmax = fmax(pmax, (double)ep);
+  }
+  double r[SNAP_K];
+  r[0] = block_sum(mass, s);
+  r[1] = block_min(tmin, s);
+  r[2] = -block_min(-tmax, s);
+  r[3] = block_min(jmin, s);
+  r[4] = -block_min(-smax, s);
+  r[5] = -block_min(-pmax, s);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < SNAP_K; ++k) partial[blockIdx.x * SNAP_K + k] = r[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 13 && s_counts[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)s_counts[threadIdx.x]);
 }
 
 // Plain constitutive update of a batch of deformation gradients: the return
@@ -744,6 +830,14 @@ void launch_return_batch(hipStream_t s, const DevMaterial& m, uint32_t n, float 
 void launch_stress_batch(hipStream_t s, const DevMaterial& m, uint32_t n, const float* F, float* tau_out) {
   if (!n) return;
   hipLaunchKernelGGL(k_stress_batch, dim3(blocks(n)), dim3(BLOCK), 0, s, m, n, F, tau_out);
+}
+
+uint32_t snapshot_blocks(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(RED_BLOCKS, blocks(n))); }
+
+void launch_snapshot(hipStream_t s, const DevState& d, uint32_t n, const int32_t* codes, uint32_t stride, float* data,
+                     double* partial, unsigned long long* counts) {
+  hipLaunchKernelGGL(k_snapshot, dim3(snapshot_blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.ids, d.cap, n,
+                     d.materials, codes, stride, data, partial, counts);
 }
 
 void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out) {

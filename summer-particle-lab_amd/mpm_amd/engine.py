@@ -165,6 +165,7 @@ class Engine:
         L.check(self._lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h
         self.config = cfg
+        self.capacity = int(capacity)
         self.count = 0
 
     def close(self):
@@ -213,6 +214,22 @@ class Engine:
 
     def min_j(self):
         return self.conservation()["min_J"]
+
+    def snapshot(self, preview_limit=100_000, codes=None):
+        """(totals dict, preview rows [render_count, 8] float32) of the
+        resident state (mpm_snapshot)."""
+        stride, rows = C.c_uint32(), C.c_uint32()
+        L.check(self._lib.mpm_preview_layout(self.count, preview_limit, C.byref(stride), C.byref(rows)))
+        data = np.zeros((rows.value, 8), np.float32)
+        c = None if codes is None else np.ascontiguousarray(codes, np.int32)
+        st = L.MpmSnapshotStats()
+        self._check(self._lib.mpm_snapshot(self.h, preview_limit,
+                                           None if c is None else c.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           0 if c is None else len(c), data.ctypes.data_as(C.POINTER(C.c_float)),
+                                           C.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in L.MpmSnapshotStats._fields_ if k != "models"}
+        stats["models"] = list(st.models)
+        return stats, data
 
     def refresh_diagnostics(self):
         out = np.zeros(self.count)

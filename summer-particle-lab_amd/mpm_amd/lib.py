@@ -61,6 +61,14 @@ class MpmParticles(C.Structure):
     ]
 
 
+class MpmSnapshotStats(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("render_count", C.c_uint32), ("sample_stride", C.c_uint32),
+                ("solids", C.c_int64), ("liquids", C.c_int64), ("models", C.c_int64 * 12),
+                ("mass", C.c_double), ("min_temperature", C.c_double), ("max_temperature", C.c_double),
+                ("min_J", C.c_double), ("max_stress", C.c_double), ("max_plastic", C.c_double),
+                ("nonfinite", C.c_int64)]
+
+
 class MpmConservation(C.Structure):
     _fields_ = [
         ("particle_mass", C.c_double), ("grid_mass", C.c_double),
@@ -126,6 +134,11 @@ SIGNATURES = {
     "mpm_stable_dt": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(C.c_double)]),
     "mpm_conservation_report": (C.c_int, [_P, C.POINTER(MpmConservation)]),
     "mpm_refresh_diagnostics": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mpm_preview_layout": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mpm_snapshot": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_float),
+                               C.POINTER(MpmSnapshotStats)]),
+    "mpm_frame_packet": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
+                                   C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_uint64)]),
     "mpm_synchronize": (C.c_int, [_P]),
     "mpm_count": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "mpm_frame_defaults": (C.c_int, [_P, C.POINTER(MpmFrameConfig)]),

@@ -82,7 +82,9 @@ class MPM3 {
       // Snow's yield strain and sand's friction angle are per particle in the
       // reference (src/three/simulation.js:40) and constant within a part.
       const ys = s.yieldStrain ? s.yieldStrain[p] : 0.025, fr = s.friction ? s.friction[p] : 35;
-      const key = `${model}|${part}|${E}|${ys}|${fr}`;
+      // Frame code of the particle's material: 11 (MPM solid) shows as 11 + model.
+      const code = s.material ? s.material[p] : 11;
+      const key = `${model}|${part}|${E}|${ys}|${fr}|${code}`;
       let m = keys.get(key);
       if (m === undefined) {
         m = materials.length;
@@ -101,7 +103,7 @@ class MPM3 {
         if (Number.isFinite(fm.onset)) fracture.onset = fm.onset;
         if (Number.isFinite(fm.failure)) fracture.failure = fm.failure;
         materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic, pressure, fracture,
-          yieldStrain: ys, friction: fr});
+          yieldStrain: ys, friction: fr, code});
       }
       material[ids.length] = m;
       ids.push(p);
@@ -143,7 +145,7 @@ class MPM3 {
     }
     native.setMaterials(this.engine, materials.length ? materials : [{model: 0, E: 1, mu: .4, lambda: .4}]);
     native.upload(this.engine, v);
-    this._ids = ids; this._view_ = v; this.onDevice = 1;
+    this._ids = ids; this._view_ = v; this._materials = materials; this.onDevice = 1;
   }
 
   // Copy device state back into `s` and this object.
@@ -209,6 +211,26 @@ class MPM3 {
     s.time = (s.time || 0) + r.advanced; s.substeps = r.substeps;
     return r;
   }
+
+  // The service's frame of the device state (gpu/solver.py:986-1032):
+  // {state, data} with the header totals and one preview row of 8 floats
+  // (x, y, z, material code, T, pinned, stress, plastic) per block of
+  // sampleStride particles. Rows follow the solid particles' order.
+  snapshot(s, previewLimit = 100000) {
+    if (!this.options.resident || !this.onDevice) this.upload(s);
+    const codes = this._materials.map((m) => m.code);
+    const r = native.snapshot(this.engine, previewLimit, codes);
+    if (r.nonfinite) throw new Error('Non-finite GPU state; simulation stopped');
+    const data = r.data;
+    delete r.data;
+    const state = Object.assign({type: 'frame', capacity: this.capacity, energy: 0, time: s.time || 0,
+      substeps: s.substeps || 0}, r);
+    delete state.nonfinite;
+    return {state, data};
+  }
+
+  // One binary frame packet (gpu/server.py:17-20) as an ArrayBuffer.
+  framePacket(state, data) { return native.framePacket(JSON.stringify(state), data); }
 
   refreshDiagnostics(s) {
     if (!this.options.resident || !this.onDevice) this.upload(s);

@@ -423,6 +423,105 @@ static napi_value js_conservation(napi_env env, napi_callback_info info) {
   return o;
 }
 
+/* snapshot(h, previewLimit, codes|null) -> {count, renderCount, sampleStride,
+ * solids, liquids, models, mass, min, max, minJ, maxStress, maxPlastic,
+ * nonfinite, data: Float32Array(renderCount * 8)} */
+static napi_value js_snapshot(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  double limit = 0;
+  if (napi_get_value_double(env, argv[1], &limit) != napi_ok || !(limit >= 1) || limit > 4294967295.0) {
+    napi_throw_range_error(env, NULL, "previewLimit must be a positive integer");
+    return NULL;
+  }
+  int32_t codes[64];
+  int32_t ncodes = 0;
+  bool is_array = false;
+  if (argc >= 3) napi_is_array(env, argv[2], &is_array);
+  if (is_array) {
+    uint32_t n = 0;
+    napi_get_array_length(env, argv[2], &n);
+    if (n > 64) n = 64;
+    for (uint32_t k = 0; k < n; ++k) {
+      napi_value v;
+      double d = 11;
+      napi_get_element(env, argv[2], k, &v);
+      napi_get_value_double(env, v, &d);
+      codes[k] = (int32_t)d;
+    }
+    ncodes = (int32_t)n;
+  }
+  mpm_snapshot_stats st;
+  int rc = mpm_snapshot(e, (uint32_t)limit, is_array ? codes : NULL, ncodes, NULL, &st);
+  if (rc) return throw_mpm(env, rc, e);
+  napi_value ab, data, o;
+  void* raw = NULL;
+  CHECK_NAPI(env, napi_create_arraybuffer(env, (size_t)st.render_count * 32, &raw, &ab));
+  rc = mpm_snapshot(e, (uint32_t)limit, is_array ? codes : NULL, ncodes, (float*)raw, &st);
+  if (rc) return throw_mpm(env, rc, e);
+  CHECK_NAPI(env, napi_create_typedarray(env, napi_float32_array, (size_t)st.render_count * 8, ab, 0, &data));
+  CHECK_NAPI(env, napi_create_object(env, &o));
+  set_num(env, o, "count", st.count);
+  set_num(env, o, "renderCount", st.render_count);
+  set_num(env, o, "sampleStride", st.sample_stride);
+  set_num(env, o, "solids", (double)st.solids);
+  set_num(env, o, "liquids", (double)st.liquids);
+  double models[12];
+  for (int k = 0; k < 12; ++k) models[k] = (double)st.models[k];
+  set_vec(env, o, "models", models, 12);
+  set_num(env, o, "mass", st.mass);
+  set_num(env, o, "min", st.min_temperature);
+  set_num(env, o, "max", st.max_temperature);
+  set_num(env, o, "minJ", st.min_J);
+  set_num(env, o, "maxStress", st.max_stress);
+  set_num(env, o, "maxPlastic", st.max_plastic);
+  set_num(env, o, "nonfinite", (double)st.nonfinite);
+  napi_set_named_property(env, o, "data", data);
+  return o;
+}
+
+/* framePacket(headerString, Float32Array rows*8) -> ArrayBuffer */
+static napi_value js_frame_packet(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  size_t len = 0;
+  if (argc < 2 || napi_get_value_string_utf8(env, argv[0], NULL, 0, &len) != napi_ok) {
+    napi_throw_type_error(env, NULL, "framePacket(header string, Float32Array)");
+    return NULL;
+  }
+  char* head = (char*)malloc(len + 1);
+  if (!head) return NULL;
+  napi_get_value_string_utf8(env, argv[0], head, len + 1, &len);
+  napi_typedarray_type type;
+  size_t n = 0, offset = 0;
+  void* p = NULL;
+  napi_value buf;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[1], &is_ta);
+  if (!is_ta || napi_get_typedarray_info(env, argv[1], &type, &n, &p, &buf, &offset) != napi_ok ||
+      type != napi_float32_array || n % 8) {
+    free(head);
+    napi_throw_type_error(env, NULL, "rows must be a Float32Array of 8 values per row");
+    return NULL;
+  }
+  uint64_t need = 0;
+  mpm_frame_packet(head, (uint32_t)len, (const float*)p, n / 8, NULL, 0, &need);
+  void* out = NULL;
+  napi_value ab;
+  if (napi_create_arraybuffer(env, (size_t)need, &out, &ab) != napi_ok) {
+    free(head);
+    return NULL;
+  }
+  int rc = mpm_frame_packet(head, (uint32_t)len, (const float*)p, n / 8, (uint8_t*)out, need, &need);
+  free(head);
+  if (rc) return throw_mpm(env, rc, NULL);
+  return ab;
+}
+
 static napi_value js_refresh_diagnostics(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -476,6 +575,8 @@ static napi_value init(napi_env env, napi_value exports) {
              {"stableDt", js_stable_dt},
              {"conservation", js_conservation},
              {"refreshDiagnostics", js_refresh_diagnostics},
+             {"snapshot", js_snapshot},
+             {"framePacket", js_frame_packet},
              {"lastError", js_last_error}};
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; ++i) {
     napi_value f;

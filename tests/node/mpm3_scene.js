@@ -29,10 +29,15 @@ const frames = [];
 if (input.frames) for (let k = 0; k < input.frames; k++) frames.push(mpm.advanceFrame(s, input.dt));
 else for (let k = 0; k < input.steps; k++) mpm.step(s, input.dt);
 mpm.refreshDiagnostics(s);
+let snapshot = null;
+if (input.snapshot) {
+  const {state, data} = mpm.snapshot(s, input.snapshot);
+  snapshot = {state, data: Array.from(data), packet: Buffer.from(mpm.framePacket(state, data)).toString('base64')};
+}
 const out = {dtStable, x: Array.from(s.x), y: Array.from(s.y), z: Array.from(s.z), vx: Array.from(s.vx),
   vy: Array.from(s.vy), vz: Array.from(s.vz), F: Array.from(mpm.F.subarray(0, n * 9)),
   plastic: Array.from(mpm.plastic.subarray(0, n)), stress: Array.from(mpm.stress.subarray(0, n)),
   jp: Array.from(mpm.Jp.subarray(0, n)), dilation: Array.from(mpm.dilation.subarray(0, n)),
-  minJ: mpm.minJ(s), conservation: mpm.conservation(), frames};
+  minJ: mpm.minJ(s), conservation: mpm.conservation(), frames, snapshot};
 mpm.destroy();
 console.log(JSON.stringify(out));

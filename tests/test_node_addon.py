@@ -37,9 +37,35 @@ console.log(JSON.stringify({{names, code, typeErr}}));
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
     assert out["names"] == sorted(["create", "destroy", "setMaterials", "upload", "download", "step", "synchronize",
-                                   "stableDt", "conservation", "refreshDiagnostics", "lastError", "advanceFrame"])
+                                   "stableDt", "conservation", "refreshDiagnostics", "lastError", "advanceFrame",
+                                   "snapshot", "framePacket"])
     assert out["code"] == "MPM1"
     assert "engine handle" in out["typeErr"]
+
+
+def test_frame_packet_from_node_reads_back_and_matches_python(tmp_path):
+    """The addon's framePacket gives the bytes of the service's packet()
+    (gpu/server.py:17-20) and passes the worker's checks; a lying header is
+    refused the way the worker refuses it."""
+    import base64
+
+    from mpm_amd import wire
+    rng = np.random.default_rng(4)
+    rows = rng.normal(size=(5, 8)).astype(np.float32)
+    for header in (dict(type="frame", count=9, capacity=16, renderCount=5, sampleStride=2, note="温度"),
+                   dict(type="frame", count=4, capacity=16, renderCount=5)):
+        path = tmp_path / "packet.json"
+        path.write_text(json.dumps(dict(header=header, rows=rows.tolist())))
+        r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "frame_packet.js"), str(path)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = json.loads(r.stdout)
+        assert base64.b64decode(got["base64"]) == wire.packet(header, rows)
+        if header["count"] >= header["renderCount"]:
+            assert got["ok"] and got["state"] == header
+            np.testing.assert_array_equal(np.array(got["data"], np.float32).reshape(-1, 8), rows)
+        else:
+            assert not got["ok"] and "invalid particle frame" in got["error"]
 
 
 def four_blocks():
@@ -123,7 +149,7 @@ def test_drop_in_reference_parity_scene(gpu, tmp_path):
     F = [.98, .1, 0, 0, .95, 0, 0, 0, 1.02]
     lo, hi = [-63, -47, -63], [63, 71, 63]
     inp = dict(points=pts.tolist(), velocity=[0, 0, 0], models=models.tolist(), F=F, dt=1 / 60, frames=12,
-               part=models.tolist(), boundsLo=lo, boundsHi=hi)
+               part=models.tolist(), boundsLo=lo, boundsHi=hi, snapshot=50)
     path = tmp_path / "reference.json"
     path.write_text(json.dumps(inp))
     r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
@@ -140,3 +166,19 @@ def test_drop_in_reference_parity_scene(gpu, tmp_path):
     np.testing.assert_allclose(np.array(got["jp"]), orc.Jp, atol=2e-5, rtol=1e-4)
     assert np.abs(orc.Jp[54:81] - 1).min() > 1e-3  # the snow block yielded under the preset F
     assert got["conservation"]["gridMassQ"] == got["conservation"]["particleMassQ"]
+    # The service frame of the same state (gpu/solver.py:986-1032): 27 particles
+    # per model, codes 11 + model, 36 preview rows at stride 3 (limit 50), and a packet the
+    # Python decoder reads with the worker's checks.
+    import base64
+
+    from mpm_amd import wire
+    snap = got["snapshot"]
+    st = snap["state"]
+    assert st["models"][:4] == [27, 27, 27, 27] and sum(st["models"]) == 108
+    assert (st["count"], st["renderCount"], st["sampleStride"]) == (108, 36, 3)
+    data = np.array(snap["data"], np.float32).reshape(-1, 8)
+    ids = np.array([r * 3 + o.lib().orc_preview_pick(r, 3, 108) for r in range(36)])
+    np.testing.assert_array_equal(data[:, 3], 11 + models[ids])
+    np.testing.assert_allclose(data[:, :3], orc.positions()[ids], atol=.003, rtol=.001)
+    meta, rows = wire.decode(base64.b64decode(snap["packet"]))
+    assert meta == st and np.array_equal(rows, data)
