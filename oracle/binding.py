@@ -483,3 +483,90 @@ def mlp_loss_grad(params, x, t):
     g = np.zeros(len(params))
     loss = lib().orc_mlp_loss_grad(_p(params), len(x), _p(x), _p(t), _p(g))
     return loss, g
+
+
+# ------------------------------------------------------------------ 2D solid
+PF = C.POINTER(C.c_float)
+
+
+class OrcScene2(C.Structure):
+    _fields_ = [("count", C.c_int), ("x", PF), ("y", PF), ("vx", PF), ("vy", PF), ("mass", PF), ("volume", PF),
+                ("pinned", PU8), ("F", PD), ("C", PD), ("stiffness", D), ("walls", C.c_int), ("bound", D),
+                ("floor", D), ("top", D), ("active_nodes", C.c_int)]
+
+
+def _lib2():
+    L = lib()
+    if not getattr(L, "_orc2", False):
+        L.orc2_solid_stress.argtypes = [D, D, D, D, D, D, PD]
+        L.orc2_stable_step.restype = D
+        L.orc2_stable_step.argtypes = [C.POINTER(OrcScene2)]
+        L.orc2_step.argtypes = [C.POINTER(OrcScene2), D]
+        L.orc2_stats.argtypes = [C.POINTER(OrcScene2), PD]
+        L.orc2_advance.restype = C.c_int
+        L.orc2_advance.argtypes = [C.POINTER(OrcScene2), D, C.c_int, D, PD]
+        L._orc2 = True
+    return L
+
+
+def solid_stress2(a, b, c, d, mu, lam):
+    out = np.zeros(4)
+    _lib2().orc2_solid_stress(a, b, c, d, mu, lam, _p(out))
+    return out
+
+
+# Wall boxes of the 2D driver: softbody demo, basin, and every other scene
+# (src/solid-mpm.js:69-70).
+WALLS2 = dict(softbody=(170, -145, 145), basin=(245, -175, 245), default=(555, -555, 555))
+
+
+class Scene2:
+    """2D scene of MPM solid particles stepped by the oracle's SolidMPM
+    restatement; x, y, vx, vy, mass and volume are float32 as in the
+    reference's 2D driver, F and C float64."""
+
+    def __init__(self, x, v=(0, 0), mass=1.1, volume=1.0, pinned=None, stiffness=1.0, walls=None):
+        x = np.asarray(x, np.float64).reshape(-1, 2)
+        n = len(x)
+        self.n = n
+        f32 = lambda a: np.array(np.broadcast_to(np.asarray(a, np.float32), (n,)))  # noqa: E731  (own, writable)
+        self.x, self.y = f32(x[:, 0]), f32(x[:, 1])
+        v = np.broadcast_to(np.asarray(v, np.float64), (n, 2))
+        self.vx, self.vy = f32(v[:, 0]), f32(v[:, 1])
+        self.mass, self.volume = f32(mass), f32(volume)
+        self.pinned = np.zeros(n, np.uint8) if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+        self.F = np.tile([1.0, 0.0, 0.0, 1.0], n)
+        self.C = np.zeros(4 * n)
+        s = OrcScene2()
+        s.count = n
+        for k in ("x", "y", "vx", "vy", "mass", "volume"):
+            setattr(s, k, _p(getattr(self, k), PF))
+        s.pinned = _p(self.pinned, PU8)
+        s.F, s.C = _p(self.F), _p(self.C)
+        s.stiffness = stiffness
+        s.walls = 0 if walls is None else 1
+        if walls is not None:
+            s.bound, s.floor, s.top = WALLS2[walls] if isinstance(walls, str) else walls
+        self.s = s
+
+    def step(self, dt):
+        _lib2().orc2_step(C.byref(self.s), dt)
+
+    def stable_step(self):
+        return _lib2().orc2_stable_step(C.byref(self.s))
+
+    def stats(self):
+        out = np.zeros(4)
+        _lib2().orc2_stats(C.byref(self.s), _p(out))
+        return dict(count=int(out[0]), minJ=out[1], maxJ=out[2], strain=out[3], nodes=self.s.active_nodes)
+
+    def advance(self, dt=1 / 60, gravity=0, strength=1.0):
+        adv = D(0)
+        n = _lib2().orc2_advance(C.byref(self.s), dt, gravity, strength, C.byref(adv))
+        return n, adv.value
+
+    def positions(self):
+        return np.stack([self.x, self.y], 1).astype(np.float64)
+
+    def velocities(self):
+        return np.stack([self.vx, self.vy], 1).astype(np.float64)
