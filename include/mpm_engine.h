@@ -528,6 +528,71 @@ int mpm_batch_set_tuning(mpm_batch* b, const mpm_batch_tuning* t);
  * resident scenes (which it advances), and keeps the fastest. */
 int mpm_batch_tune(mpm_batch* b, int32_t trial_steps, mpm_batch_tuning* out);
 
+/* ---- 2D elastic MLS-MPM (the reference's SolidMPM) ----
+ *
+ * The reference's 2D solver for its MPM solid material (src/solid-mpm.js):
+ * quadratic MLS-MPM/APIC with the 2D fixed-corotated law (mu = lambda =
+ * 0.4 E, E = m / (12.96 V) 16000 stiffness) on a fixed grid of 149 x 149
+ * nodes at cell 8 from -592 (src/solid-mpm.js:4). Particle state stays in
+ * the caller's order. */
+typedef struct mpm2_engine mpm2_engine;
+
+/* Host view of 2D particles (structure of arrays, FP64 or FP32 as dtype). F
+ * and C hold 4 values per particle: 00, 01, 10, 11 (SolidMPM's f00..c11). On
+ * download NULL pointers are skipped. */
+typedef struct {
+  uint32_t count;
+  int32_t dtype;
+  void *x, *y, *vx, *vy;   /* [count] */
+  void *F, *C;             /* [4 count] */
+  void *mass, *volume;     /* [count] (the reference's s.mass, s.volume) */
+  uint8_t* pinned;         /* [count] */
+} mpm2_particles;
+
+/* The settings the 2D step reads: settings.solidStiffness, and the wall box
+ * that applies when settings.boundary is on or the scene is the softbody
+ * demo or the basin: |x| < bound, floor < y < top (src/solid-mpm.js:69-80). */
+typedef struct {
+  float stiffness;
+  int32_t walls;
+  float bound, floor, top;
+} mpm2_settings;
+
+/* SolidMPM.stats (src/solid-mpm.js:106-115) plus the exact mass totals. */
+typedef struct {
+  uint32_t count;
+  double min_J, max_J;     /* 1 and 1 without particles */
+  double strain;           /* largest Green strain norm */
+  uint32_t nodes;          /* grid nodes with mass in the last step */
+  uint32_t substeps;       /* substeps of the last step call or frame */
+  int64_t grid_mass_q;     /* fixed-point node mass of the last step */
+  int64_t particle_mass_q; /* fixed-point particle total; equal to grid_mass_q */
+} mpm2_stats;
+
+/* new SolidMPM(capacity). */
+int mpm2_create(uint32_t capacity, mpm2_engine** out);
+int mpm2_destroy(mpm2_engine* e);
+const char* mpm2_last_error(const mpm2_engine* e);
+/* Settings of a scene: "softbody", "basin" or any other name (NULL) for the
+ * 555 box; stiffness 1 and walls on (src/simulation.js:26). Host only. */
+int mpm2_settings_defaults(const char* scene, mpm2_settings* out);
+int mpm2_upload(mpm2_engine* e, const mpm2_particles* p);
+int mpm2_download(mpm2_engine* e, mpm2_particles* p);
+/* nsteps SolidMPM.step(s, dt) calls (src/solid-mpm.js:48-92), asynchronous. */
+int mpm2_step(mpm2_engine* e, const mpm2_settings* s, double dt, int32_t nsteps);
+/* SolidMPM.stableStep(s) (src/solid-mpm.js:38-47). */
+int mpm2_stable_dt(mpm2_engine* e, const mpm2_settings* s, double* out);
+/* One frame of the 2D driver for a scene of MPM solids only
+ * (src/simulation.js:240-300): dt clamped to [0, .05]; substeps of
+ * min(remaining, stableStep) while time remains, at most max_substeps (the
+ * reference's 96); each adds gravity (0 none, 1 uniform, 2 centre) times 27
+ * strength to unpinned velocities, zeroes pinned ones, clamps to +-180 and
+ * steps. The loop runs on the device. */
+int mpm2_advance(mpm2_engine* e, const mpm2_settings* s, double dt, int32_t gravity_mode, float strength,
+                 int32_t max_substeps, int32_t* substeps, double* advanced);
+int mpm2_statistics(mpm2_engine* e, mpm2_stats* out);
+int mpm2_synchronize(mpm2_engine* e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,23 @@ class MpmSnapshotStats(C.Structure):
                 ("nonfinite", C.c_int64)]
 
 
+class Mpm2Particles(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("dtype", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("vx", C.c_void_p), ("vy", C.c_void_p), ("F", C.c_void_p), ("C", C.c_void_p),
+                ("mass", C.c_void_p), ("volume", C.c_void_p), ("pinned", C.c_void_p)]
+
+
+class Mpm2Settings(C.Structure):
+    _fields_ = [("stiffness", C.c_float), ("walls", C.c_int32), ("bound", C.c_float), ("floor", C.c_float),
+                ("top", C.c_float)]
+
+
+class Mpm2Stats(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("min_J", C.c_double), ("max_J", C.c_double), ("strain", C.c_double),
+                ("nodes", C.c_uint32), ("substeps", C.c_uint32), ("grid_mass_q", C.c_int64),
+                ("particle_mass_q", C.c_int64)]
+
+
 class MpmConservation(C.Structure):
     _fields_ = [
         ("particle_mass", C.c_double), ("grid_mass", C.c_double),
@@ -135,6 +152,18 @@ SIGNATURES = {
     "mpm_conservation_report": (C.c_int, [_P, C.POINTER(MpmConservation)]),
     "mpm_refresh_diagnostics": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mpm_preview_layout": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mpm2_create": (C.c_int, [C.c_uint32, C.POINTER(_P)]),
+    "mpm2_destroy": (C.c_int, [_P]),
+    "mpm2_last_error": (C.c_char_p, [_P]),
+    "mpm2_settings_defaults": (C.c_int, [C.c_char_p, C.POINTER(Mpm2Settings)]),
+    "mpm2_upload": (C.c_int, [_P, C.POINTER(Mpm2Particles)]),
+    "mpm2_download": (C.c_int, [_P, C.POINTER(Mpm2Particles)]),
+    "mpm2_step": (C.c_int, [_P, C.POINTER(Mpm2Settings), C.c_double, C.c_int32]),
+    "mpm2_stable_dt": (C.c_int, [_P, C.POINTER(Mpm2Settings), C.POINTER(C.c_double)]),
+    "mpm2_advance": (C.c_int, [_P, C.POINTER(Mpm2Settings), C.c_double, C.c_int32, C.c_float, C.c_int32,
+                               C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "mpm2_statistics": (C.c_int, [_P, C.POINTER(Mpm2Stats)]),
+    "mpm2_synchronize": (C.c_int, [_P]),
     "mpm_snapshot": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_float),
                                C.POINTER(MpmSnapshotStats)]),
     "mpm_frame_packet": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,

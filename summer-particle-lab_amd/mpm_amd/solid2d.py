@@ -45,3 +45,97 @@ def panel(x1, y1, x2, y2, shape="panel", fixed=False):
             pts.append((x, y))
             pins.append(1 if fixed and col < 2 else 0)
     return np.array(pts, np.float64), np.array(pins, np.uint8)
+
+
+def settings(scene=None, stiffness=None, walls=None):
+    """mpm2_settings of a scene (softbody, basin, other), optionally with
+    another stiffness or walls on/off (the reference's settings.boundary)."""
+    s = L.Mpm2Settings()
+    L.check(L.load().mpm2_settings_defaults(None if scene is None else scene.encode(), C.byref(s)))
+    if stiffness is not None:
+        s.stiffness = stiffness
+    if walls is not None:
+        s.walls = int(bool(walls))
+    return s
+
+
+class Solid2D:
+    """The reference's SolidMPM (src/solid-mpm.js) on the device: particles
+    in the caller's order, F and C as 4 values per particle (00, 01, 10,
+    11)."""
+
+    def __init__(self, capacity):
+        self._lib = L.load()
+        h = C.c_void_p()
+        L.check(self._lib.mpm2_create(int(capacity), C.byref(h)))
+        self.h = h
+        self.capacity = int(capacity)
+        self.count = 0
+
+    def close(self):
+        if self.h:
+            self._lib.mpm2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != L.MPM_OK:
+            raise L.MpmError(rc, self._lib.mpm2_last_error(self.h).decode())
+
+    def upload(self, x, v=(0, 0), mass=1.1, volume=1.0, F=None, Cmat=None, pinned=None):
+        x = np.asarray(x, np.float64).reshape(-1, 2)
+        n = len(x)
+        col = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))  # noqa: E731
+        v = np.broadcast_to(np.asarray(v, np.float64), (n, 2))
+        keep = dict(x=col(x[:, 0]), y=col(x[:, 1]), vx=col(v[:, 0]), vy=col(v[:, 1]),
+                    F=np.ascontiguousarray(np.tile([1.0, 0, 0, 1], n) if F is None else np.asarray(F, np.float64)
+                                           .reshape(4 * n)),
+                    C=np.ascontiguousarray(np.zeros(4 * n) if Cmat is None else np.asarray(Cmat, np.float64)
+                                           .reshape(4 * n)),
+                    mass=col(mass), volume=col(volume),
+                    pinned=np.ascontiguousarray(np.broadcast_to(np.asarray(0 if pinned is None else pinned, np.uint8),
+                                                                (n,))))
+        p = L.Mpm2Particles()
+        p.count, p.dtype = n, L.MPM_F64
+        for k, a in keep.items():
+            setattr(p, k, a.ctypes.data_as(C.c_void_p))
+        self._check(self._lib.mpm2_upload(self.h, C.byref(p)))
+        self.count = n
+
+    def download(self):
+        n = self.count
+        out = dict(x=np.zeros(n), y=np.zeros(n), vx=np.zeros(n), vy=np.zeros(n), F=np.zeros(4 * n),
+                   C=np.zeros(4 * n), mass=np.zeros(n), volume=np.zeros(n))
+        p = L.Mpm2Particles()
+        p.count, p.dtype = n, L.MPM_F64
+        for k, a in out.items():
+            setattr(p, k, a.ctypes.data_as(C.c_void_p))
+        self._check(self._lib.mpm2_download(self.h, C.byref(p)))
+        out["positions"] = np.stack([out["x"], out["y"]], 1)
+        out["velocities"] = np.stack([out["vx"], out["vy"]], 1)
+        return out
+
+    def step(self, dt, nsteps=1, s=None):
+        self._check(self._lib.mpm2_step(self.h, C.byref(s or settings(walls=False)), float(dt), int(nsteps)))
+
+    def stable_step(self, s=None):
+        out = C.c_double()
+        self._check(self._lib.mpm2_stable_dt(self.h, C.byref(s or settings(walls=False)), C.byref(out)))
+        return out.value
+
+    def advance(self, dt=1 / 60, gravity="none", strength=1.0, s=None, max_substeps=96):
+        n, adv = C.c_int32(), C.c_double()
+        g = GRAVITY[gravity] if isinstance(gravity, str) else int(gravity)
+        self._check(self._lib.mpm2_advance(self.h, C.byref(s or settings(walls=False)), float(dt), g, float(strength),
+                                           int(max_substeps), C.byref(n), C.byref(adv)))
+        return n.value, adv.value
+
+    def stats(self):
+        st = L.Mpm2Stats()
+        self._check(self._lib.mpm2_statistics(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in L.Mpm2Stats._fields_}

@@ -251,4 +251,6 @@ class MPM3 {
   destroy() { if (this.engine) native.destroy(this.engine); this.engine = null; }
 }
 
-module.exports = {MPM3, DEFAULT_DOMAIN, native};
+const {SolidMPM, solidStress, isMPMSolid} = require('./solid_mpm.js');
+
+module.exports = {MPM3, DEFAULT_DOMAIN, native, SolidMPM, solidStress, isMPMSolid};

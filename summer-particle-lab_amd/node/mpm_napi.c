@@ -522,6 +522,239 @@ static napi_value js_frame_packet(napi_env env, napi_callback_info info) {
   return ab;
 }
 
+/* ------------------------------------------------------------------ 2D */
+static void finalize_engine2(napi_env env, void* data, void* hint) {
+  (void)env;
+  (void)hint;
+  mpm2_engine** slot = (mpm2_engine**)data;
+  if (*slot) mpm2_destroy(*slot);
+  free(slot);
+}
+
+static mpm2_engine* get_engine2(napi_env env, napi_value v) {
+  void* data = NULL;
+  if (napi_get_value_external(env, v, &data) != napi_ok || !data) {
+    napi_throw_type_error(env, NULL, "expected a 2D engine handle");
+    return NULL;
+  }
+  mpm2_engine* e = *(mpm2_engine**)data;
+  if (!e) napi_throw_error(env, NULL, "engine was destroyed");
+  return e;
+}
+
+static napi_value throw_mpm2(napi_env env, int rc, mpm2_engine* e) {
+  char code[16], msg[600];
+  snprintf(code, sizeof code, "MPM%d", rc);
+  snprintf(msg, sizeof msg, "mpm error %d: %s", rc, e ? mpm2_last_error(e) : "");
+  napi_throw_error(env, code, msg);
+  return NULL;
+}
+
+/* create2(capacity) */
+static napi_value js_create2(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  double cap = 0;
+  if (argc < 1 || napi_get_value_double(env, argv[0], &cap) != napi_ok || !(cap >= 1) || cap > 4294967295.0) {
+    napi_throw_range_error(env, NULL, "capacity must be a positive integer");
+    return NULL;
+  }
+  mpm2_engine* e = NULL;
+  int rc = mpm2_create((uint32_t)cap, &e);
+  if (rc) return throw_mpm2(env, rc, NULL);
+  mpm2_engine** slot = (mpm2_engine**)malloc(sizeof *slot);
+  *slot = e;
+  napi_value ext;
+  CHECK_NAPI(env, napi_create_external(env, slot, finalize_engine2, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_destroy2(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  void* data = NULL;
+  if (argc < 1 || napi_get_value_external(env, argv[0], &data) != napi_ok || !data) return NULL;
+  mpm2_engine** slot = (mpm2_engine**)data;
+  if (*slot) mpm2_destroy(*slot);
+  *slot = NULL;
+  return NULL;
+}
+
+/* {count, x, y, vx, vy, F, C, mass, volume, pinned}: floats all Float64Array
+ * or all Float32Array (F and C 4 per particle), pinned a Uint8Array. */
+static int particle_view2(napi_env env, napi_value obj, mpm2_particles* p) {
+  memset(p, 0, sizeof *p);
+  p->count = (uint32_t)get_num(env, obj, "count", 0);
+  p->dtype = -1;
+  struct {
+    const char* key;
+    void** dst;
+    int per, flag;
+  } fields[] = {{"x", &p->x, 1, 0},       {"y", &p->y, 1, 0},       {"vx", &p->vx, 1, 0},
+                {"vy", &p->vy, 1, 0},     {"F", &p->F, 4, 0},       {"C", &p->C, 4, 0},
+                {"mass", &p->mass, 1, 0}, {"volume", &p->volume, 1, 0}, {"pinned", (void**)&p->pinned, 1, 1}};
+  for (size_t k = 0; k < sizeof fields / sizeof fields[0]; ++k) {
+    bool has = false, is_ta = false;
+    napi_value v, ab;
+    if (napi_has_named_property(env, obj, fields[k].key, &has) != napi_ok || !has) continue;
+    napi_get_named_property(env, obj, fields[k].key, &v);
+    napi_is_typedarray(env, v, &is_ta);
+    if (!is_ta) continue;
+    napi_typedarray_type type;
+    size_t length, offset;
+    void* data;
+    napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &offset);
+    if (length < (size_t)p->count * fields[k].per) {
+      napi_throw_range_error(env, NULL, fields[k].key);
+      return -1;
+    }
+    if (fields[k].flag) {
+      if (type != napi_uint8_array) {
+        napi_throw_type_error(env, NULL, "pinned must be a Uint8Array");
+        return -1;
+      }
+    } else {
+      const int dt = type == napi_float64_array ? MPM_F64 : type == napi_float32_array ? MPM_F32 : -2;
+      if (dt < 0 || (p->dtype >= 0 && p->dtype != dt)) {
+        napi_throw_type_error(env, NULL, "floating arrays must all be Float64Array or all Float32Array");
+        return -1;
+      }
+      p->dtype = dt;
+    }
+    *fields[k].dst = data;
+  }
+  if (p->dtype < 0) p->dtype = MPM_F64;
+  return 0;
+}
+
+/* settings object {stiffness, walls, bound, floor, top} */
+static void settings2(napi_env env, napi_value obj, mpm2_settings* s) {
+  mpm2_settings_defaults(NULL, s);
+  s->stiffness = (float)get_num(env, obj, "stiffness", s->stiffness);
+  s->walls = (int32_t)get_num(env, obj, "walls", s->walls);
+  s->bound = (float)get_num(env, obj, "bound", s->bound);
+  s->floor = (float)get_num(env, obj, "floor", s->floor);
+  s->top = (float)get_num(env, obj, "top", s->top);
+}
+
+static napi_value js_upload2(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm2_engine* e = argc >= 2 ? get_engine2(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm2_particles p;
+  if (particle_view2(env, argv[1], &p)) return NULL;
+  int rc = mpm2_upload(e, &p);
+  if (rc) return throw_mpm2(env, rc, e);
+  return NULL;
+}
+
+static napi_value js_download2(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm2_engine* e = argc >= 2 ? get_engine2(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm2_particles p;
+  if (particle_view2(env, argv[1], &p)) return NULL;
+  int rc = mpm2_download(e, &p);
+  if (rc) return throw_mpm2(env, rc, e);
+  return NULL;
+}
+
+/* step2(h, settings, dt, n) */
+static napi_value js_step2(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm2_engine* e = argc >= 3 ? get_engine2(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm2_settings s;
+  settings2(env, argv[1], &s);
+  double dt = 0;
+  int32_t n = 1;
+  napi_get_value_double(env, argv[2], &dt);
+  if (argc >= 4) napi_get_value_int32(env, argv[3], &n);
+  int rc = mpm2_step(e, &s, dt, n);
+  if (!rc) rc = mpm2_synchronize(e);
+  if (rc) return throw_mpm2(env, rc, e);
+  return NULL;
+}
+
+/* stableDt2(h, settings) */
+static napi_value js_stable_dt2(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm2_engine* e = argc >= 2 ? get_engine2(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm2_settings s;
+  settings2(env, argv[1], &s);
+  double out = 0;
+  int rc = mpm2_stable_dt(e, &s, &out);
+  if (rc) return throw_mpm2(env, rc, e);
+  napi_value v;
+  napi_create_double(env, out, &v);
+  return v;
+}
+
+/* advance2(h, settings, dt, gravityMode, strength, maxSubsteps) -> {substeps, advanced} */
+static napi_value js_advance2(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm2_engine* e = argc >= 3 ? get_engine2(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm2_settings s;
+  settings2(env, argv[1], &s);
+  double dt = 0, strength = 1;
+  int32_t mode = 0, maxs = 96;
+  napi_get_value_double(env, argv[2], &dt);
+  if (argc >= 4) napi_get_value_int32(env, argv[3], &mode);
+  if (argc >= 5) napi_get_value_double(env, argv[4], &strength);
+  if (argc >= 6) napi_get_value_int32(env, argv[5], &maxs);
+  int32_t n = 0;
+  double adv = 0;
+  int rc = mpm2_advance(e, &s, dt, mode, (float)strength, maxs, &n, &adv);
+  if (rc) return throw_mpm2(env, rc, e);
+  napi_value o;
+  CHECK_NAPI(env, napi_create_object(env, &o));
+  set_num(env, o, "substeps", n);
+  set_num(env, o, "advanced", adv);
+  return o;
+}
+
+/* stats2(h) -> {count, minJ, maxJ, strain, nodes, substeps, gridMassQ, particleMassQ} */
+static napi_value js_stats2(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm2_engine* e = argc >= 1 ? get_engine2(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm2_stats st;
+  int rc = mpm2_statistics(e, &st);
+  if (rc) return throw_mpm2(env, rc, e);
+  napi_value o, b;
+  CHECK_NAPI(env, napi_create_object(env, &o));
+  set_num(env, o, "count", st.count);
+  set_num(env, o, "minJ", st.min_J);
+  set_num(env, o, "maxJ", st.max_J);
+  set_num(env, o, "strain", st.strain);
+  set_num(env, o, "nodes", st.nodes);
+  set_num(env, o, "substeps", st.substeps);
+  char buf[32];
+  snprintf(buf, sizeof buf, "%lld", (long long)st.grid_mass_q);
+  napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &b);
+  napi_set_named_property(env, o, "gridMassQ", b);
+  snprintf(buf, sizeof buf, "%lld", (long long)st.particle_mass_q);
+  napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &b);
+  napi_set_named_property(env, o, "particleMassQ", b);
+  return o;
+}
+
 static napi_value js_refresh_diagnostics(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -577,6 +810,14 @@ static napi_value init(napi_env env, napi_value exports) {
              {"refreshDiagnostics", js_refresh_diagnostics},
              {"snapshot", js_snapshot},
              {"framePacket", js_frame_packet},
+             {"create2", js_create2},
+             {"destroy2", js_destroy2},
+             {"upload2", js_upload2},
+             {"download2", js_download2},
+             {"step2", js_step2},
+             {"stableDt2", js_stable_dt2},
+             {"advance2", js_advance2},
+             {"stats2", js_stats2},
              {"lastError", js_last_error}};
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; ++i) {
     napi_value f;

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "mpm_engine.h")
 def header_symbols():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(mpm_[a-z0-9_]+)\s*\(", text))
+    return set(re.findall(r"\b(mpm2?_[a-z0-9_]+)\s*\(", text))
 
 
 def test_library_exports_every_declared_symbol():
@@ -32,7 +32,9 @@ def test_struct_layout_matches_c(tmp_path):
     structs = {"mpm_config": L.MpmConfig, "mpm_material": L.MpmMaterial, "mpm_particles": L.MpmParticles,
                "mpm_conservation": L.MpmConservation, "mpm_kernel_time": L.MpmKernelTime,
                "mpm_mlp_train_config": L.MpmMlpTrainConfig, "mpm_mlp_norm": L.MpmMlpNorm,
-               "mpm_batch_options": L.MpmBatchOptions, "mpm_batch_tuning": L.MpmBatchTuning}
+               "mpm_batch_options": L.MpmBatchOptions, "mpm_batch_tuning": L.MpmBatchTuning,
+               "mpm_snapshot_stats": L.MpmSnapshotStats, "mpm2_particles": L.Mpm2Particles,
+               "mpm2_settings": L.Mpm2Settings, "mpm2_stats": L.Mpm2Stats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_engine.h"', "int main(void){"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -38,7 +38,8 @@ console.log(JSON.stringify({{names, code, typeErr}}));
     out = json.loads(r.stdout)
     assert out["names"] == sorted(["create", "destroy", "setMaterials", "upload", "download", "step", "synchronize",
                                    "stableDt", "conservation", "refreshDiagnostics", "lastError", "advanceFrame",
-                                   "snapshot", "framePacket"])
+                                   "snapshot", "framePacket", "create2", "destroy2", "upload2", "download2",
+                                   "step2", "stableDt2", "advance2", "stats2"])
     assert out["code"] == "MPM1"
     assert "engine handle" in out["typeErr"]
 
@@ -182,3 +183,66 @@ def test_drop_in_reference_parity_scene(gpu, tmp_path):
     np.testing.assert_allclose(data[:, :3], orc.positions()[ids], atol=.003, rtol=.001)
     meta, rows = wire.decode(base64.b64decode(snap["packet"]))
     assert meta == st and np.array_equal(rows, data)
+
+
+def test_solid_stress_js_matches_the_oracle():
+    """The JS closed-form 2D stress of the drop-in (src/solid-mpm.js:10-18)
+    against the oracle's, including the reference's test cases."""
+    cases = [(1, 0, 0, 1, 400, 400), (.7648421873, -.6442176872, .6442176872, .7648421873, 400, 400),
+             (1, .3, 0, 1, 400, 400), (1.2, 0, 0, 1, 400, 400), (.9, .2, -.1, 1.1, 3, 5), (1e-13, 0, 0, 1e-13, 1, 1)]
+    r = node(f"""
+const {{solidStress}} = require({json.dumps(os.path.join(NODE_DIR, 'index.js'))});
+console.log(JSON.stringify({json.dumps(cases)}.map((c) => solidStress(...c))));
+""")
+    assert r.returncode == 0, r.stderr
+    got = np.array(json.loads(r.stdout))
+    for k, c in enumerate(cases):
+        np.testing.assert_allclose(got[k], o.solid_stress2(*c), rtol=1e-14, atol=1e-12)
+
+
+def run_solid2d(tmp_path, inp):
+    path = tmp_path / "solid2d.json"
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "solid2d_scene.js"), str(path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return json.loads(r.stdout)
+
+
+@pytest.mark.gpu
+def test_drop_in_solid_mpm_softbody_frames_match_oracle(gpu, tmp_path):
+    """SolidMPM.advance on the softbody demo's two bodies (a fixed cantilever
+    and a free panel, src/simulation.js:66-67), centre gravity, 20 frames:
+    the same substep counts as the oracle's driver, positions within the
+    reference's bar, pins in place."""
+    from mpm_amd.solid2d import panel
+    panels = [dict(x1=-120, y1=70, x2=-45, y2=98, fixed=True), dict(x1=30, y1=10, x2=90, y2=70)]
+    got = run_solid2d(tmp_path, dict(panels=panels, scene="softbody", frames=20, gravity="center", mode="frames"))
+    assert got["created"] == [True, True]
+    parts = [panel(**p) for p in panels]
+    pts = np.concatenate([p[0] for p in parts])
+    pins = np.concatenate([p[1] for p in parts])
+    assert got["pinned"] == pins.tolist()
+    sc = o.Scene2(pts, pinned=pins, walls="softbody")
+    counts = [sc.advance(1 / 60, gravity=2)[0] for _ in range(20)]
+    assert got["frames"] == counts
+    x = np.stack([got["x"], got["y"]], 1)
+    np.testing.assert_allclose(x, sc.positions(), atol=.003, rtol=.001)
+    np.testing.assert_array_equal(x[pins == 1], pts[pins == 1].astype(np.float32))
+    st = got["stats"]
+    assert st["count"] == len(pts) and st["minJ"] == pytest.approx(sc.stats()["minJ"], abs=1e-4)
+
+
+@pytest.mark.gpu
+def test_drop_in_solid_mpm_host_loop_matches_oracle(gpu, tmp_path):
+    """The reference driver's own loop of stableStep and step from JS
+    (src/simulation.js:246-254) on a free panel without gravity or walls:
+    the oracle's substep counts and positions within the reference's bar."""
+    from mpm_amd.solid2d import panel
+    panels = [dict(x1=-20, y1=-10, x2=20, y2=10)]
+    got = run_solid2d(tmp_path, dict(panels=panels, frames=3, mode="steps"))
+    pts, _ = panel(**panels[0])
+    sc = o.Scene2(pts)
+    counts = [sc.advance(1 / 60, gravity=0)[0] for _ in range(3)]
+    assert got["frames"] == counts
+    np.testing.assert_allclose(np.stack([got["x"], got["y"]], 1), sc.positions(), atol=.003, rtol=.001)
