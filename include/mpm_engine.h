@@ -227,6 +227,33 @@ int mpm_stable_dt(mpm_engine* e, double cap, double cfl, double* out);
  * min_J replaces MPM3.minJ(s) (src/three/mpm.js:135). Synchronising. */
 int mpm_conservation_report(mpm_engine* e, mpm_conservation* out);
 
+/* ---- split step: the grid between its update and G2P ----
+ *
+ * The reference lets its callers act on the grid inside MPM3.step: the
+ * externalGridContact hook changes node velocities after the grid update
+ * (src/three/mpm.js:104) and contactGridDeltas adds a per-part velocity to
+ * every node a particle of that part gathers from (:114). A split step is
+ * mpm_step_to_grid, any number of grid reads and velocity writes, then
+ * mpm_step_from_grid with the same dt. mpm_step, mpm_advance_frame and a
+ * second mpm_step_to_grid refuse (MPM_ERR_NOT_READY) while one is pending;
+ * mpm_upload drops it. */
+
+/* Stress, P2G and the grid update of one step (src/three/mpm.js:61-102). */
+int mpm_step_to_grid(mpm_engine* e, double dt);
+/* The grid as the reference's MPM3 holds it: mass [N^3], v [3 N^3]
+ * (node-major x, y, z) and fixed [N^3] (1 where a pinned particle holds the
+ * node), node index (z N + y) N + x; nodes no particle reached hold 0. Any
+ * pointer may be NULL. */
+int mpm_grid_download(mpm_engine* e, double* mass, double* v, uint8_t* fixed);
+/* Writes node velocities v [3 N^3] back for the nodes particles reached. */
+int mpm_grid_upload_velocity(mpm_engine* e, const double* v);
+/* G2P (plus per-part deltas), F update, advection and the plastic return of
+ * the pending step (src/three/mpm.js:107-131). deltas holds nslots arrays of
+ * [3 N^3] velocities; material_slot [nmat] gives each material's slot or -1.
+ * nslots = 0 means no deltas. */
+int mpm_step_from_grid(mpm_engine* e, double dt, const int32_t* material_slot, int32_t nmat, const double* deltas,
+                       int32_t nslots);
+
 /* ---- frame snapshot and wire packet (the GPU service protocol) ----
  *
  * The reference's GPU service sends one binary packet per frame

@@ -138,6 +138,10 @@ def lib():
         L.orc_apply_particle_gravity.argtypes = [C.POINTER(OrcScene), PD, D]
         L.orc_step.restype = C.c_int
         L.orc_step.argtypes = [C.POINTER(OrcScene), D]
+        L.orc_step_to_grid.restype = C.c_int
+        L.orc_step_to_grid.argtypes = [C.POINTER(OrcScene), D]
+        L.orc_step_from_grid.restype = C.c_int
+        L.orc_step_from_grid.argtypes = [C.POINTER(OrcScene), D, C.POINTER(PD)]
         L.orc_p2g.argtypes = [C.POINTER(OrcScene), D]
         L.orc_min_j.restype = D
         L.orc_min_j.argtypes = [C.POINTER(OrcScene)]
@@ -407,6 +411,23 @@ class Scene:
             lib().orc_apply_particle_gravity(C.byref(self.s), _p(g), dt)
         rc = lib().orc_step(C.byref(self.s), dt)
         if rc != 0:
+            raise ValueError("unsupported model or part index in scene")
+
+    def step_to_grid(self, dt):
+        """Stress, P2G and grid update; self.gmass, self.gv, self.gfixed then
+        hold the grid (the externalGridContact hook's view)."""
+        if lib().orc_step_to_grid(C.byref(self.s), dt) != 0:
+            raise ValueError("unsupported model or part index in scene")
+
+    def step_from_grid(self, dt, deltas=None):
+        """G2P from the grid plus per-part velocity deltas ({part: [3 N^3]}),
+        then the plastic return."""
+        arr = None
+        if deltas:
+            self._deltas = {k: np.ascontiguousarray(v, np.float64).reshape(-1) for k, v in deltas.items()}
+            arr = (PD * self.s.nparts)(*[_p(self._deltas[k]) if k in self._deltas else None
+                                         for k in range(self.s.nparts)])
+        if lib().orc_step_from_grid(C.byref(self.s), dt, arr) != 0:
             raise ValueError("unsupported model or part index in scene")
 
     def advance_frame(self, dt, *, time_step_scale=1.0, max_substeps=48, velocity_limit=160.0, bounds=None,

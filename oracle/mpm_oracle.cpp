@@ -743,45 +743,53 @@ static bool parts_valid(const OrcScene* s) {
   return true;
 }
 
-int orc_step(const OrcScene* s, double dt) {
+// Quadratic B-spline stencil of particle p: base node, offsets and weights
+// per axis. src/three/mpm.js:25-32
+static void stencil3(const OrcScene* s, int p, int b[3], double f[3], double w[9]) {
+  const double H = s->H;
+  const int N = s->N;
+  for (int a = 0; a < 3; ++a) {
+    const double value = a == 0 ? s->x[p] : a == 1 ? s->y[p] : s->z[p];
+    const double q = (value - s->origin[a]) / H;
+    b[a] = (int)clampd(std::floor(q - .5), 0, N - 3);
+    const double t = q - b[a];
+    f[a] = t;
+    w[a * 3] = .5 * (1.5 - t) * (1.5 - t);
+    w[a * 3 + 1] = .75 - (t - 1) * (t - 1);
+    w[a * 3 + 2] = .5 * (t - .5) * (t - .5);
+  }
+}
+
+// First half of MPM3.step: this step's stress, P2G and the grid update
+// (src/three/mpm.js:61-102). The grid then holds node masses and the
+// velocities a caller may change before orc_step_from_grid (the
+// externalGridContact hook, :104).
+int orc_step_to_grid(const OrcScene* s, double dt) {
   if (!parts_valid(s)) return -1;
   const double H = s->H;
   const int N = s->N;
   const double* O = s->origin;
   const int n = s->count;
-  std::vector<int32_t> base(3 * (size_t)n);
-  std::vector<double> fraction(3 * (size_t)n), weights(9 * (size_t)n);
   for (int p = 0; p < n; ++p) orc_update_stress(s, p);  // :61
   const size_t nodes = (size_t)N * N * N;
   std::fill(s->gmass, s->gmass + nodes, 0.0);  // :64
   std::fill(s->gv, s->gv + 3 * nodes, 0.0);
   std::fill(s->gfixed, s->gfixed + nodes, (uint8_t)0);
   for (int p = 0; p < n; ++p) {  // P2G :66-85
-    for (int a = 0; a < 3; ++a) {  // stencil :25-32
-      const double value = a == 0 ? s->x[p] : a == 1 ? s->y[p] : s->z[p];
-      const double q = (value - O[a]) / H;
-      const int b = (int)clampd(std::floor(q - .5), 0, N - 3);
-      const double t = q - b;
-      base[p * 3 + a] = b;
-      fraction[p * 3 + a] = t;
-      weights[p * 9 + a * 3] = .5 * (1.5 - t) * (1.5 - t);
-      weights[p * 9 + a * 3 + 1] = .75 - (t - 1) * (t - 1);
-      weights[p * 9 + a * 3 + 2] = .5 * (t - .5) * (t - .5);
-    }
+    int b[3];
+    double f[3], w[9];
+    stencil3(s, p, b, f, w);
     const double m = s->mass[p], V = s->vol_scale * s->volume[p];
     double A[9];
     const double* C = s->C + p * 9;
     const double* tau = s->tau + p * 9;
     for (int k = 0; k < 9; ++k) A[k] = m * C[k] - dt * V * 4 / (H * H) * tau[k];
-    const int bx = base[p * 3], by = base[p * 3 + 1], bz = base[p * 3 + 2];
-    const double fx = fraction[p * 3], fy = fraction[p * 3 + 1], fz = fraction[p * 3 + 2];
-    const double* w = &weights[p * 9];
     for (int k = 0; k < 3; ++k)
       for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) {
-          const size_t node = ((size_t)(bz + k) * N + by + j) * N + bx + i;
+          const size_t node = ((size_t)(b[2] + k) * N + b[1] + j) * N + b[0] + i;
           const double weight = w[i] * w[3 + j] * w[6 + k];
-          const double x = (i - fx) * H, y = (j - fy) * H, z = (k - fz) * H;
+          const double x = (i - f[0]) * H, y = (j - f[1]) * H, z = (k - f[2]) * H;
           s->gmass[node] += weight * m;
           s->gv[node * 3] += weight * (m * s->vx[p] + A[0] * x + A[1] * y + A[2] * z);
           s->gv[node * 3 + 1] += weight * (m * s->vy[p] + A[3] * x + A[4] * y + A[5] * z);
@@ -809,20 +817,34 @@ int orc_step(const OrcScene* s, double dt) {
       s->gv[node * 3 + a] = v;
     }
   }
+  return 0;
+}
+
+// Second half of MPM3.step: G2P from the grid velocity plus the particle's
+// part delta (the contactGridDeltas hook, src/three/mpm.js:114; deltas may be
+// NULL, or hold one 3 N^3 array or NULL per part), F update, damage,
+// advection and the plastic return (:107-131). Positions must be those of
+// orc_step_to_grid.
+int orc_step_from_grid(const OrcScene* s, double dt, const double* const* deltas) {
+  const double H = s->H;
+  const int N = s->N;
+  const int n = s->count;
   for (int p = 0; p < n; ++p) {  // G2P :107-123
     if (s->pinned[p]) continue;
-    const int bx = base[p * 3], by = base[p * 3 + 1], bz = base[p * 3 + 2];
-    const double fx = fraction[p * 3], fy = fraction[p * 3 + 1], fz = fraction[p * 3 + 2];
-    const double* w = &weights[p * 9];
+    int b[3];
+    double f[3], w[9];
+    stencil3(s, p, b, f, w);
+    const double* delta = deltas ? deltas[s->part[p]] : nullptr;
     double vx = 0, vy = 0, vz = 0, C[9] = {0};
     for (int k = 0; k < 3; ++k)
       for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) {
-          const size_t node = ((size_t)(bz + k) * N + by + j) * N + bx + i;
+          const size_t node = ((size_t)(b[2] + k) * N + b[1] + j) * N + b[0] + i;
           const double weight = w[i] * w[3 + j] * w[6 + k];
-          const double dx = (i - fx) * 4 / H, dy = (j - fy) * 4 / H, dz = (k - fz) * 4 / H;
-          const double x = weight * s->gv[node * 3], y = weight * s->gv[node * 3 + 1],
-                       z = weight * s->gv[node * 3 + 2];
+          const double dx = (i - f[0]) * 4 / H, dy = (j - f[1]) * 4 / H, dz = (k - f[2]) * 4 / H;
+          const double x = weight * (s->gv[node * 3] + (delta ? delta[node * 3] : 0)),
+                       y = weight * (s->gv[node * 3 + 1] + (delta ? delta[node * 3 + 1] : 0)),
+                       z = weight * (s->gv[node * 3 + 2] + (delta ? delta[node * 3 + 2] : 0));
           vx += x;
           vy += y;
           vz += z;
@@ -833,10 +855,10 @@ int orc_step(const OrcScene* s, double dt) {
     double F[9];
     std::memcpy(F, s->F + p * 9, sizeof F);
     for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) {
-        double value = F[a * 3 + b];
-        for (int k = 0; k < 3; ++k) value += dt * C[a * 3 + k] * F[k * 3 + b];
-        s->F[p * 9 + a * 3 + b] = value;
+      for (int c = 0; c < 3; ++c) {
+        double value = F[a * 3 + c];
+        for (int k = 0; k < 3; ++k) value += dt * C[a * 3 + k] * F[k * 3 + c];
+        s->F[p * 9 + a * 3 + c] = value;
       }
     orc_update_damage(s, p);
     std::memcpy(s->C + p * 9, C, sizeof C);
@@ -864,6 +886,12 @@ int orc_step(const OrcScene* s, double dt) {
     s->cacheValid[p] = 1;
   }
   return 0;
+}
+
+// One MPM3.step. src/three/mpm.js:56-134
+int orc_step(const OrcScene* s, double dt) {
+  const int rc = orc_step_to_grid(s, dt);
+  return rc ? rc : orc_step_from_grid(s, dt, nullptr);
 }
 
 // P2G only (stress + scatter), leaving the grid holding mass and momentum;

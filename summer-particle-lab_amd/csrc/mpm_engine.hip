@@ -43,6 +43,9 @@ struct mpm_engine {
   float* frame_disp = nullptr;
   uint32_t* frame_cell_start = nullptr;
   unsigned int* frame_max = nullptr;
+  // split step (grid hooks): the grid of a step waits for mpm_step_from_grid
+  bool split_pending = false;
+  double split_dt = 0;
   // profiling
   bool profiling = false;
   std::vector<PendingTiming> pending;
@@ -451,6 +454,7 @@ int mpm_set_materials(mpm_engine* e, const mpm_material* mats, int32_t count) {
 }
 
 int mpm_upload(mpm_engine* e, const mpm_particles* p) {
+  if (e) e->split_pending = false;  // a new state drops a pending split step
   if (!e || !p) return MPM_ERR_INVALID_ARGUMENT;
   const uint32_t n = p->count;
   if (n > e->d.cap) return fail(e, MPM_ERR_CAPACITY, "count %u exceeds capacity %u", n, e->d.cap);
@@ -501,6 +505,7 @@ int mpm_download(mpm_engine* e, mpm_particles* p) {
 int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
   if (!e) return MPM_ERR_INVALID_ARGUMENT;
   if (!(dt > 0) || !std::isfinite(dt) || nsteps < 0) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt or nsteps");
+  if (e->split_pending) return fail(e, MPM_ERR_NOT_READY, "a split step waits for mpm_step_from_grid");
   if (nsteps == 0 || e->n == 0) return MPM_OK;
   const StepParams sp = params(e, dt);
   if (e->steps_since_sort >= e->cfg.sort_every) {
@@ -540,6 +545,129 @@ int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
     }
   }
   HIP_OK(e, hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_step_to_grid(mpm_engine* e, double dt) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt");
+  if (e->split_pending) return fail(e, MPM_ERR_NOT_READY, "a split step waits for mpm_step_from_grid");
+  if (e->n == 0) return MPM_OK;
+  const StepParams sp = params(e, dt);
+  if (e->steps_since_sort >= e->cfg.sort_every) {
+    int rc = rebin(e);
+    if (rc) return rc;
+  }
+  {
+    Timed t(e, "p2g", true);
+    launch_fused(e->stream, e->d, e->n, e->g, sp, 0, t.ev());
+  }
+  {
+    Timed t(e, "pieces", true);
+    launch_compact(e->stream, e->d, grid_pieces(e->g), 1, t.ev());
+  }
+  {
+    Timed t(e, "grid", true);
+    launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx, e->grid_blocks, t.ev());
+  }
+  HIP_OK(e, hipGetLastError());
+  e->split_pending = true;
+  e->split_dt = dt;
+  return MPM_OK;
+}
+
+int mpm_grid_download(mpm_engine* e, double* mass, double* v, uint8_t* fixed) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  if (!e->split_pending) return fail(e, MPM_ERR_NOT_READY, "no grid: call mpm_step_to_grid first");
+  const size_t Nn = (size_t)e->g.N * e->g.N * e->g.N;
+  float *dm = nullptr, *dv = nullptr;
+  uint8_t* df = nullptr;
+  hipError_t st = hipMallocAsync((void**)&dm, sizeof(float) * Nn, e->stream);
+  if (st == hipSuccess) st = hipMallocAsync((void**)&dv, sizeof(float) * 3 * Nn, e->stream);
+  if (st == hipSuccess) st = hipMallocAsync((void**)&df, Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(dm, 0, sizeof(float) * Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(dv, 0, sizeof(float) * 3 * Nn, e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(df, 0, Nn, e->stream);
+  if (st == hipSuccess) {
+    launch_grid_export(e->stream, e->d, e->g, dm, dv, df);
+    st = hipGetLastError();
+  }
+  std::vector<float> hm(mass ? Nn : 0), hv(v ? 3 * Nn : 0);
+  if (st == hipSuccess && mass) st = hipMemcpyAsync(hm.data(), dm, sizeof(float) * Nn, hipMemcpyDeviceToHost, e->stream);
+  if (st == hipSuccess && v) st = hipMemcpyAsync(hv.data(), dv, sizeof(float) * 3 * Nn, hipMemcpyDeviceToHost, e->stream);
+  if (st == hipSuccess && fixed) st = hipMemcpyAsync(fixed, df, Nn, hipMemcpyDeviceToHost, e->stream);
+  for (void* p : {(void*)dm, (void*)dv, (void*)df})
+    if (p) (void)hipFreeAsync(p, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "grid download: %s", hipGetErrorString(st));
+  for (size_t k = 0; k < hm.size(); ++k) mass[k] = hm[k];
+  for (size_t k = 0; k < hv.size(); ++k) v[k] = hv[k];
+  return MPM_OK;
+}
+
+int mpm_grid_upload_velocity(mpm_engine* e, const double* v) {
+  if (!e || !v) return MPM_ERR_INVALID_ARGUMENT;
+  if (!e->split_pending) return fail(e, MPM_ERR_NOT_READY, "no grid: call mpm_step_to_grid first");
+  const size_t Nn = (size_t)e->g.N * e->g.N * e->g.N;
+  std::vector<float> hv(3 * Nn);
+  for (size_t k = 0; k < hv.size(); ++k) hv[k] = (float)v[k];
+  float* dv = nullptr;
+  hipError_t st = hipMallocAsync((void**)&dv, sizeof(float) * 3 * Nn, e->stream);
+  if (st == hipSuccess) st = hipMemcpyAsync(dv, hv.data(), sizeof(float) * 3 * Nn, hipMemcpyHostToDevice, e->stream);
+  if (st == hipSuccess) {
+    launch_grid_import(e->stream, e->d, e->g, dv);
+    st = hipGetLastError();
+  }
+  if (dv) (void)hipFreeAsync(dv, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);  // hv must outlive the copy
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "grid upload: %s", hipGetErrorString(st));
+  return MPM_OK;
+}
+
+int mpm_step_from_grid(mpm_engine* e, double dt, const int32_t* material_slot, int32_t nmat, const double* deltas,
+                       int32_t nslots) {
+  if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  if (e->n == 0) return MPM_OK;
+  if (!e->split_pending) return fail(e, MPM_ERR_NOT_READY, "no grid: call mpm_step_to_grid first");
+  if (dt != e->split_dt) return fail(e, MPM_ERR_INVALID_ARGUMENT, "dt differs from mpm_step_to_grid's");
+  if (nslots < 0 || (nslots > 0 && (!deltas || !material_slot || nmat < (int32_t)e->materials.size())))
+    return fail(e, MPM_ERR_INVALID_ARGUMENT, "deltas need a slot for every material");
+  const size_t Nn = (size_t)e->g.N * e->g.N * e->g.N;
+  float4* dd = nullptr;
+  int32_t* ds = nullptr;
+  std::vector<float4> hd;
+  hipError_t st = hipSuccess;
+  if (nslots > 0) {
+    int32_t table[64];
+    for (int k = 0; k < 64; ++k) {
+      table[k] = k < nmat ? material_slot[k] : -1;
+      if (table[k] >= nslots || table[k] < -1) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad delta slot %d", table[k]);
+    }
+    hd.resize((size_t)nslots * Nn);
+    for (size_t k = 0; k < hd.size(); ++k)
+      hd[k] = make_float4((float)deltas[3 * k], (float)deltas[3 * k + 1], (float)deltas[3 * k + 2], 0.f);
+    st = hipMallocAsync((void**)&dd, sizeof(float4) * hd.size(), e->stream);
+    if (st == hipSuccess) st = hipMallocAsync((void**)&ds, sizeof table, e->stream);
+    if (st == hipSuccess)
+      st = hipMemcpyAsync(dd, hd.data(), sizeof(float4) * hd.size(), hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) st = hipMemcpyAsync(ds, table, sizeof table, hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);  // host buffers go out of scope
+  }
+  if (st == hipSuccess) {
+    const StepParams sp = params(e, dt);
+    e->d.deltas = dd;
+    e->d.delta_slot = ds;
+    Timed t(e, "g2p", true);
+    launch_fused(e->stream, e->d, e->n, e->g, sp, 2, t.ev());
+    e->d.deltas = nullptr;
+    e->d.delta_slot = nullptr;
+    st = hipGetLastError();
+  }
+  if (dd) (void)hipFreeAsync(dd, e->stream);
+  if (ds) (void)hipFreeAsync(ds, e->stream);
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "step from grid: %s", hipGetErrorString(st));
+  e->steps_since_sort++;
+  e->split_pending = false;
   return MPM_OK;
 }
 
@@ -932,6 +1060,7 @@ int mpm_frame_defaults(const mpm_engine* e, mpm_frame_config* c) {
 }
 
 int mpm_advance_frame(mpm_engine* e, const mpm_frame_config* c, double dt, mpm_frame_report* out) {
+  if (e && e->split_pending) return fail(e, MPM_ERR_NOT_READY, "a split step waits for mpm_step_from_grid");
   if (!e || !c) return MPM_ERR_INVALID_ARGUMENT;
   if (!(dt > 0) || !std::isfinite(dt)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt");
   if (!(c->time_step_scale > 0) || c->max_substeps < 1 || !(c->velocity_limit > 0) || c->gravity_mode < 0 ||

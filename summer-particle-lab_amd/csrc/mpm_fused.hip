@@ -127,7 +127,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                                             float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
                                             uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp,
                                             unsigned long long* __restrict__ sur_counts, unsigned char* s_raw,
-                                            int* s_box) {
+                                            int* s_box, const float4* __restrict__ deltas = nullptr,
+                                            const int32_t* __restrict__ delta_slot = nullptr) {
   using S = Shape<FBK>;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
@@ -199,6 +200,14 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     }
     __syncthreads();
     M3 Fn = F;
+    // Per-part grid velocity deltas of a split step (src/three/mpm.js:114).
+    const float4* dl = nullptr;
+    if constexpr (MODE == 2) {
+      if (deltas) {
+        const int slot = delta_slot[mt & 255u];
+        if (slot >= 0) dl = deltas + (size_t)slot * N * N * N;
+      }
+    }
     if (gather) {
       float V[3] = {0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0}, Z[3] = {0, 0, 0};
 #pragma unroll
@@ -216,6 +225,21 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
             n0 = vel[r];
             n1 = vel[r + 1];
             n2 = vel[r + 2];
+          }
+          if constexpr (MODE == 2) {
+            if (dl) {
+              const size_t r = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
+              const float4 d0 = dl[r], d1 = dl[r + 1], d2 = dl[r + 2];
+              n0.x += d0.x;
+              n0.y += d0.y;
+              n0.z += d0.z;
+              n1.x += d1.x;
+              n1.y += d1.y;
+              n1.z += d1.z;
+              n2.x += d2.x;
+              n2.y += d2.y;
+              n2.z += d2.z;
+            }
           }
           const float wjk = w[1][j] * w[2][k];
           const float r0x = w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x;
@@ -516,11 +540,12 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, uint32_t n,
     const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
     unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
-    uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts) {
+    uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts,
+    const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB>::LDS_BYTES];
   __shared__ int s_box[6 * Shape<FB>::WAVES];
   fused_chunk<MODE, SUR, FB, EXT>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
-                                  acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box);
+                                  acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, deltas, delta_slot);
 }
 
 // Many scenes in one persistent launch: each workgroup takes chunks from a
@@ -564,7 +589,8 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
   const dim3 grid(fblocks(n)), block(FB);
 #define MPM_FUSED(M, S, X)                                                                                      \
   hipExtLaunchKernelGGL((k_fused<M, S, X>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n, \
-                        d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts)
+                        d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts,      \
+                        mode == 2 ? d.deltas : nullptr, d.delta_slot)
 #define MPM_FUSED_X(M, S) \
   if (d.ext_laws)         \
     MPM_FUSED(M, S, true); \

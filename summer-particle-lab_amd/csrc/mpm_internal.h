@@ -55,6 +55,8 @@ struct StepParams {
 struct DevState {
   float* planes;      // NPLANES * cap
   bool ext_laws;      // some material uses damage or a pressure model (8, 9)
+  const float4* deltas;      // per-slot grid velocity deltas of a split step (N^3 each), or null
+  const int32_t* delta_slot; // per material: delta slot or -1 [64]
   float* planes_alt;  // permutation target
   uint32_t* ids;      // original id per slot
   uint32_t* ids_alt;
@@ -161,6 +163,10 @@ void launch_particle_reduce(hipStream_t s, const DevState& d, uint32_t n, float 
 // Per block {max wave speed, max sum |C|}; particle_reduce_blocks() records.
 void launch_stable(hipStream_t s, const DevState& d, uint32_t n, float* out);
 void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out);
+// Dense view of the last grid update (mass, 3 velocities per node, pinned
+// flag) over the listed pieces, and velocities written back into them.
+void launch_grid_export(hipStream_t s, const DevState& d, const GridDims& g, float* mass, float* v, uint8_t* fixed);
+void launch_grid_import(hipStream_t s, const DevState& d, const GridDims& g, const float* v);
 // Frame snapshot: rows of `data` [ceil(n / stride) x 8] (may be null), FP64
 // block totals `partial` [snapshot_blocks(n) x 6] and counters `counts`
 // [13] (per model, then non-finite), which the caller zeroes.

@@ -147,6 +147,7 @@ __device__ __forceinline__ void grid_node(size_t node, size_t Nn, unsigned long 
                                           double& p1, double& p2) {
   if (node >= Nn) return;
   const unsigned long long M = acc_m[node];
+  const bool pin = fixed[node] != 0;
   Mq += (long long)M;
   const float px = acc_p[node], py = acc_p[pstride + node], pz = acc_p[2 * pstride + node];
   p0 += px;
@@ -163,7 +164,7 @@ __device__ __forceinline__ void grid_node(size_t node, size_t Nn, unsigned long 
     const int N = g.N;
     const int ix = (int)(node % N), iy = (int)((node / N) % N), iz = (int)(node / ((size_t)N * N));
     const int idx[3] = {ix, iy, iz};
-    bool sticky = fixed[node] != 0;
+    bool sticky = pin;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if ((g.wall[2 * a] == MPM_WALL_STICKY && idx[a] <= lo_idx[a]) ||
@@ -177,12 +178,56 @@ __device__ __forceinline__ void grid_node(size_t node, size_t Nn, unsigned long 
       if (sticky || lo_hit || hi_hit) v[a] = 0.f;
     }
   }
-  vel[node] = make_float4(v[0], v[1], v[2], m);
+  // w carries the node mass, negative at a node a pinned particle holds (the
+  // transfers read only xyz; grid views read the flag back).
+  vel[node] = make_float4(v[0], v[1], v[2], pin ? -m : m);
   acc_m[node] = 0ull;
   acc_p[node] = 0.f;
   acc_p[pstride + node] = 0.f;
   acc_p[2 * pstride + node] = 0.f;
   fixed[node] = 0;
+}
+
+// Dense view of this step's grid (after k_grid_list): mass, velocity and the
+// pinned flag of every node of the listed pieces. The outputs hold zeros
+// elsewhere (the caller clears them). Used by callers that act on the grid
+// between the update and G2P (src/three/mpm.js:104).
+__global__ __launch_bounds__(BLOCK) void k_grid_export(const float4* __restrict__ vel,
+                                                       const uint32_t* __restrict__ list, uint32_t npieces,
+                                                       const unsigned long long* __restrict__ counters, size_t Nn,
+                                                       float* __restrict__ mass, float* __restrict__ v,
+                                                       uint8_t* __restrict__ fixed) {
+  const unsigned long long start = counters[1], end = counters[2];
+  const uint32_t lane = threadIdx.x & 63u, W = gridDim.x * (BLOCK / 64);
+  for (uint64_t e = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); e < end - start; e += W) {
+    const size_t node = (size_t)list[(start + e) % npieces] * 64 + lane;
+    if (node >= Nn) continue;
+    const float4 u = vel[node];
+    mass[node] = fabsf(u.w);
+    fixed[node] = signbit(u.w) ? 1 : 0;
+    v[3 * node] = u.x;
+    v[3 * node + 1] = u.y;
+    v[3 * node + 2] = u.z;
+  }
+}
+
+// Velocities written back into the listed pieces (xyz; the mass and pinned
+// flag are kept).
+__global__ __launch_bounds__(BLOCK) void k_grid_import(float4* __restrict__ vel, const uint32_t* __restrict__ list,
+                                                       uint32_t npieces,
+                                                       const unsigned long long* __restrict__ counters, size_t Nn,
+                                                       const float* __restrict__ v) {
+  const unsigned long long start = counters[1], end = counters[2];
+  const uint32_t lane = threadIdx.x & 63u, W = gridDim.x * (BLOCK / 64);
+  for (uint64_t e = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); e < end - start; e += W) {
+    const size_t node = (size_t)list[(start + e) % npieces] * 64 + lane;
+    if (node >= Nn) continue;
+    float4 u = vel[node];
+    u.x = v[3 * node];
+    u.y = v[3 * node + 1];
+    u.z = v[3 * node + 2];
+    vel[node] = u;
+  }
 }
 
 // Adds a wave's grid sums of one scene to that scene's slot.
@@ -838,6 +883,18 @@ void launch_snapshot(hipStream_t s, const DevState& d, uint32_t n, const int32_t
                      double* partial, unsigned long long* counts) {
   hipLaunchKernelGGL(k_snapshot, dim3(snapshot_blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.ids, d.cap, n,
                      d.materials, codes, stride, data, partial, counts);
+}
+
+void launch_grid_export(hipStream_t s, const DevState& d, const GridDims& g, float* mass, float* v, uint8_t* fixed) {
+  const size_t Nn = (size_t)g.N * g.N * g.N;
+  hipLaunchKernelGGL(k_grid_export, dim3(1024), dim3(BLOCK), 0, s, d.vel, d.piece_list, grid_pieces(g),
+                     d.piece_counters, Nn, mass, v, fixed);
+}
+
+void launch_grid_import(hipStream_t s, const DevState& d, const GridDims& g, const float* v) {
+  const size_t Nn = (size_t)g.N * g.N * g.N;
+  hipLaunchKernelGGL(k_grid_import, dim3(1024), dim3(BLOCK), 0, s, d.vel, d.piece_list, grid_pieces(g),
+                     d.piece_counters, Nn, v);
 }
 
 void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out) {

@@ -215,6 +215,43 @@ class Engine:
     def min_j(self):
         return self.conservation()["min_J"]
 
+    def grid_nodes(self):
+        out = C.c_int32()
+        self._check(self._lib.mpm_debug_p2g(self.h, 0.0, None, None, C.byref(out)))
+        return out.value
+
+    def step_to_grid(self, dt):
+        """First half of a split step: stress, P2G and grid update."""
+        self._check(self._lib.mpm_step_to_grid(self.h, float(dt)))
+
+    def grid(self):
+        """(mass [N^3], v [N^3, 3], fixed [N^3]) of the pending split step,
+        node index (z N + y) N + x."""
+        N = self.grid_nodes()
+        m, v, f = np.zeros(N ** 3), np.zeros(3 * N ** 3), np.zeros(N ** 3, np.uint8)
+        self._check(self._lib.mpm_grid_download(self.h, m.ctypes.data_as(C.POINTER(C.c_double)),
+                                                v.ctypes.data_as(C.POINTER(C.c_double)),
+                                                f.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return m, v.reshape(-1, 3), f
+
+    def set_grid_velocity(self, v):
+        v = np.ascontiguousarray(v, np.float64).reshape(-1)
+        self._check(self._lib.mpm_grid_upload_velocity(self.h, v.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def step_from_grid(self, dt, deltas=None):
+        """Second half of a split step; deltas maps a material index to a
+        [N^3, 3] velocity added where that material's particles gather."""
+        if not deltas:
+            self._check(self._lib.mpm_step_from_grid(self.h, float(dt), None, 0, None, 0))
+            return
+        mats = sorted(deltas)
+        slot = np.full(64, -1, np.int32)
+        for k, m in enumerate(mats):
+            slot[m] = k
+        d = np.ascontiguousarray(np.concatenate([np.asarray(deltas[m], np.float64).reshape(-1) for m in mats]))
+        self._check(self._lib.mpm_step_from_grid(self.h, float(dt), slot.ctypes.data_as(C.POINTER(C.c_int32)), 64,
+                                                 d.ctypes.data_as(C.POINTER(C.c_double)), len(mats)))
+
     def snapshot(self, preview_limit=100_000, codes=None):
         """(totals dict, preview rows [render_count, 8] float32) of the
         resident state (mpm_snapshot)."""

@@ -152,6 +152,11 @@ SIGNATURES = {
     "mpm_conservation_report": (C.c_int, [_P, C.POINTER(MpmConservation)]),
     "mpm_refresh_diagnostics": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mpm_preview_layout": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mpm_step_to_grid": (C.c_int, [_P, C.c_double]),
+    "mpm_grid_download": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint8)]),
+    "mpm_grid_upload_velocity": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mpm_step_from_grid": (C.c_int, [_P, C.c_double, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double),
+                                     C.c_int32]),
     "mpm2_create": (C.c_int, [C.c_uint32, C.POINTER(_P)]),
     "mpm2_destroy": (C.c_int, [_P]),
     "mpm2_last_error": (C.c_char_p, [_P]),
