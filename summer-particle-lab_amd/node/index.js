@@ -48,6 +48,11 @@ class MPM3 {
 
   configure(domain) {
     this.domain = domain; this.H = domain.cellSize;
+    // The grid's origin and edge (src/three/discretization.js:13): two guard
+    // cells below, nodes = ceil((upper - origin) / h) + 3 per axis.
+    this.O = domain.lower.map((v) => Math.floor(v / domain.cellSize) * domain.cellSize - 2 * domain.cellSize);
+    this.N = Math.max(...domain.upper.map((v, a) => Math.ceil((v - this.O[a]) / domain.cellSize) + 3));
+    this.mass = this.v = this.fixed = null; this.base = new Int32Array(this.capacity * 3);
     if (this.engine) native.destroy(this.engine);
     this.engine = native.create({lower: domain.lower, upper: domain.upper, cellSize: domain.cellSize,
       capacity: this.capacity, wallMode: [0, 0, 0, 0, 0, 0], gravity: [0, 0, 0], gravityOnParticles: 0,
@@ -75,6 +80,7 @@ class MPM3 {
     const material = new Uint8Array(s.count);
     for (let p = 0; p < s.count; p++) {
       if (s.isSolid && !s.isSolid(p)) continue;
+      if (this._part !== undefined && (s.lab && s.lab.part ? s.lab.part[p] : 0) !== this._part) continue;
       const model = s.model ? s.model[p] : 0;
       if (!SUPPORTED.has(model)) throw new Error(`model ${model} is not supported by the MI355X engine`);
       const part = s.lab && s.lab.part ? s.lab.part[p] : 0;
@@ -103,7 +109,7 @@ class MPM3 {
         if (Number.isFinite(fm.onset)) fracture.onset = fm.onset;
         if (Number.isFinite(fm.failure)) fracture.failure = fm.failure;
         materials.push({model, E, mu: 0.4 * E, lambda: 0.4 * E, thermal, plastic, pressure, fracture,
-          yieldStrain: ys, friction: fr, code});
+          yieldStrain: ys, friction: fr, code, part});
       }
       material[ids.length] = m;
       ids.push(p);
@@ -179,9 +185,103 @@ class MPM3 {
   }
 
   step(s, dt) {
+    if (s.separatePartGrids) { this._stepParts(s, dt); return; }
     if (!this.options.resident || !this.onDevice) this.upload(s);
-    native.step(this.engine, dt, 1);
+    if (s.externalGridContact || s.contactGridDeltas) this._splitStep(s, dt);
+    else native.step(this.engine, dt, 1);
     if (!this.options.resident) this.download(s);
+  }
+
+  // One engine per part for s.separatePartGrids (src/three/mpm.js:58,69):
+  // parts never meet on a grid, so each part's particles step on their own
+  // engine and grid; the children share this object's per-particle arrays.
+  _child(part) {
+    this._children = this._children || new Map();
+    let c = this._children.get(part);
+    if (!c) {
+      c = new MPM3(this.capacity, Object.assign({}, this.options, {domain: this.domain, resident: false}));
+      for (const k of ['F', 'C', 'Jp', 'dilation', 'stress', 'plastic', 'tau', 'cachedTau', 'cacheValid']) c[k] = this[k];
+      c._part = part;
+      this._children.set(part, c);
+    }
+    return c;
+  }
+
+  // The hooks see this object with `fields[part]` = {mass, v, fixed}, as the
+  // reference's part-separated grids (src/three/mpm.js:58).
+  _stepParts(s, dt) {
+    if (this.options.resident) throw new Error('separatePartGrids needs the default (non-resident) mode');
+    const parts = new Set();
+    for (let p = 0; p < s.count; p++) if (!s.isSolid || s.isSolid(p)) parts.add(s.lab && s.lab.part ? s.lab.part[p] : 0);
+    const kids = [...parts].sort((a, b) => a - b).map((k) => this._child(k));
+    const n3 = this.N * this.N * this.N;
+    for (const c of kids) { c.upload(s); native.stepToGrid(c.engine, dt); }
+    if (s.externalGridContact) {
+      const count = Math.max((s.projectPartCount || 0) + 1, ...kids.map((c) => c._part + 1));
+      if (!this.fields || this.fields.length !== count || !this.fields[0] || this.fields[0].mass.length !== n3)
+        this.fields = Array.from({length: count}, () => ({mass: new Float64Array(n3), v: new Float64Array(3 * n3),
+          fixed: new Uint8Array(n3)}));
+      if (!this.mass || this.mass.length !== n3) {
+        this.mass = new Float64Array(n3); this.v = new Float64Array(3 * n3); this.fixed = new Uint8Array(n3);
+      }
+      for (const f of this.fields) { f.mass.fill(0); f.v.fill(0); f.fixed.fill(0); }
+      for (const c of kids) { const f = this.fields[c._part]; native.gridDownload(c.engine, f.mass, f.v, f.fixed); }
+      for (let p = 0; p < s.count; p++) {
+        const x = [s.x[p], s.y[p], s.z[p]];
+        for (let a = 0; a < 3; a++) {
+          this.base[p * 3 + a] = Math.max(0, Math.min(this.N - 3, Math.floor((x[a] - this.O[a]) / this.H - .5)));
+        }
+      }
+      s.externalGridContact(this, dt);
+      for (const c of kids) native.gridUploadVelocity(c.engine, this.fields[c._part].v);
+    }
+    for (const c of kids) {
+      const d = s.contactGridDeltas ? s.contactGridDeltas[c._part] : null;
+      const slots = new Int32Array(64).fill(d ? 0 : -1);
+      native.stepFromGrid(c.engine, dt, slots, d ? Float64Array.from(d.subarray(0, 3 * n3)) : null, d ? 1 : 0);
+      c.download(s);
+    }
+  }
+
+  // A step with the reference's grid hooks (src/three/mpm.js:104,114): P2G
+  // and the grid update on the device, then s.externalGridContact(this, dt)
+  // on this object's mass, v, fixed, base, N, O, H (the reference's own
+  // fields), the changed velocities back to the device, and G2P with each
+  // part's s.contactGridDeltas added.
+  _splitStep(s, dt) {
+    native.stepToGrid(this.engine, dt);
+    const n3 = this.N * this.N * this.N;
+    if (s.externalGridContact) {
+      if (!this.mass || this.mass.length !== n3) {
+        this.mass = new Float64Array(n3); this.v = new Float64Array(3 * n3); this.fixed = new Uint8Array(n3);
+      }
+      native.gridDownload(this.engine, this.mass, this.v, this.fixed);
+      for (let p = 0; p < s.count; p++) {  // the step's stencil bases (src/three/mpm.js:25-28)
+        const x = [s.x[p], s.y[p], s.z[p]];
+        for (let a = 0; a < 3; a++) {
+          this.base[p * 3 + a] = Math.max(0, Math.min(this.N - 3, Math.floor((x[a] - this.O[a]) / this.H - .5)));
+        }
+      }
+      s.externalGridContact(this, dt);
+      native.gridUploadVelocity(this.engine, this.v);
+    }
+    const slots = new Int32Array(64).fill(-1), arrays = [];
+    if (s.contactGridDeltas) {
+      this._materials.forEach((m, k) => {
+        const d = s.contactGridDeltas[m.part];
+        if (!d) return;
+        if (d.length < 3 * n3) throw new Error(`contactGridDeltas[${m.part}] needs ${3 * n3} values`);
+        let slot = arrays.indexOf(d);
+        if (slot < 0) { slot = arrays.length; arrays.push(d); }
+        slots[k] = slot;
+      });
+    }
+    let deltas = null;
+    if (arrays.length) {
+      deltas = new Float64Array(arrays.length * 3 * n3);
+      arrays.forEach((d, k) => deltas.set(d.subarray(0, 3 * n3), k * 3 * n3));
+    }
+    native.stepFromGrid(this.engine, dt, slots, deltas, arrays.length);
   }
 
   // One frame of Simulation3.step for an all-solid scene, run on the device
@@ -248,7 +348,12 @@ class MPM3 {
 
   conservation() { return native.conservation(this.engine); }
 
-  destroy() { if (this.engine) native.destroy(this.engine); this.engine = null; }
+  destroy() {
+    if (this.engine) native.destroy(this.engine);
+    this.engine = null;
+    if (this._children) for (const c of this._children.values()) c.destroy();
+    this._children = null;
+  }
 }
 
 const {SolidMPM, solidStress, isMPMSolid} = require('./solid_mpm.js');

@@ -522,6 +522,95 @@ static napi_value js_frame_packet(napi_env env, napi_callback_info info) {
   return ab;
 }
 
+/* ------------------------------------------------------------ split step */
+static int typed(napi_env env, napi_value v, napi_typedarray_type want, size_t min_len, void** data) {
+  bool is_ta = false;
+  napi_typedarray_type type;
+  size_t length, offset;
+  napi_value ab;
+  *data = NULL;
+  napi_is_typedarray(env, v, &is_ta);
+  if (!is_ta || napi_get_typedarray_info(env, v, &type, &length, data, &ab, &offset) != napi_ok || type != want ||
+      length < min_len) {
+    napi_throw_type_error(env, NULL, "typed array of the wrong type or length");
+    return -1;
+  }
+  return 0;
+}
+
+static size_t grid_nodes_of(mpm_engine* e) {
+  int32_t N = 0;
+  mpm_debug_p2g(e, 0.0, NULL, NULL, &N);
+  return (size_t)N * N * N;
+}
+
+/* stepToGrid(h, dt) */
+static napi_value js_step_to_grid(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  double dt = 0;
+  napi_get_value_double(env, argv[1], &dt);
+  int rc = mpm_step_to_grid(e, dt);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+/* gridDownload(h, mass Float64Array N^3, v Float64Array 3N^3, fixed Uint8Array N^3) */
+static napi_value js_grid_download(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 4 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  const size_t nn = grid_nodes_of(e);
+  void *m, *v, *f;
+  if (typed(env, argv[1], napi_float64_array, nn, &m) || typed(env, argv[2], napi_float64_array, 3 * nn, &v) ||
+      typed(env, argv[3], napi_uint8_array, nn, &f))
+    return NULL;
+  int rc = mpm_grid_download(e, (double*)m, (double*)v, (uint8_t*)f);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+/* gridUploadVelocity(h, v Float64Array 3N^3) */
+static napi_value js_grid_upload_velocity(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  void* v;
+  if (typed(env, argv[1], napi_float64_array, 3 * grid_nodes_of(e), &v)) return NULL;
+  int rc = mpm_grid_upload_velocity(e, (const double*)v);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+/* stepFromGrid(h, dt, slots Int32Array|null, deltas Float64Array|null, nslots) */
+static napi_value js_step_from_grid(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  double dt = 0;
+  int32_t nslots = 0;
+  napi_get_value_double(env, argv[1], &dt);
+  if (argc >= 5) napi_get_value_int32(env, argv[4], &nslots);
+  void *slots = NULL, *deltas = NULL;
+  if (nslots > 0) {
+    if (typed(env, argv[2], napi_int32_array, 64, &slots) ||
+        typed(env, argv[3], napi_float64_array, (size_t)nslots * 3 * grid_nodes_of(e), &deltas))
+      return NULL;
+  }
+  int rc = mpm_step_from_grid(e, dt, (const int32_t*)slots, nslots > 0 ? 64 : 0, (const double*)deltas, nslots);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
 /* ------------------------------------------------------------------ 2D */
 static void finalize_engine2(napi_env env, void* data, void* hint) {
   (void)env;
@@ -810,6 +899,10 @@ static napi_value init(napi_env env, napi_value exports) {
              {"refreshDiagnostics", js_refresh_diagnostics},
              {"snapshot", js_snapshot},
              {"framePacket", js_frame_packet},
+             {"stepToGrid", js_step_to_grid},
+             {"gridDownload", js_grid_download},
+             {"gridUploadVelocity", js_grid_upload_velocity},
+             {"stepFromGrid", js_step_from_grid},
              {"create2", js_create2},
              {"destroy2", js_destroy2},
              {"upload2", js_upload2},

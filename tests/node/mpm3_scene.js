@@ -23,6 +23,24 @@ if (input.part) for (let i = 0; i < n; i++) s.lab.part[i] = input.part[i];
 if (input.settings) Object.assign(s.settings, input.settings);
 if (input.boundsLo) { s.boundsLo = input.boundsLo; s.boundsHi = input.boundsHi; }
 const mpm = new MPM3(n, {sortEvery: 1});
+// Grid hooks (src/three/mpm.js:104,114): a rigid plate below plateY that
+// stops nodes moving into it, and a constant velocity delta for one part.
+if (input.plateY !== undefined) {
+  s.externalGridContact = (grid, dt) => {
+    const N = grid.N;
+    for (let node = 0; node < N * N * N; node++) {
+      const y = grid.O[1] + (Math.floor(node / N) % N) * grid.H;
+      if (grid.mass[node] > 1e-10 && y < input.plateY && grid.v[node * 3 + 1] < 0) grid.v[node * 3 + 1] = 0;
+    }
+  };
+}
+if (input.separate) { s.separatePartGrids = true; s.projectPartCount = Math.max(...input.part); }
+if (input.deltaPart !== undefined) {
+  const d = new Float64Array(3 * mpm.N ** 3);
+  for (let k = 0; k < d.length; k += 3) d.set(input.delta, k);
+  s.contactGridDeltas = [];
+  s.contactGridDeltas[input.deltaPart] = d;
+}
 if (input.F) for (let i = 0; i < n; i++) mpm.F.set(input.F, i * 9);
 const dtStable = mpm.stableStep(s);
 const frames = [];

@@ -39,7 +39,8 @@ console.log(JSON.stringify({{names, code, typeErr}}));
     assert out["names"] == sorted(["create", "destroy", "setMaterials", "upload", "download", "step", "synchronize",
                                    "stableDt", "conservation", "refreshDiagnostics", "lastError", "advanceFrame",
                                    "snapshot", "framePacket", "create2", "destroy2", "upload2", "download2",
-                                   "step2", "stableDt2", "advance2", "stats2"])
+                                   "step2", "stableDt2", "advance2", "stats2", "stepToGrid", "gridDownload",
+                                   "gridUploadVelocity", "stepFromGrid"])
     assert out["code"] == "MPM1"
     assert "engine handle" in out["typeErr"]
 
@@ -246,3 +247,63 @@ def test_drop_in_solid_mpm_host_loop_matches_oracle(gpu, tmp_path):
     counts = [sc.advance(1 / 60, gravity=0)[0] for _ in range(3)]
     assert got["frames"] == counts
     np.testing.assert_allclose(np.stack([got["x"], got["y"]], 1), sc.positions(), atol=.003, rtol=.001)
+
+
+@pytest.mark.gpu
+def test_drop_in_grid_hooks_match_oracle(gpu, tmp_path):
+    """MPM3.step with s.externalGridContact (a rigid plate that stops nodes
+    moving down below y = 6) and s.contactGridDeltas (a constant velocity for
+    part 1) -- the reference's grid hooks (src/three/mpm.js:104,114) -- on two
+    blocks, 20 steps, against the oracle's split step with the same hooks."""
+    pts, _ = four_blocks()
+    pts = pts[:54]
+    part = [0] * 27 + [1] * 27
+    delta = [.5, 0, -.25]
+    inp = dict(points=pts.tolist(), velocity=[0, -40, 0], models=[0] * 54, dt=1 / 240, steps=20, part=part,
+               plateY=6.0, deltaPart=1, delta=delta)
+    path = tmp_path / "hooks.json"
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout)
+    orc = o.Scene(pts, (0, -40, 0), 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8, part=part)
+    N = orc.N
+    iy = (np.arange(N ** 3) // N) % N
+    y = orc.origin[1] + iy * 8
+    d = np.tile(delta, N ** 3)
+    for _ in range(20):
+        orc.step_to_grid(1 / 240)
+        v = orc.gv.reshape(-1, 3)
+        v[(orc.gmass > 1e-10) & (y < 6.0) & (v[:, 1] < 0), 1] = 0
+        orc.step_from_grid(1 / 240, {1: d})
+    x = np.stack([got["x"], got["y"], got["z"]], 1)
+    np.testing.assert_allclose(x, orc.positions(), atol=.003, rtol=.001)
+    assert orc.positions()[:, 1].min() > -4  # the plate held
+
+
+@pytest.mark.gpu
+def test_drop_in_separate_part_grids_match_independent_oracles(gpu, tmp_path):
+    """s.separatePartGrids (src/three/mpm.js:58,69): two blocks moving through
+    the same space never meet on a grid, so each part steps as it would alone.
+    The JS class against one oracle scene per part."""
+    a = np.arange(3) * 4.0 - 4
+    blk = np.array([[x, y + 4, z] for z in a for y in a for x in a])
+    pts = np.concatenate([blk - [10, 0, 0], blk + [10, 0, 0]])
+    part = [0] * 27 + [1] * 27
+    vel = [[40, 0, 0]] * 27 + [[-40, 0, 0]] * 27
+    inp = dict(points=pts.tolist(), velocity=vel, models=[0] * 54, dt=1 / 240, steps=60, part=part, separate=True)
+    path = tmp_path / "parts.json"
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout)
+    x = np.stack([got["x"], got["y"], got["z"]], 1)
+    for k, sl in ((0, slice(0, 27)), (1, slice(27, 54))):
+        orc = o.Scene(pts[sl], vel[sl][0], 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8)
+        for _ in range(60):
+            orc.step(1 / 240)
+        np.testing.assert_allclose(x[sl], orc.positions(), atol=.003, rtol=.001)
+    # the blocks moved into each other's space (60 steps, 20 units relative) without meeting
+    assert x[:27, 0].mean() > -2 and x[27:, 0].mean() < 2
