@@ -36,9 +36,11 @@ namespace {
 constexpr int FB = 256;            // particles per chunk of the single-scene launch
 constexpr int TR_ROWS = 36;        // per-wave transpose: 9 node masses + 27 momentum rows
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 word: rows start in different banks
-constexpr int CHUNK = 8;           // lanes per scatter chunk (run pieces summed by one lane)
+constexpr int CHUNK = 32;          // lanes per scatter chunk (run pieces summed by one lane)
 constexpr int WAVE_BYTES = TR_ROWS * TR_STRIDE * 4 + 64 * 16;
 static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0, "surrogate activations must fit the wave buffer");
+static_assert(((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK) * 4 <= WAVE_BYTES,
+              "the run-sum reads past a row's last lane stay inside the wave buffer");
 
 // LDS plan of a workgroup of FBK particles (one per lane): a node box for the
 // scatter (20 B per node, 2 nodes per particle) and one transpose buffer per
@@ -476,17 +478,21 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     for (int t = (int)lane; t < TR_ROWS * nchunks; t += 64) {
       const int row = t % TR_ROWS;
       const int4 ch = s_chunk[t / TR_ROWS];
-      const float* src = s_tr + row * TR_STRIDE;
+      // All CHUNK words are read unconditionally, so the loads issue back to
+      // back with one wait; words past the chunk's last lane (at most 6 past
+      // the row, still inside this wave's buffer) are masked out.
+      const float* src = s_tr + row * TR_STRIDE + ch.x;
+      float vals[CHUNK];
+#pragma unroll
+      for (int q = 0; q < CHUNK; ++q) vals[q] = src[q];
+      const int len = ch.y - ch.x;
       int si32 = 0;
       float sf = 0.f;
 #pragma unroll
       for (int q = 0; q < CHUNK; ++q) {
-        const int p = ch.x + q;
-        if (p <= ch.y) {
-          const float vq = src[p];
-          si32 += __float_as_int(vq);
-          sf += vq;
-        }
+        const bool in = q <= len;
+        si32 += in ? __float_as_int(vals[q]) : 0;
+        sf += in ? vals[q] : 0.f;
       }
       if (ch.w >= 0) {
         const int nn = row < 9 ? row : (row - 9) / 3;
