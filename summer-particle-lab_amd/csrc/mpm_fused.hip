@@ -34,13 +34,16 @@ namespace mpm {
 namespace {
 
 constexpr int FB = 256;            // particles per chunk of the single-scene launch
-constexpr int TR_ROWS = 36;        // per-wave transpose: 9 node masses + 27 momentum rows
-constexpr int TR_STRIDE = 65;      // 64 lanes + 1 word: rows start in different banks
-constexpr int CHUNK = 32;          // lanes per scatter chunk (run pieces summed by one lane)
-constexpr int WAVE_BYTES = TR_ROWS * TR_STRIDE * 4 + 64 * 16;
+constexpr int TR_ROWS = 9;         // per-wave transpose: one row per node of a slab, {mass, momentum} per lane
+constexpr int TR_STRIDE = 65;      // 64 lanes + 1 entry: rows start in different banks
+constexpr int CHUNK = 16;          // lanes per scatter chunk (run pieces summed by one lane)
+constexpr int TR_TAIL = 16;        // zero entries after the last row
+constexpr int TR_ENTRIES = TR_ROWS * TR_STRIDE + TR_TAIL;
+constexpr int WAVE_BYTES = TR_ENTRIES * 16 + 64 * 16;
 static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0, "surrogate activations must fit the wave buffer");
-static_assert(((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK) * 4 <= WAVE_BYTES,
-              "the run-sum reads past a row's last lane stay inside the wave buffer");
+static_assert((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK <= TR_ENTRIES,
+              "the run-sum reads past a row's last lane stay inside the rows and the zero tail");
+static_assert(TR_ROWS + TR_TAIL <= 64, "one store per lane zeroes the row pads and the tail");
 
 // LDS plan of a workgroup of FBK particles (one per lane): a node box for the
 // scatter (20 B per node, 2 nodes per particle) and one transpose buffer per
@@ -395,8 +398,10 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int k = 0; k < 3; ++k) wyz[j][k] = w[1][j] * w[2][k];
-  // Particle mass in fixed-point units (at most 2^24, so every part and every
-  // sum of up to 64 parts fits 32 bits).
+  // Particle mass in fixed-point units, at most 2^24: a part is at most
+  // 0.75^3 of it (below 2^23) and the centre remainder at least 0.125 Q - 13,
+  // so every part fits a signed 24-bit factor and every sum of up to 64 parts
+  // fits 32 bits.
   const float Qf = valid ? rintf(m * sp.mass_scale) : 0.f;
   const int Q = (int)Qf;
 
@@ -429,12 +434,21 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
   const int gbase = (int)key;  // linear index of the base node
 
-  // Per-wave LDS: 36 transpose rows (9 node masses, 27 node momenta) of 64
-  // particles, and the chunk table {first lane, last lane, box base, grid
-  // base (-1 when the chunk holds no particle)}.
+  // Per-wave LDS: 9 transpose rows (the nodes of a slab) of 64 particles,
+  // each entry {fixed-point mass part (as bits), momentum x, y, z}, a zero
+  // tail, and the chunk table {first lane, last lane, box base, grid base
+  // (-1 when the chunk holds no particle)}. The run sums read whole chunks
+  // and mask by multiplying with 0 or 1, so every entry they can reach past
+  // a chunk's end -- the next lanes, a row's pad entry, the tail -- must hold
+  // a finite value: the pads and the tail are zeroed here (the buffer held
+  // the gather box or surrogate activations before), the lanes are written
+  // in every slab.
   unsigned char* wbuf = s_raw + S::ACC_BYTES + (tid >> 6) * WAVE_BYTES;
-  float* s_tr = reinterpret_cast<float*>(wbuf);
-  int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ROWS * TR_STRIDE * 4);
+  float4* s_tr = reinterpret_cast<float4*>(wbuf);
+  int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ENTRIES * 16);
+  if (lane < (uint32_t)(TR_ROWS + TR_TAIL))
+    s_tr[lane < (uint32_t)TR_ROWS ? lane * TR_STRIDE + 64 : TR_ROWS * TR_STRIDE + (lane - TR_ROWS)] =
+        make_float4(0.f, 0.f, 0.f, 0.f);
   if ((chunk_starts >> lane) & 1ull) {
     const int c = __popcll(chunk_starts & (upto >> 1));  // chunks before this lane
     const uint64_t later = chunk_starts & ~upto;
@@ -444,9 +458,9 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 
   // Transpose through LDS: each lane writes its particle's contributions to
   // the 9 nodes of one z-slab as a column; then every lane sums whole
-  // (row, chunk) tasks and adds each chunk sum to the node box with one LDS
-  // atomic. The centre slab goes last: its centre node takes the remainder
-  // of the fixed-point mass so the 27 parts sum exactly to Q.
+  // (node, chunk) tasks and adds each chunk sum to the node box with one LDS
+  // atomic per channel. The centre slab goes last: its centre node takes
+  // the remainder of the fixed-point mass so the 27 parts sum exactly to Q.
   int qsum = 0;
 #pragma unroll
   for (int si = 0; si < 3; ++si) {
@@ -465,53 +479,59 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         qm = __float2int_rn(wt * Qf);
         qsum += qm;
       }
-      s_tr[nn * TR_STRIDE + lane] = __int_as_float(qm);
+      float pc[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        s_tr[(9 + 3 * nn + c) * TR_STRIDE + lane] =
-            wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
+      for (int c = 0; c < 3; ++c) pc[c] = wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
+      s_tr[nn * TR_STRIDE + lane] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
       if (pinned && valid && wt > 1e-8f) fixed[(size_t)gbase + ((size_t)k * N + j) * N + ii] = 1;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int t = (int)lane; t < TR_ROWS * nchunks; t += 64) {
-      const int row = t % TR_ROWS;
+      const int nn = t % TR_ROWS;
       const int4 ch = s_chunk[t / TR_ROWS];
-      // All CHUNK words are read unconditionally, so the loads issue back to
-      // back with one wait; words past the chunk's last lane (at most 6 past
-      // the row, still inside this wave's buffer) are masked out.
-      const float* src = s_tr + row * TR_STRIDE + ch.x;
-      float vals[CHUNK];
+      // All CHUNK entries are read unconditionally, so the loads issue back
+      // to back with one wait; entries past the chunk's last lane (at most
+      // 15 past the row, still inside this wave's buffer) are masked out by
+      // the bits of `in`. Mass parts are below 2^23 in magnitude, so the
+      // 24-bit multiply-add sums them exactly.
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      const f4* src = reinterpret_cast<const f4*>(s_tr + nn * TR_STRIDE + ch.x);
+      const uint32_t in = (2u << (ch.y - ch.x)) - 1u;
+      int sq = 0;
+      float px = 0.f, py = 0.f, pz = 0.f;
 #pragma unroll
-      for (int q = 0; q < CHUNK; ++q) vals[q] = src[q];
-      const int len = ch.y - ch.x;
-      int si32 = 0;
-      float sf = 0.f;
+      for (int h = 0; h < CHUNK; h += 8) {
+      f4 vals[8];
 #pragma unroll
-      for (int q = 0; q < CHUNK; ++q) {
-        const bool in = q <= len;
-        si32 += in ? __float_as_int(vals[q]) : 0;
-        sf += in ? vals[q] : 0.f;
+      for (int q = 0; q < 8; ++q) vals[q] = src[h + q];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t mi = (in >> (h + q)) & 1u;
+        const float mf = (float)mi;
+        sq += __mul24(__float_as_int(vals[q].x), (int)mi);
+        px = fmaf(vals[q].y, mf, px);
+        py = fmaf(vals[q].z, mf, py);
+        pz = fmaf(vals[q].w, mf, pz);
+      }
       }
       if (ch.w >= 0) {
-        const int nn = row < 9 ? row : (row - 9) / 3;
-        const int c = row < 9 ? 0 : (row - 9) % 3;
         const int ii = nn % 3, j = nn / 3;
+        const unsigned long long mq = (unsigned long long)(long long)sq;
         if (staged) {
           const int ln = ch.z + (k * ey + j) * ex + ii;
-          if (row < 9)
-            atomicAdd(&s_m[ln], (unsigned long long)(unsigned int)si32);
-          else
-            atomicAdd(&s_p[c * S::PATCH_ACC + ln], sf);
+          atomicAdd(&s_m[ln], mq);
+          atomicAdd(&s_p[ln], px);
+          atomicAdd(&s_p[S::PATCH_ACC + ln], py);
+          atomicAdd(&s_p[2 * S::PATCH_ACC + ln], pz);
         } else {
           const size_t node = (size_t)(uint32_t)ch.w + ((size_t)k * N + j) * N + ii;
-          if (row < 9) {
-            atomicAdd(&acc_m[node], (unsigned long long)(unsigned int)si32);
-            pieces[node >> 6] = 1;
-          } else {
-            atomicAdd(&acc_p[(size_t)c * pstride + node], sf);
-          }
+          atomicAdd(&acc_m[node], mq);
+          atomicAdd(&acc_p[node], px);
+          atomicAdd(&acc_p[pstride + node], py);
+          atomicAdd(&acc_p[2 * pstride + node], pz);
+          pieces[node >> 6] = 1;
         }
       }
     }
@@ -538,7 +558,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 }
 
 // Single scene: one chunk of FB particles per workgroup. Three workgroups
-// per CU is the LDS limit (3 x 51.8 KB of 160 KB); the register budget is
+// per CU is the LDS limit (3 x 52.8 KB of 160 KB); the register budget is
 // held to the same occupancy so the surrogate variant's MFMA operands do not
 // cost a wave.
 template <int MODE, bool SUR, bool EXT>
