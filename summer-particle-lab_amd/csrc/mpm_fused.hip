@@ -73,23 +73,40 @@ __device__ __forceinline__ int base_axis(float x, float O, float invH, int N, fl
   return b;
 }
 
+// DPP controls (GFX9): row shifts within 16-lane rows, the broadcasts of
+// lane 15 of each row to the next row and of lane 31 to rows 2 and 3, and
+// the shift of the whole wave by one lane.
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143, DPP_WAVE_SHR1 = 0x138;
+
+// Minimum (MAX = false) or maximum over the 64 lanes of a wave, all lanes
+// active: lane i folds in lanes i-1, i-3, i-7, i-15 of its row (a source
+// outside the row reads as the identity), so lane 15 of each row holds the
+// row's result; the two broadcasts fold rows 0..2 into lane 63, whose value
+// is returned to every lane.
+template <bool MAX>
+__device__ __forceinline__ int wave_extreme(int x) {
+  constexpr int ID = MAX ? (int)0x80000000 : 0x7fffffff;
+  auto fold = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR1, 0xf, 0xf, false));
+  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR2, 0xf, 0xf, false));
+  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR4, 0xf, 0xf, false));
+  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR8, 0xf, 0xf, false));
+  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_BCAST15, 0xa, 0xf, false));
+  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_BCAST31, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+
 // Block-wide box of base cells: lo[a] = min b[a], hi[a] = max b[a] over the
-// lanes with `use` set. Returns false when no lane is in use.
+// lanes with `use` set. Returns false when no lane is in use. Called with
+// every lane of the block active.
 template <int WAVES>
 __device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], int hi[3], int* s_box) {
   int mn[3], mx[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    mn[a] = use ? b[a] : 0x7fffffff;
-    mx[a] = use ? b[a] : -1;
-  }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
-      mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
-    }
+    mn[a] = wave_extreme<false>(use ? b[a] : 0x7fffffff);
+    mx[a] = wave_extreme<true>(use ? b[a] : -1);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
@@ -112,6 +129,16 @@ __device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], i
     }
   }
   return hi[0] >= 0;
+}
+
+// A box of base cells that is non-empty and inside [0, N - 3] on every axis,
+// so its nodes (up to hi + 2) are grid nodes. An empty box (no lane in use)
+// fails.
+__device__ __forceinline__ bool box_inside(const int lo[3], const int hi[3], int N) {
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) ok = ok && lo[a] >= 0 && lo[a] <= hi[a] && hi[a] <= N - 3;
+  return ok;
 }
 
 // Material record, read through the scalar path when the wave shares it.
@@ -194,7 +221,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     const bool gather = valid && !pinned;
     block_box<S::WAVES>(b, gather, lo, hi, s_box);
     const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
-    const bool staged = hi[0] >= 0 && ex * ey * ez <= S::PATCH_VEL;
+    const bool staged = box_inside(lo, hi, N) && ex * ey * ez <= S::PATCH_VEL;
     float4* s_vel = reinterpret_cast<float4*>(s_raw);
     if (staged) {
       const int cnt = ex * ey * ez;
@@ -409,7 +436,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // lanes of one run are contiguous), cut into chunks of at most CHUNK lanes.
   const uint32_t key = valid ? ((uint32_t)b[2] * (uint32_t)N + (uint32_t)b[1]) * (uint32_t)N + (uint32_t)b[0]
                              : 0xffffffffu - lane;
-  const uint32_t kprev = __shfl_up(key, 1, 64);
+  const uint32_t kprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, DPP_WAVE_SHR1, 0xf, 0xf, false);
   const uint64_t run_starts = __ballot(lane == 0u || kprev != key);
   const uint64_t upto = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
   const int run_start = 63 - __clzll(run_starts & upto);
@@ -420,7 +447,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   block_box<S::WAVES>(b, valid, lo, hi, s_box);
   const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
   const int cnt = ex * ey * ez;
-  const bool staged = hi[0] >= 0 && cnt <= S::PATCH_ACC;
+  const bool staged = box_inside(lo, hi, N) && cnt <= S::PATCH_ACC;
   unsigned long long* s_m = reinterpret_cast<unsigned long long*>(s_raw);
   float* s_p = reinterpret_cast<float*>(s_raw + 8 * S::PATCH_ACC);
   if (staged) {
@@ -456,6 +483,71 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     s_chunk[c] = make_int4((int)lane, next - 1, lbase, valid ? gbase : -1);
   }
 
+  // Sum of one (node, chunk) task of slab k, added to the node box (or to
+  // the grid when the box does not fit). All CHUNK entries are read
+  // unconditionally, so the loads issue back to back; entries past the
+  // chunk's last lane (at most 15 past the row, still inside this wave's
+  // buffer) are masked out by the bits of `in`. Mass parts are below 2^23 in
+  // magnitude, so the 24-bit multiply-add sums them exactly.
+  auto run_task = [&](int nn, const int4 ch, int k) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4* src = reinterpret_cast<const f4*>(s_tr + nn * TR_STRIDE + ch.x);
+    const uint32_t in = (2u << (ch.y - ch.x)) - 1u;
+    int sq = 0;
+    float px = 0.f, py = 0.f, pz = 0.f;
+#pragma unroll
+    for (int h = 0; h < CHUNK; h += 8) {
+      f4 vals[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) vals[q] = src[h + q];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t mi = (in >> (h + q)) & 1u;
+        const float mf = (float)mi;
+        sq += __mul24(__float_as_int(vals[q].x), (int)mi);
+        px = fmaf(vals[q].y, mf, px);
+        py = fmaf(vals[q].z, mf, py);
+        pz = fmaf(vals[q].w, mf, pz);
+      }
+    }
+    if (ch.w < 0) return;
+    const int ii = nn % 3, j = nn / 3;
+    const unsigned long long mq = (unsigned long long)(long long)sq;
+    if (staged) {
+      const int ln = ch.z + (k * ey + j) * ex + ii;
+      atomicAdd(&s_m[ln], mq);
+      atomicAdd(&s_p[ln], px);
+      atomicAdd(&s_p[S::PATCH_ACC + ln], py);
+      atomicAdd(&s_p[2 * S::PATCH_ACC + ln], pz);
+    } else {
+      const size_t node = (size_t)(uint32_t)ch.w + ((size_t)k * N + j) * N + ii;
+      atomicAdd(&acc_m[node], mq);
+      atomicAdd(&acc_p[node], px);
+      atomicAdd(&acc_p[pstride + node], py);
+      atomicAdd(&acc_p[2 * pstride + node], pz);
+      pieces[node >> 6] = 1;
+    }
+  };
+  // The chunk of this lane's first task is the same in every slab: read once.
+  const int ntask = TR_ROWS * nchunks;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int4 ch0 = s_chunk[lane / TR_ROWS];
+
+  // Nodes a pinned particle touches with weight above 1e-8 are pinned
+  // (src/three/mpm.js:83).
+  if (__any(pinned && valid)) {
+    if (pinned && valid) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int ii = 0; ii < 3; ++ii)
+            if (w[0][ii] * wyz[j][k] > 1e-8f) fixed[(size_t)gbase + ((size_t)k * N + j) * N + ii] = 1;
+    }
+  }
   // Transpose through LDS: each lane writes its particle's contributions to
   // the 9 nodes of one z-slab as a column; then every lane sums whole
   // (node, chunk) tasks and adds each chunk sum to the node box with one LDS
@@ -483,57 +575,14 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 #pragma unroll
       for (int c = 0; c < 3; ++c) pc[c] = wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
       s_tr[nn * TR_STRIDE + lane] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
-      if (pinned && valid && wt > 1e-8f) fixed[(size_t)gbase + ((size_t)k * N + j) * N + ii] = 1;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int t = (int)lane; t < TR_ROWS * nchunks; t += 64) {
-      const int nn = t % TR_ROWS;
-      const int4 ch = s_chunk[t / TR_ROWS];
-      // All CHUNK entries are read unconditionally, so the loads issue back
-      // to back with one wait; entries past the chunk's last lane (at most
-      // 15 past the row, still inside this wave's buffer) are masked out by
-      // the bits of `in`. Mass parts are below 2^23 in magnitude, so the
-      // 24-bit multiply-add sums them exactly.
-      typedef float f4 __attribute__((ext_vector_type(4)));
-      const f4* src = reinterpret_cast<const f4*>(s_tr + nn * TR_STRIDE + ch.x);
-      const uint32_t in = (2u << (ch.y - ch.x)) - 1u;
-      int sq = 0;
-      float px = 0.f, py = 0.f, pz = 0.f;
-#pragma unroll
-      for (int h = 0; h < CHUNK; h += 8) {
-      f4 vals[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) vals[q] = src[h + q];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const uint32_t mi = (in >> (h + q)) & 1u;
-        const float mf = (float)mi;
-        sq += __mul24(__float_as_int(vals[q].x), (int)mi);
-        px = fmaf(vals[q].y, mf, px);
-        py = fmaf(vals[q].z, mf, py);
-        pz = fmaf(vals[q].w, mf, pz);
-      }
-      }
-      if (ch.w >= 0) {
-        const int ii = nn % 3, j = nn / 3;
-        const unsigned long long mq = (unsigned long long)(long long)sq;
-        if (staged) {
-          const int ln = ch.z + (k * ey + j) * ex + ii;
-          atomicAdd(&s_m[ln], mq);
-          atomicAdd(&s_p[ln], px);
-          atomicAdd(&s_p[S::PATCH_ACC + ln], py);
-          atomicAdd(&s_p[2 * S::PATCH_ACC + ln], pz);
-        } else {
-          const size_t node = (size_t)(uint32_t)ch.w + ((size_t)k * N + j) * N + ii;
-          atomicAdd(&acc_m[node], mq);
-          atomicAdd(&acc_p[node], px);
-          atomicAdd(&acc_p[pstride + node], py);
-          atomicAdd(&acc_p[2 * pstride + node], pz);
-          pieces[node >> 6] = 1;
-        }
-      }
+    for (int t = (int)lane; t < ntask; t += 64) {
+      int4 ch = ch0;
+      if (t >= 64) ch = s_chunk[t / TR_ROWS];
+      run_task(t % TR_ROWS, ch, k);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
