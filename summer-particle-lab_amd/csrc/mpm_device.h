@@ -371,6 +371,80 @@ __device__ __forceinline__ float pressure_return(float e[3], int model, float& j
   return inc;
 }
 
+// Hencky strain eps = 1/2 log(F F^T) without a decomposition, for F near a
+// rotation: with G = F - I (exact for diagonal entries in [0.5, 2]) and
+// X = F F^T - I = G + G^T + G G^T (no digits cancel), log(I + X) is the
+// series X - X^2/2 + X^3/3 - X^4/4 + X^5/5, by Horner steps whose factors
+// are polynomials in X (so every product is symmetric). For |X|_F <= 1/32
+// the truncation error is below (1/32)^6 / 6 < 2e-10. eps has F's left
+// singular vectors and the log stretches as eigenvalues, so its trace and
+// deviatoric Frobenius norm are those of the principal log strains. Returns
+// false (eps unset) when |X|_F is above 1/32 or not finite. Symmetric
+// layout {xx, yy, zz, xy, xz, yz}.
+__device__ __forceinline__ bool hencky_small(const M3& F, float eps[6]) {
+  const float g0 = F[0] - 1.f, g4 = F[4] - 1.f, g8 = F[8] - 1.f;
+  const float g1 = F[1], g2 = F[2], g3 = F[3], g5 = F[5], g6 = F[6], g7 = F[7];
+  float X[6];
+  X[0] = 2.f * g0 + (g0 * g0 + g1 * g1 + g2 * g2);
+  X[1] = 2.f * g4 + (g3 * g3 + g4 * g4 + g5 * g5);
+  X[2] = 2.f * g8 + (g6 * g6 + g7 * g7 + g8 * g8);
+  X[3] = (g1 + g3) + (g0 * g3 + g1 * g4 + g2 * g5);
+  X[4] = (g2 + g6) + (g0 * g6 + g1 * g7 + g2 * g8);
+  X[5] = (g5 + g7) + (g3 * g6 + g4 * g7 + g5 * g8);
+  const float r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + 2.f * (X[3] * X[3] + X[4] * X[4] + X[5] * X[5]);
+  if (!(r2 <= 1.f / 1024.f)) return false;
+  // P = X/5 - I/4, then P <- X P + c I for c = 1/3, -1/2, 1; log = X P.
+  float P[6] = {0.2f * X[0] - 0.25f, 0.2f * X[1] - 0.25f, 0.2f * X[2] - 0.25f, 0.2f * X[3], 0.2f * X[4], 0.2f * X[5]};
+  auto mul = [&](const float A[6], float c, float R[6]) {
+    R[0] = X[0] * A[0] + X[3] * A[3] + X[4] * A[4] + c;
+    R[1] = X[3] * A[3] + X[1] * A[1] + X[5] * A[5] + c;
+    R[2] = X[4] * A[4] + X[5] * A[5] + X[2] * A[2] + c;
+    R[3] = X[0] * A[3] + X[3] * A[1] + X[4] * A[5];
+    R[4] = X[0] * A[4] + X[3] * A[5] + X[4] * A[2];
+    R[5] = X[3] * A[4] + X[1] * A[5] + X[5] * A[2];
+  };
+  float Q[6];
+  mul(P, 1.f / 3.f, Q);
+  mul(Q, -0.5f, P);
+  mul(P, 1.f, Q);
+  mul(Q, 0.f, P);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) eps[k] = 0.5f * P[k];
+  return true;
+}
+
+// Hencky stress 2 mu eps + lambda tr(eps) I of a symmetric strain.
+__device__ __forceinline__ void hencky_stress(const float eps[6], const DevMaterial& m, float tau[6]) {
+  const float tr = eps[0] + eps[1] + eps[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) tau[k] = 2.f * m.mu * eps[k] + m.lambda * tr;
+#pragma unroll
+  for (int k = 3; k < 6; ++k) tau[k] = 2.f * m.mu * eps[k];
+}
+
+// Elastic step of the radial-return models 4-7 and 10 (those whose return
+// leaves the state alone when q <= the flow stress at zero increment), from
+// the series strain: when the trial state is inside the yield surface the
+// return keeps every principal strain, so F and ep stay and the stress is
+// the Hencky stress of F's own strain. Same q, J and yield test as
+// return_map. Returns false (nothing written) when the strain is too large
+// for the series or the state yields; the caller then takes the full return.
+__device__ __forceinline__ bool elastic_return(const M3& F, int model, float ep, float T, const DevMaterial& m,
+                                               float tau[6]) {
+  float eps[6];
+  if (!hencky_small(F, eps)) return false;
+  const float tr = eps[0] + eps[1] + eps[2];
+  const float mean = tr * (1.f / 3.f);
+  const float d0 = eps[0] - mean, d1 = eps[1] - mean, d2 = eps[2] - mean;
+  const float length = sqrtf(d0 * d0 + d1 * d1 + d2 * d2 + 2.f * (eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5]));
+  const float J = fmaxf(1e-6f, expf(tr));
+  const float q = 2.4494897427831781f * m.G * length / J;
+  const float y = model == 10 ? thermal_strength(ep, T, m) : flow_stress(model, ep, 0.f, T, m);
+  if (!(q <= y)) return false;
+  hencky_stress(eps, m, tau);
+  return true;
+}
+
 // Plastic projection of F (models 2-11): SVD, return on the principal
 // stretches, recompose F, and the stress of the returned state on the
 // returned principal directions as the next step's stress (6-vector):
@@ -383,6 +457,9 @@ __device__ __forceinline__ float pressure_return(float e[3], int model, float& j
 template <bool PRESSURE = true>
 __device__ __forceinline__ float plastic_project(M3& F, int model, float ep, float& jp, float& dil, float dt,
                                                  float T, const DevMaterial& m, float tau[6]) {
+  if ((model >= 4 && model <= 7) || model == 10) {
+    if (elastic_return(F, model, ep, T, m, tau)) return 0.f;
+  }
   float U[9], V[9], s[3];
   svd3(F, U, V, s);
   float sp[3];
@@ -458,6 +535,11 @@ __device__ __forceinline__ float plastic_project(M3& F, float ep, float dt, floa
 // Hencky stress 2 mu log(sigma) + lambda tr from F's own decomposition (the
 // first-step stress of a plastic model before any return has been cached).
 __device__ __forceinline__ void hencky_tau(const M3& F, const DevMaterial& m, float tau[6]) {
+  float eps[6];
+  if (hencky_small(F, eps)) {
+    hencky_stress(eps, m, tau);
+    return;
+  }
   float U[9], V[9], s[3];
   svd3(F, U, V, s);
   float e[3];
