@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--sort-every", type=int, default=int(os.environ.get("MPM_SORT_EVERY", "16")))
+    p.add_argument("--sort-every", type=int, default=int(os.environ.get("MPM_SORT_EVERY", "32")))
     p.add_argument("--cpu-steps", type=int, default=8)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")

@@ -561,7 +561,7 @@ int mpm_batch_set_tuning(mpm_batch* b, const mpm_batch_tuning* t) {
 // with events on the engine stream and keeps the fastest. Per size bucket,
 // the particles per workgroup in {64, 128, 256} are timed on that bucket's
 // particle launches alone (one G2P+P2G launch per trial step after a warm
-// step); then the re-bin period in {4, 8, 16, 32} is timed on whole steps.
+// step); then the re-bin period in {4, 8, 16, 32, 64} is timed on whole steps.
 int mpm_batch_tune(mpm_batch* b, int32_t trial_steps, mpm_batch_tuning* out) {
   if (!b || trial_steps < 1) return bfail(b, MPM_ERR_INVALID_ARGUMENT, "trial_steps must be >= 1");
   for (int32_t s = 0; s < b->S; ++s)
@@ -619,7 +619,7 @@ int mpm_batch_tune(mpm_batch* b, int32_t trial_steps, mpm_batch_tuning* out) {
   if (rc) return rc;
   int best_p = b->sort_every;
   double best_ms = 1e300;
-  for (int P : {4, 8, 16, 32}) {
+  for (int P : {4, 8, 16, 32, 64}) {
     const int steps = std::max(P, (trial_steps + P - 1) / P * P);
     BHIP(b, hipEventRecord(e0, b->stream));
     for (int s = 0; s < steps; s += P) period(b, P);

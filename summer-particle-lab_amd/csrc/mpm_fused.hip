@@ -38,6 +38,7 @@ constexpr int TR_ROWS = 9;         // per-wave transpose: one row per node of a 
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 entry: rows start in different banks
 constexpr int CHUNK = 16;          // lanes per scatter chunk (run pieces summed by one lane)
 constexpr int TR_TAIL = 16;        // zero entries after the last row
+constexpr int SORT_PASSES = 8;     // cells of a wave grouped by minimum passes before P2G
 constexpr int TR_ENTRIES = TR_ROWS * TR_STRIDE + TR_TAIL;
 constexpr int WAVE_BYTES = TR_ENTRIES * 16 + 64 * 16;
 static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0, "surrogate activations must fit the wave buffer");
@@ -432,16 +433,32 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const float Qf = valid ? rintf(m * sp.mass_scale) : 0.f;
   const int Q = (int)Qf;
 
-  // Runs of equal base cell in this wave (particles are cell-sorted, so the
-  // lanes of one run are contiguous), cut into chunks of at most CHUNK lanes.
+  // Base cell of this lane (the linear index of its base node) and the
+  // lane's position in the wave's cell order: lanes are grouped by cell
+  // (ascending, the lanes of one cell in lane order), so each cell's
+  // particles form one run however far the slot order has drifted since the
+  // last re-bin. Up to SORT_PASSES cells are placed by minimum passes; the
+  // lanes left over, then the lanes past the end, follow in lane order. Any
+  // order gives the same sums (a run only has to hold one cell); grouping
+  // only cuts the number of chunks.
   const uint32_t key = valid ? ((uint32_t)b[2] * (uint32_t)N + (uint32_t)b[1]) * (uint32_t)N + (uint32_t)b[0]
                              : 0xffffffffu - lane;
-  const uint32_t kprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, DPP_WAVE_SHR1, 0xf, 0xf, false);
-  const uint64_t run_starts = __ballot(lane == 0u || kprev != key);
-  const uint64_t upto = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
-  const int run_start = 63 - __clzll(run_starts & upto);
-  const uint64_t chunk_starts = __ballot(((int)lane - run_start) % CHUNK == 0);
-  const int nchunks = __popcll(chunk_starts);
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint64_t valid_lanes = __ballot(valid);
+  int dest = -1, placed = 0;
+  {
+    uint64_t rem = valid_lanes;
+    for (int pass = 0; pass < SORT_PASSES && rem; ++pass) {
+      const bool mine = (rem >> lane) & 1ull;
+      const uint32_t kmin = (uint32_t)wave_extreme<false>(mine ? (int)key : 0x7fffffff);
+      const uint64_t cell = __ballot(mine && key == kmin);
+      if (mine && key == kmin) dest = placed + __popcll(cell & below);
+      placed += __popcll(cell);
+      rem &= ~cell;
+    }
+    const uint64_t rest = ~__ballot(dest >= 0);
+    if (dest < 0) dest = placed + __popcll(rest & below);
+  }
 
   int lo[3], hi[3];
   block_box<S::WAVES>(b, valid, lo, hi, s_box);
@@ -473,14 +490,40 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   unsigned char* wbuf = s_raw + S::ACC_BYTES + (tid >> 6) * WAVE_BYTES;
   float4* s_tr = reinterpret_cast<float4*>(wbuf);
   int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ENTRIES * 16);
+
+  // The cell order as seen from each position: position p reads the key and
+  // box base of the lane placed at p (through the first 512 bytes of the
+  // transpose rows, free until the first slab is written). Valid lanes take
+  // positions [0, count).
+  int2* s_sort = reinterpret_cast<int2*>(wbuf);
+  s_sort[dest] = make_int2((int)key, lbase);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int2 at = s_sort[lane];
+  const uint32_t skey = (uint32_t)at.x;
+  const bool svalid = (int)lane < __popcll(valid_lanes);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // Runs of equal cell in position order, cut into chunks of at most CHUNK
+  // positions.
+  const uint32_t kprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)skey, DPP_WAVE_SHR1, 0xf, 0xf, false);
+  const uint64_t run_starts = __ballot(lane == 0u || kprev != skey);
+  const uint64_t upto = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
+  const int run_start = 63 - __clzll(run_starts & upto);
+  const uint64_t chunk_starts = __ballot(((int)lane - run_start) % CHUNK == 0);
+  const int nchunks = __popcll(chunk_starts);
+
   if (lane < (uint32_t)(TR_ROWS + TR_TAIL))
     s_tr[lane < (uint32_t)TR_ROWS ? lane * TR_STRIDE + 64 : TR_ROWS * TR_STRIDE + (lane - TR_ROWS)] =
         make_float4(0.f, 0.f, 0.f, 0.f);
   if ((chunk_starts >> lane) & 1ull) {
-    const int c = __popcll(chunk_starts & (upto >> 1));  // chunks before this lane
+    const int c = __popcll(chunk_starts & (upto >> 1));  // chunks before this position
     const uint64_t later = chunk_starts & ~upto;
     const int next = later ? __ffsll((long long)later) - 1 : 64;
-    s_chunk[c] = make_int4((int)lane, next - 1, lbase, valid ? gbase : -1);
+    s_chunk[c] = make_int4((int)lane, next - 1, at.y, svalid ? (int)skey : -1);
   }
 
   // Sum of one (node, chunk) task of slab k, added to the node box (or to
@@ -574,7 +617,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       float pc[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) pc[c] = wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
-      s_tr[nn * TR_STRIDE + lane] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
+      s_tr[nn * TR_STRIDE + dest] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
