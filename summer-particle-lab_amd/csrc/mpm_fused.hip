@@ -243,61 +243,69 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     }
     if (gather) {
       float V[3] = {0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0}, Z[3] = {0, 0, 0};
+      // One x-row of three nodes at (j, k): the weighted sums of v and of
+      // its first moments.
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      auto add_row = [&](f4 n0, f4 n1, f4 n2, int j, int k) {
+        const float wjk = w[1][j] * w[2][k];
+        const float r0x = w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x;
+        const float r0y = w[0][0] * n0.y + w[0][1] * n1.y + w[0][2] * n2.y;
+        const float r0z = w[0][0] * n0.z + w[0][1] * n1.z + w[0][2] * n2.z;
+        const float r1x = w[0][1] * n1.x + 2.f * w[0][2] * n2.x;
+        const float r1y = w[0][1] * n1.y + 2.f * w[0][2] * n2.y;
+        const float r1z = w[0][1] * n1.z + 2.f * w[0][2] * n2.z;
+        const float ax = wjk * r0x, ay = wjk * r0y, az = wjk * r0z;
+        V[0] += ax;
+        V[1] += ay;
+        V[2] += az;
+        X[0] += wjk * r1x;
+        X[1] += wjk * r1y;
+        X[2] += wjk * r1z;
+        if (j) {
+          Y[0] += (float)j * ax;
+          Y[1] += (float)j * ay;
+          Y[2] += (float)j * az;
+        }
+        if (k) {
+          Z[0] += (float)k * ax;
+          Z[1] += (float)k * ay;
+          Z[2] += (float)k * az;
+        }
+      };
+      if (staged) {
+        // Through an LDS-qualified pointer, so the reads are ds_read (a
+        // pointer shared with the global path would make them flat loads).
+        typedef const __attribute__((address_space(3))) f4 lds_f4;
+        lds_f4* sv = (lds_f4*)s_vel;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          float4 n0, n1, n2;
-          if (staged) {
+          for (int j = 0; j < 3; ++j) {
             const int r = ((b[2] - lo[2] + k) * ey + (b[1] - lo[1] + j)) * ex + (b[0] - lo[0]);
-            n0 = s_vel[r];
-            n1 = s_vel[r + 1];
-            n2 = s_vel[r + 2];
-          } else {
+            add_row(sv[r], sv[r + 1], sv[r + 2], j, k);
+          }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
             const size_t r = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
-            n0 = vel[r];
-            n1 = vel[r + 1];
-            n2 = vel[r + 2];
+            const f4* gv = reinterpret_cast<const f4*>(vel);
+            add_row(gv[r], gv[r + 1], gv[r + 2], j, k);
           }
-          if constexpr (MODE == 2) {
-            if (dl) {
+      }
+      if constexpr (MODE == 2) {
+        // The gather is linear in the node velocity: the part's grid deltas
+        // add their own weighted sums.
+        if (dl) {
+          const f4* gd = reinterpret_cast<const f4*>(dl);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
               const size_t r = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
-              const float4 d0 = dl[r], d1 = dl[r + 1], d2 = dl[r + 2];
-              n0.x += d0.x;
-              n0.y += d0.y;
-              n0.z += d0.z;
-              n1.x += d1.x;
-              n1.y += d1.y;
-              n1.z += d1.z;
-              n2.x += d2.x;
-              n2.y += d2.y;
-              n2.z += d2.z;
+              add_row(gd[r], gd[r + 1], gd[r + 2], j, k);
             }
-          }
-          const float wjk = w[1][j] * w[2][k];
-          const float r0x = w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x;
-          const float r0y = w[0][0] * n0.y + w[0][1] * n1.y + w[0][2] * n2.y;
-          const float r0z = w[0][0] * n0.z + w[0][1] * n1.z + w[0][2] * n2.z;
-          const float r1x = w[0][1] * n1.x + 2.f * w[0][2] * n2.x;
-          const float r1y = w[0][1] * n1.y + 2.f * w[0][2] * n2.y;
-          const float r1z = w[0][1] * n1.z + 2.f * w[0][2] * n2.z;
-          const float ax = wjk * r0x, ay = wjk * r0y, az = wjk * r0z;
-          V[0] += ax;
-          V[1] += ay;
-          V[2] += az;
-          X[0] += wjk * r1x;
-          X[1] += wjk * r1y;
-          X[2] += wjk * r1z;
-          if (j) {
-            Y[0] += (float)j * ax;
-            Y[1] += (float)j * ay;
-            Y[2] += (float)j * az;
-          }
-          if (k) {
-            Z[0] += (float)k * ax;
-            Z[1] += (float)k * ay;
-            Z[2] += (float)k * az;
-          }
         }
       }
       const float s4 = 4.f * invH;
