@@ -502,8 +502,9 @@ int mpm_surrogate_counts(mpm_engine* e, uint64_t* used, uint64_t* elastic, uint6
 /* ---- batched scenes (no reference counterpart) ----
  *
  * Many independent scenes advanced together: one particle pool and one
- * concatenated grid on one GPU, one persistent particle launch per size
- * bucket and one grid launch per step for all scenes, and optionally the
+ * concatenated grid on one GPU, one particle launch per size bucket (a
+ * workgroup per chunk of 64-256 particles) and one grid launch per step for
+ * all scenes, and optionally the
  * whole re-bin period replayed as a captured HIP graph. The reference runs
  * one scene per Worker or Python process (gpu/server.py:34-40); these calls
  * batch what would be many MPM3 instances. Scenes share one material table;
@@ -514,8 +515,8 @@ typedef struct mpm_batch mpm_batch;
 typedef struct {
   int32_t sort_every;      /* re-bin period in steps (>= 1; default 16) */
   int32_t use_graph;       /* 1: replay whole re-bin periods as HIP graphs */
-  int32_t particle_blocks; /* workgroups of the persistent particle launch
-                              (0: three per compute unit) */
+  int32_t particle_blocks; /* ignored (kept for ABI compatibility): the
+                              particle launch runs one workgroup per chunk */
 } mpm_batch_options;
 
 /* Launch configuration: scenes are bucketed by grid edge (nodes per axis
@@ -527,7 +528,8 @@ typedef struct {
   int32_t bucket_scenes[4];
   int32_t particles_per_group[4];
   int32_t sort_every;
-  int32_t particle_blocks;
+  int32_t particle_blocks;  /* reported: particle workgroups per step (one per
+                               chunk); ignored by mpm_batch_set_tuning */
   double ms_per_step;   /* from mpm_batch_tune: whole-step time of the choice */
   double bucket_ms[4];  /* from mpm_batch_tune: particle launch time per step */
 } mpm_batch_tuning;

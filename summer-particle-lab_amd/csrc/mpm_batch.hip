@@ -8,12 +8,12 @@
 // sort per re-bin orders the whole pool by node0_s + cell key, so every
 // scene stays in its own slot range, cell-sorted, ties by original id.
 //
-// Per step: one persistent particle launch per size bucket (scenes grouped by
-// grid edge; each bucket has its own particles-per-workgroup, chosen by the
-// autotuner) and one grid launch over all scenes. A whole re-bin period
-// (re-bin, P2G, P-1 fused steps, grid, G2P) can be captured once as a HIP
-// graph and replayed; launches clear no memory with memset, and the
-// persistent launches reset their own work counters.
+// Per step: one particle launch per size bucket (scenes grouped by grid
+// edge; each bucket has its own particles-per-workgroup, chosen by the
+// autotuner; one workgroup per chunk) and one grid launch over all scenes. A
+// whole re-bin period (re-bin, P2G, P-1 fused steps, grid, G2P) can be
+// captured once as a HIP graph and replayed; launches clear no memory with
+// memset.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,12 +62,10 @@ struct mpm_batch {
     int fb = 256;
     uint2* chunks = nullptr;
     uint32_t nchunks = 0;
-    unsigned int* work = nullptr;
     uint32_t scenes = 0;
   } bucket[NBUCKETS];
   uint32_t* d_block_scene = nullptr;
   uint32_t total_n = 0, total_nodes = 0, total_blocks = 0, key_bits = 0;
-  uint32_t particle_blocks = 0;
   uint32_t grid_blocks = 1;
   int sort_every = 16;
   int steps_since_sort = 0;
@@ -143,12 +141,12 @@ int build_chunks(mpm_batch* b) {
 }
 
 BatchArgs bin_args(const mpm_batch* b) {
-  return BatchArgs{b->d_scenes, b->d_bin_chunks, b->n_bin_chunks, b->total_nodes, nullptr};
+  return BatchArgs{b->d_scenes, b->d_bin_chunks, b->n_bin_chunks, b->total_nodes};
 }
 
 BatchArgs bucket_args(const mpm_batch* b, int k) {
   const auto& bk = b->bucket[k];
-  return BatchArgs{b->d_scenes, bk.chunks, bk.nchunks, b->total_nodes, bk.work};
+  return BatchArgs{b->d_scenes, bk.chunks, bk.nchunks, b->total_nodes};
 }
 
 void rebin(mpm_batch* b) {
@@ -162,7 +160,7 @@ void rebin(mpm_batch* b) {
 void particles(mpm_batch* b, int mode) {
   for (int k = 0; k < NBUCKETS; ++k)
     if (b->bucket[k].nchunks)
-      launch_fused_batch(b->stream, b->d, bucket_args(b, k), b->particle_blocks, mode, b->bucket[k].fb);
+      launch_fused_batch(b->stream, b->d, bucket_args(b, k), mode, b->bucket[k].fb);
 }
 
 void grid(mpm_batch* b) {
@@ -266,7 +264,6 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   hipError_t st = hipGetDevice(&b->device);
   int cus = 256;
   if (st == hipSuccess) st = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-  b->particle_blocks = (uint32_t)(opt && opt->particle_blocks > 0 ? opt->particle_blocks : 3 * cus);
   if (st == hipSuccess) st = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   DevState& d = b->d;
   const uint32_t cap = std::max<uint32_t>(b->total_n, 1);
@@ -299,7 +296,6 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
   alloc((void**)&b->d_scenes, sizeof(SceneDesc) * nscenes);
   alloc((void**)&b->d_block_scene, sizeof(uint32_t) * b->total_blocks);
-  for (auto& bk : b->bucket) alloc((void**)&bk.work, sizeof(unsigned int));
   if (st == hipSuccess)
     st = hipHostMalloc((void**)&b->red_host, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes, hipHostMallocDefault);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nt, b->stream);
@@ -311,8 +307,6 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   if (st == hipSuccess) st = hipMemsetAsync(d.piece_counters, 0, sizeof(unsigned long long) * 3, b->stream);
   b->grid_blocks = (uint32_t)std::max(1, std::min<int>(cus, (int)((Nt / 64 + 3) / 4)));
   if (st == hipSuccess) st = hipMemsetAsync(d.planes, 0, sizeof(float) * NPLANES * (size_t)cap, b->stream);
-  for (auto& bk : b->bucket)
-    if (st == hipSuccess) st = hipMemsetAsync(bk.work, 0, sizeof(unsigned int), b->stream);
   std::vector<uint32_t> block_scene(b->total_blocks);
   for (int32_t s = 0; s < nscenes; ++s) {
     const SceneDesc& sd = b->sc[s];
@@ -349,10 +343,8 @@ int mpm_batch_destroy(mpm_batch* b) {
                   d.piece_list, d.piece_counters};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto& bk : b->bucket) {
+  for (auto& bk : b->bucket)
     if (bk.chunks) (void)hipFree(bk.chunks);
-    if (bk.work) (void)hipFree(bk.work);
-  }
   if (b->red_host) (void)hipHostFree(b->red_host);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -538,7 +530,8 @@ int mpm_batch_get_tuning(const mpm_batch* b, mpm_batch_tuning* out) {
   }
   for (int32_t s = 0; s < b->S; ++s) out->bucket_scenes[bucket_of(b->sc[s].g.N)]++;
   out->sort_every = b->sort_every;
-  out->particle_blocks = (int32_t)b->particle_blocks;
+  out->particle_blocks = 0;  // one workgroup per chunk: the chunk counts
+  for (const auto& bk : b->bucket) out->particle_blocks += (int32_t)bk.nchunks;
   return MPM_OK;
 }
 
@@ -553,7 +546,6 @@ int mpm_batch_set_tuning(mpm_batch* b, const mpm_batch_tuning* t) {
   if (t->particle_blocks < 0) return bfail(b, MPM_ERR_INVALID_ARGUMENT, "particle_blocks must be >= 0");
   for (int k = 0; k < NBUCKETS; ++k) b->bucket[k].fb = t->particles_per_group[k];
   b->sort_every = t->sort_every;
-  if (t->particle_blocks > 0) b->particle_blocks = (uint32_t)t->particle_blocks;
   return build_chunks(b);
 }
 
@@ -594,7 +586,7 @@ int mpm_batch_tune(mpm_batch* b, int32_t trial_steps, mpm_batch_tuning* out) {
         for (int o = 0; o < NBUCKETS; ++o) {
           if (!b->bucket[o].nchunks) continue;
           const LaunchEvents le = o == k ? LaunchEvents{ev[2 * t], ev[2 * t + 1]} : LaunchEvents{};
-          launch_fused_batch(b->stream, b->d, bucket_args(b, o), b->particle_blocks, 1, b->bucket[o].fb, le);
+          launch_fused_batch(b->stream, b->d, bucket_args(b, o), 1, b->bucket[o].fb, le);
         }
       }
       grid(b);

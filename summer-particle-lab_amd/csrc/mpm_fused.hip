@@ -142,6 +142,18 @@ __device__ __forceinline__ bool box_inside(const int lo[3], const int hi[3], int
   return ok;
 }
 
+// A wave-uniform value moved into scalar registers, four bytes at a time.
+template <typename T>
+__device__ __forceinline__ T to_scalar(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "whole words only");
+  T out;
+  const int* src = reinterpret_cast<const int*>(&v);
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return out;
+}
+
 // Material record, read through the scalar path when the wave shares it.
 __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restrict__ mats, uint32_t id) {
   const uint32_t first = __builtin_amdgcn_readfirstlane(id);
@@ -674,10 +686,10 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                   acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, deltas, delta_slot);
 }
 
-// Many scenes in one persistent launch: each workgroup takes chunks from a
-// shared counter until the list is exhausted (every workgroup reaches the
-// exit); the workgroup that takes the last ticket resets the counter for the
-// next launch, so no clearing operation sits between launches.
+// Many scenes in one launch: one workgroup per chunk of the bucket's chunk
+// list (chunks never straddle scenes). A persistent form (workgroups taking
+// chunks from a counter) kept the loop's invariants live across chunks and
+// needed about 45 more VGPRs, which spilled at three waves per SIMD.
 template <int MODE, int FBK, bool EXT>
 __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_batch(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
@@ -685,24 +697,21 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FBK>::LDS_BYTES];
   __shared__ int s_box[6 * Shape<FBK>::WAVES];
-  __shared__ uint32_t s_ticket;
-  for (;;) {
-    __syncthreads();  // the previous chunk is done with LDS and s_ticket
-    if (threadIdx.x == 0) {
-      const uint32_t t = atomicAdd(b.work, 1u);
-      if (t == b.nchunks + gridDim.x - 1) atomicExch(b.work, 0u);
-      s_ticket = t;
-    }
-    __syncthreads();
-    const uint32_t t = __builtin_amdgcn_readfirstlane(s_ticket);
-    if (t >= b.nchunks) break;
-    const uint2 ch = b.chunks[t];
-    const SceneDesc* sd = b.scenes + ch.x;
-    const uint32_t node0 = sd->node0;
-    fused_chunk<MODE, false, FBK, EXT>(P, meta, cap, ch.y, sd->p0 + sd->n, mats, sd->g, sd->sp, b.node_stride,
-                                  vel + node0, acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> 6),
-                                  nullptr, nullptr, s_raw, s_box);
-  }
+  const uint32_t t = blockIdx.x;
+  if (t >= b.nchunks) return;
+  // The chunk and its scene's parameters are the same for every lane, but
+  // the kernel writes memory, so the compiler reads them with vector loads;
+  // moving them to scalar registers keeps ~20 uniform words and the
+  // addresses built from them out of the VGPR budget.
+  const uint2 ch = to_scalar(b.chunks[t]);
+  const SceneDesc* sd = b.scenes + ch.x;
+  const GridDims g = to_scalar(sd->g);
+  const StepParams sp = to_scalar(sd->sp);
+  const uint32_t node0 = __builtin_amdgcn_readfirstlane(sd->node0);
+  const uint32_t end = __builtin_amdgcn_readfirstlane(sd->p0 + sd->n);
+  fused_chunk<MODE, false, FBK, EXT>(P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0, acc_m + node0,
+                                     acc_p + node0, fixed + node0, pieces + (node0 >> 6), nullptr, nullptr, s_raw,
+                                     s_box);
 }
 
 inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
@@ -738,10 +747,9 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
 #undef MPM_FUSED
 }
 
-void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, uint32_t particle_blocks, int mode,
-                        int fb, LaunchEvents ev) {
+void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, int mode, int fb, LaunchEvents ev) {
   if (!b.nchunks) return;
-  const dim3 grid(std::min(particle_blocks, b.nchunks));
+  const dim3 grid(b.nchunks);
 #define MPM_FUSED_B(M, K, X)                                                                                       \
   hipExtLaunchKernelGGL((k_fused_batch<M, K, X>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta,     \
                         d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces)

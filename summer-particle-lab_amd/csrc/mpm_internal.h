@@ -98,15 +98,13 @@ struct SceneDesc {
 };
 
 // Particle-kernel arguments of a batched launch: the scene table, the chunk
-// list {scene, first slot} (chunks never straddle scenes), the plane stride of
-// the momentum accumulators (total nodes) and the self-resetting work
-// counter of the persistent launch.
+// list {scene, first slot} (chunks never straddle scenes) and the plane
+// stride of the momentum accumulators (total nodes).
 struct BatchArgs {
   const SceneDesc* scenes;
   const uint2* chunks;
   uint32_t nchunks;
   uint32_t node_stride;
-  unsigned int* work;
 };
 
 // Settings of the frame wrapper as the kernels see them.
@@ -140,12 +138,11 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
                   LaunchEvents ev = {});
 uint32_t grid_pieces(const GridDims& g);
 // Batched launchers (mpm_kernels.hip, mpm_fused.hip): all scenes of a batch
-// in one launch; `particle_blocks` is the persistent grid size.
+// in one launch; the particle launch runs one workgroup per chunk.
 void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b);
 void launch_grid_batch(hipStream_t s, const DevState& d, const SceneDesc* scenes, const uint32_t* block_scene,
                        uint32_t node_stride, uint32_t blocks, LaunchEvents ev = {});
-void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, uint32_t particle_blocks, int mode,
-                        int fb, LaunchEvents ev = {});
+void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, int mode, int fb, LaunchEvents ev = {});
 // Frame wrapper kernels (mpm_kernels.hip). `ox` and `disp` are 3 planes of
 // cap floats; cell_start is N^3 entries; maxbits one word.
 void launch_frame_pre(hipStream_t s, const DevState& d, uint32_t n, float* ox, const FrameDev& f, float dt);
