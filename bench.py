@@ -18,6 +18,9 @@ Extra fields:
                 a batch of its own, then 300 steps timed with graph replay and
                 the same without graphs (each from the fresh state), and the
                 same scenes stepped one after another by single-scene engines.
+  solid2d       the 2D SolidMPM drop-in at the reference's particle cap
+                (14,400 particles, frames of 1/60 s through the device frame
+                loop), with the FP64 2D oracle timed on the same scene.
   surrogate     the same workload with the learned constitutive surrogate:
                 network trained on the device first (timed separately), then
                 K steps timed as above; its stress error on held-out trial
@@ -59,6 +62,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-2d", action="store_true", help="skip the 2D SolidMPM leg")
+    p.add_argument("--frames-2d", type=int, default=120)
     p.add_argument("--large", action="store_true",
                    help="also run the large single scene (256^3 cells, ~16.5M particles), plain and surrogate")
     p.add_argument("--large-steps", type=int, default=1000)
@@ -265,6 +270,53 @@ def surrogate_leg(sc, args, plain_value):
                 kernels={k: dict(launches=c, avg_us=round(ms / c * 1e3, 2)) for k, (c, ms) in times.items()})
 
 
+def solid2d_leg(args):
+    """The 2D SolidMPM drop-in (src/solid-mpm.js) on a scene at the reference's
+    particle cap: four 216 x 216 panels at the reference's spacing 3.6
+    (14,400 particles; the 2D driver holds at most 16,000,
+    src/simulation.js:7), centre gravity, the default walls. Frames of 1/60 s
+    through the device frame loop (stable substeps as the driver takes them),
+    the FP64 2D oracle timed on the first frames beside it."""
+    import numpy as np
+    from mpm_amd.solid2d import Solid2D, panel, settings
+    pts = [panel(x0, y0, x0 + 216, y0 + 216)[0] for x0 in (-420, 60) for y0 in (-420, 60)]
+    x = np.concatenate(pts)
+    n = len(x)
+    s = settings()
+    e = Solid2D(n)
+    e.upload(x)
+    e.advance(1 / 60, "center", s=s)
+    e.stats()
+    t0 = time.perf_counter()
+    substeps = 0
+    frames = args.frames_2d
+    for _ in range(frames):
+        k, _ = e.advance(1 / 60, "center", s=s)
+        substeps += k
+    st = e.stats()  # reads back: the frames are done
+    el = time.perf_counter() - t0
+    assert st["grid_mass_q"] == st["particle_mass_q"] and st["min_J"] > 0, st
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as o
+        sc = o.Scene2(x, walls="default")
+        sc.advance(1 / 60, gravity=2)
+        cpu_frames, cpu_sub = 3, 0
+        t1 = time.perf_counter()
+        for _ in range(cpu_frames):
+            k, _ = sc.advance(1 / 60, gravity=2)
+            cpu_sub += k
+        el_cpu = time.perf_counter() - t1
+        cpu = dict(value=round(n * cpu_sub / el_cpu, 1), unit="particle-substeps/s", cores=1, kind="port",
+                   sample=f"{cpu_frames} frames ({cpu_sub} substeps) of the same scene, FP64 2D oracle",
+                   seconds=round(el_cpu, 3))
+    return dict(workload=f"2D SolidMPM: 4 panels, {n} particles at spacing 3.6, 149^2 grid, centre gravity, "
+                         f"default walls, {frames} frames of 1/60 s", particles=n, frames=frames,
+                substeps=substeps, value=round(n * substeps / el, 1), unit="particle-substeps/s",
+                ms_per_frame=round(el / frames * 1e3, 4), us_per_substep=round(el / max(substeps, 1) * 1e6, 2),
+                cpu_baseline=cpu)
+
+
 def cpu_baseline(sc, steps):
     """Time the FP64 oracle (one thread) on `steps` steps of the same scene."""
     import numpy as np
@@ -339,6 +391,7 @@ def main():
     sur = None if args.no_surrogate else surrogate_leg(sc, args, value)
     batched = None if args.no_batched else batched_leg(args)
     large = large_leg(args) if args.large else None
+    solid2d = None if args.no_2d else solid2d_leg(args)
     cpu = None if args.no_cpu else cpu_baseline(sc, args.cpu_steps)
     out = {
         "metric": "particle-steps/s (P2G+grid+G2P), plain vs MLP-constitutive at <=1% stress err",
@@ -362,6 +415,7 @@ def main():
         "surrogate": sur,
         "batched": batched,
         "large": large,
+        "solid2d": solid2d,
     }
     print(json.dumps(out))
 
