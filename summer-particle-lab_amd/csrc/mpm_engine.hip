@@ -127,7 +127,7 @@ void collect_times(mpm_engine* e) {
 }
 
 uint32_t key_bits(int N) {
-  const uint64_t cells = (uint64_t)N * N * N;
+  const uint64_t cells = blocked_key_count(N);
   uint32_t b = 1;
   while (b < 32 && (1ull << b) < cells) ++b;
   return b;
@@ -1075,7 +1075,7 @@ int mpm_advance_frame(mpm_engine* e, const mpm_frame_config* c, double dt, mpm_f
   if (c->anti_clump && h < 2.8)
     return fail(e, MPM_ERR_INVALID_ARGUMENT, "anti-clump spacing rule needs a cell size >= 2.8 (reference units)");
   const uint32_t cap = e->d.cap;
-  const size_t Nn = (size_t)e->g.N * e->g.N * e->g.N;
+  const size_t Nn = (size_t)blocked_key_count(e->g.N);  // cell-start table, indexed by sort key
   if (!e->frame_ox) HIP_OK(e, hipMalloc((void**)&e->frame_ox, sizeof(float) * 3 * (size_t)cap));
   if (c->anti_clump) {
     if (!e->frame_disp) HIP_OK(e, hipMalloc((void**)&e->frame_disp, sizeof(float) * 3 * (size_t)cap));

@@ -83,6 +83,30 @@ struct DevState {
   unsigned long long* sur_counts;
 };
 
+// Sort key of a base cell: cells are ordered by 4x4x4 block (blocks in
+// z, y, x order), then x fastest inside the block. A run of consecutive
+// cells then stays inside one block, so the node box of a chunk of
+// particles is a few cells wide on every axis instead of a strip along x
+// that wraps to the next row. Equal cells get equal keys (the runs of P2G
+// only need that); keys are below blocked_key_count(N).
+__host__ __device__ inline uint32_t blocked_cell_key(int bx, int by, int bz, int N) {
+  const uint32_t nb = ((uint32_t)N + 3u) >> 2;
+  const uint32_t block = (((uint32_t)bz >> 2) * nb + ((uint32_t)by >> 2)) * nb + ((uint32_t)bx >> 2);
+  return (block << 6) | (((uint32_t)bz & 3u) << 4) | (((uint32_t)by & 3u) << 2) | ((uint32_t)bx & 3u);
+}
+__host__ __device__ inline uint64_t blocked_key_count(int N) {
+  const uint64_t nb = ((uint64_t)N + 3u) >> 2;
+  return nb * nb * nb * 64u;
+}
+// Base cell of a blocked key (inverse of blocked_cell_key).
+__host__ __device__ inline void blocked_cell_decode(uint32_t key, int N, int& bx, int& by, int& bz) {
+  const uint32_t nb = ((uint32_t)N + 3u) >> 2;
+  const uint32_t block = key >> 6;
+  bx = (int)(((block % nb) << 2) | (key & 3u));
+  by = (int)((((block / nb) % nb) << 2) | ((key >> 2) & 3u));
+  bz = (int)(((block / (nb * nb)) << 2) | ((key >> 4) & 3u));
+}
+
 constexpr int SUR_SLOTS = 64;
 constexpr int SUM_SLOTS = 16;  // slots per scene of the grid sums (spreads the atomics)
 

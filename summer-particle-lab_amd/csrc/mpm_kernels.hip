@@ -36,9 +36,10 @@ __device__ __forceinline__ int base_of(float x, float O, float invH, int N, floa
   return b;
 }
 
-// Cell key of every slot, written at the slot's original particle id so the
-// stable sort breaks ties by ascending id (the reference's P2G visiting order,
-// src/three/mpm.js:66); the value is the slot the particle lives in.
+// Cell key of every slot (blocked_cell_key), written at the slot's original
+// particle id so the stable sort breaks ties by ascending id (the reference's
+// P2G visiting order within a cell, src/three/mpm.js:66); the value is the
+// slot the particle lives in.
 __global__ __launch_bounds__(BLOCK) void k_bin_keys(const float* __restrict__ P, const uint32_t* __restrict__ ids,
                                                     uint32_t cap, uint32_t n, GridDims g,
                                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_keys(const float* __restrict__ P,
   const int by = base_of(plane(P, cap, P_Y, i), g.O[1], g.invH, g.N, t);
   const int bz = base_of(plane(P, cap, P_Z, i), g.O[2], g.invH, g.N, t);
   const uint32_t id = ids[i];
-  keys[id] = ((uint32_t)bz * (uint32_t)g.N + (uint32_t)by) * (uint32_t)g.N + (uint32_t)bx;
+  keys[id] = blocked_cell_key(bx, by, bz, g.N);
   vals[id] = i;
 }
 
@@ -389,8 +390,9 @@ __global__ __launch_bounds__(BLOCK) void k_frame_post(float* __restrict__ P, con
   }
 }
 
-// First slot of every base cell in the sorted slot order (entries of cells
-// with no particle are left stale; readers check them).
+// First slot of every base cell in the sorted slot order, indexed by the
+// cell's sort key (entries of cells with no particle are left stale; readers
+// check them).
 __global__ __launch_bounds__(BLOCK) void k_cell_starts(const uint32_t* __restrict__ keys, uint32_t n,
                                                        uint32_t* __restrict__ start) {
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -427,15 +429,15 @@ __global__ __launch_bounds__(BLOCK) void k_regularize(const float* __restrict__ 
     const float xi = plane(P, cap, P_X, i), yi = plane(P, cap, P_Y, i), zi = plane(P, cap, P_Z, i);
     const float ri = frame_spacing(P, cap, i, f.vol_unit);
     const uint32_t oi = ids[i];
-    const uint32_t c = keys[i];
-    const int bx = (int)(c % (uint32_t)N), by = (int)((c / (uint32_t)N) % (uint32_t)N), bz = (int)(c / ((uint32_t)N * N));
+    int bx, by, bz;
+    blocked_cell_decode(keys[i], N, bx, by, bz);
     float dx = 0.f, dy = 0.f, dz = 0.f;
     for (int oz = -R; oz <= R; ++oz)
       for (int oy = -R; oy <= R; ++oy)
         for (int oxo = -R; oxo <= R; ++oxo) {
           const int cx = bx + oxo, cy = by + oy, cz = bz + oz;
           if (cx < 0 || cy < 0 || cz < 0 || cx > N - 3 || cy > N - 3 || cz > N - 3) continue;
-          const uint32_t cc = ((uint32_t)cz * N + cy) * N + cx;
+          const uint32_t cc = blocked_cell_key(cx, cy, cz, N);
           uint32_t j = cstart[cc];
           if (j >= n || keys[j] != cc || (j > 0 && keys[j - 1] == cc)) continue;
           for (; j < n && keys[j] == cc; ++j) {

@@ -57,7 +57,10 @@ def cpu_keys(x, lower, upper, h):
     O = origin.astype(np.float32)
     q = (x32 - O) * np.float32(1.0 / h)
     b = np.clip(np.floor(q - np.float32(0.5)).astype(np.int64), 0, N - 3)
-    return ((b[:, 2] * N + b[:, 1]) * N + b[:, 0]).astype(np.uint32)
+    # Cells in 4x4x4 blocks (blocks in z, y, x order), x fastest inside.
+    nb = (N + 3) // 4
+    blk = ((b[:, 2] >> 2) * nb + (b[:, 1] >> 2)) * nb + (b[:, 0] >> 2)
+    return (blk * 64 + (b[:, 2] & 3) * 16 + (b[:, 1] & 3) * 4 + (b[:, 0] & 3)).astype(np.uint32)
 
 
 def test_binning_keys_and_permutation_bit_exact(gpu):
