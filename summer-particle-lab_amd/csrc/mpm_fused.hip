@@ -9,10 +9,11 @@
 //      return for models >= 2 (src/three/mpm.js:126-131) and the stress of the
 //      next step (src/three/mpm.js:33-39);
 //   3. scatter of mass and APIC momentum into the next grid
-//      (src/three/mpm.js:66-85). The node contributions of a wave are
-//      transposed through LDS and summed over each run of particles sharing a
-//      base cell; each run sum is added to an LDS box of nodes, and the box is
-//      added to the global grid with one atomic per node and channel.
+//      (src/three/mpm.js:66-85). The lanes of a wave are grouped by base cell,
+//      their node contributions transposed through LDS in that order and
+//      summed over each run of particles sharing a base cell; each run sum is
+//      added to an LDS box of nodes, and the box is added to the global grid
+//      with one atomic per node and channel.
 // Node mass is 64-bit fixed point: a particle's 27 parts sum exactly to its
 // own fixed-point mass (the centre node takes the remainder), so the grid
 // total equals the particle total bit for bit in any summation order.
