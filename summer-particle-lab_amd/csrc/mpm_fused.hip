@@ -155,6 +155,28 @@ __device__ __forceinline__ T to_scalar(const T& v) {
   return out;
 }
 
+// Box coordinates (lx, ly, lz) of entry p of an ex x ey x ez box, without
+// integer division. For 0 <= p < 2^21 and any divisor d >= 1,
+// (p + 1/2) / d lies at least 0.5 / d inside [q, q + 1). The float product
+// (p + 1/2) * (1/d), with the reciprocal within 1 ulp and one rounding, is
+// off by a relative 2^-22 at most, that is by less than
+// (p + 1/2) / d * 2^-22 < 0.5 / d, so its truncation is the exact quotient q.
+// Boxes hold at most a few thousand entries.
+struct BoxIndex {
+  int ex, exy;
+  float inv_ex, inv_exy;
+  __device__ __forceinline__ BoxIndex(int ex_, int ey_) : ex(ex_), exy(ex_ * ey_) {
+    inv_ex = 1.f / (float)ex;
+    inv_exy = 1.f / (float)exy;
+  }
+  __device__ __forceinline__ void operator()(int p, int& lx, int& ly, int& lz) const {
+    lz = (int)(((float)p + 0.5f) * inv_exy);
+    const int rest = p - lz * exy;
+    ly = (int)(((float)rest + 0.5f) * inv_ex);
+    lx = rest - ly * ex;
+  }
+};
+
 // Material record, read through the scalar path when the wave shares it.
 __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restrict__ mats, uint32_t id) {
   const uint32_t first = __builtin_amdgcn_readfirstlane(id);
@@ -239,8 +261,10 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     float4* s_vel = reinterpret_cast<float4*>(s_raw);
     if (staged) {
       const int cnt = ex * ey * ez;
+      const BoxIndex at(ex, ey);
       for (int p = tid; p < cnt; p += FBK) {
-        const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
+        int lx, ly, lz;
+        at(p, lx, ly, lz);
         s_vel[p] = vel[((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx)];
       }
     }
@@ -654,11 +678,13 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   }
   if (staged) {
     __syncthreads();
+    const BoxIndex at(ex, ey);
     for (int p = tid; p < cnt; p += FBK) {
       const unsigned long long mq = s_m[p];
       const float px = s_p[p], py = s_p[S::PATCH_ACC + p], pz = s_p[2 * S::PATCH_ACC + p];
       if (mq != 0ull || px != 0.f || py != 0.f || pz != 0.f) {
-        const int lx = p % ex, ly = (p / ex) % ey, lz = p / (ex * ey);
+        int lx, ly, lz;
+        at(p, lx, ly, lz);
         const size_t node = ((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx);
         atomicAdd(&acc_m[node], mq);
         atomicAdd(&acc_p[node], px);
