@@ -30,6 +30,7 @@
 
 #include "mpm_internal.h"
 #include "mpm_mlp_device.h"
+#include "mpm_wave.h"
 
 namespace mpm {
 namespace {
@@ -75,28 +76,11 @@ __device__ __forceinline__ int base_axis(float x, float O, float invH, int N, fl
   return b;
 }
 
-// DPP controls (GFX9): row shifts within 16-lane rows, the broadcasts of
-// lane 15 of each row to the next row and of lane 31 to rows 2 and 3, and
-// the shift of the whole wave by one lane.
-constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
-constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143, DPP_WAVE_SHR1 = 0x138;
-
 // Minimum (MAX = false) or maximum over the 64 lanes of a wave, all lanes
-// active: lane i folds in lanes i-1, i-3, i-7, i-15 of its row (a source
-// outside the row reads as the identity), so lane 15 of each row holds the
-// row's result; the two broadcasts fold rows 0..2 into lane 63, whose value
-// is returned to every lane.
+// active; the result is wave-uniform.
 template <bool MAX>
 __device__ __forceinline__ int wave_extreme(int x) {
-  constexpr int ID = MAX ? (int)0x80000000 : 0x7fffffff;
-  auto fold = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR1, 0xf, 0xf, false));
-  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR2, 0xf, 0xf, false));
-  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR4, 0xf, 0xf, false));
-  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_SHR8, 0xf, 0xf, false));
-  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_BCAST15, 0xa, 0xf, false));
-  x = fold(x, __builtin_amdgcn_update_dpp(ID, x, DPP_ROW_BCAST31, 0xc, 0xf, false));
-  return __builtin_amdgcn_readlane(x, 63);
+  return wave_reduce(x, MAX ? (int)0x80000000 : 0x7fffffff, [](int a, int b) { return MAX ? max(a, b) : min(a, b); });
 }
 
 // Block-wide box of base cells: lo[a] = min b[a], hi[a] = max b[a] over the

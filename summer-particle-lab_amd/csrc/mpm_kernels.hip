@@ -11,6 +11,7 @@
 #include <cmath>
 
 #include "mpm_internal.h"
+#include "mpm_wave.h"
 
 namespace mpm {
 namespace {
@@ -162,8 +163,8 @@ __device__ __forceinline__ void grid_node(size_t node, size_t Nn, unsigned long 
     v[0] = px * im + g.gravity[0] * gd;
     v[1] = py * im + g.gravity[1] * gd;
     v[2] = pz * im + g.gravity[2] * gd;
-    const int N = g.N;
-    const int ix = (int)(node % N), iy = (int)((node / N) % N), iz = (int)(node / ((size_t)N * N));
+    const uint32_t N = (uint32_t)g.N, nd = (uint32_t)node;  // a scene's N^3 nodes fit 32 bits
+    const int ix = (int)(nd % N), iy = (int)((nd / N) % N), iz = (int)(nd / (N * N));
     const int idx[3] = {ix, iy, iz};
     bool sticky = pin;
 #pragma unroll
@@ -270,15 +271,8 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
       const uint32_t scene = block_scene[gp >> 2];
       if (scene != cur) {
         if (cur != 0xffffffffu) {
-          long long m = Mq;
-          double a = p0, b = p1, c = p2;
-#pragma unroll
-          for (int d = 32; d > 0; d >>= 1) {
-            m += __shfl_down(m, d, 64);
-            a += __shfl_down(a, d, 64);
-            b += __shfl_down(b, d, 64);
-            c += __shfl_down(c, d, 64);
-          }
+          const long long m = wave_sum(Mq);
+          const double a = wave_sum(p0), b = wave_sum(p1), c = wave_sum(p2);
           if (lane == 0) flush_sums(sums, cur, blockIdx.x % SUM_SLOTS, m, a, b, c);
         }
         cur = scene;
@@ -297,13 +291,10 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
       grid_node((size_t)gp * 64 + lane, Nn, acc_m, acc_p, Nn, fixed, vel, g1, sp1, lo, hi, Mq, p0, p1, p2);
     }
   }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    Mq += __shfl_down(Mq, d, 64);
-    p0 += __shfl_down(p0, d, 64);
-    p1 += __shfl_down(p1, d, 64);
-    p2 += __shfl_down(p2, d, 64);
-  }
+  Mq = wave_sum(Mq);
+  p0 = wave_sum(p0);
+  p1 = wave_sum(p1);
+  p2 = wave_sum(p2);
   if (lane == 0) {
     s_scene[wave] = cur;
     s_sum[wave][0] = __longlong_as_double(Mq);
