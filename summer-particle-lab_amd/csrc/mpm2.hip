@@ -22,8 +22,12 @@
 #include <vector>
 
 #include "mpm_engine.h"
+#include "mpm_wave.h"
 
 namespace {
+
+using mpm::wave_reduce;
+using mpm::wave_sum;
 
 constexpr int BLOCK = 256;
 constexpr int N2 = 149;  // nodes per axis (src/solid-mpm.js:4)
@@ -163,9 +167,7 @@ __global__ __launch_bounds__(BLOCK) void k2_grid(unsigned long long* __restrict_
     acc_fix[node] = 0.f;
   }
   const uint64_t b = __ballot(act);
-  unsigned long long s = Mq;
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+  const unsigned long long s = wave_sum(Mq);
   if ((threadIdx.x & 63u) == 0) {
     if (b) atomicAdd(active, (unsigned int)__popcll(b));
     if (s) atomicAdd(mass_q, s);
@@ -235,11 +237,9 @@ __global__ __launch_bounds__(BLOCK) void k2_stable(const float* __restrict__ P, 
     grad = fmaxf(grad, fabsf(ld2(P, cap, Q_C0, p)) + fabsf(ld2(P, cap, Q_C0 + 1, p)) + fabsf(ld2(P, cap, Q_C0 + 2, p)) +
                            fabsf(ld2(P, cap, Q_C0 + 3, p)));
   }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    wave = fmaxf(wave, __shfl_down(wave, d, 64));
-    grad = fmaxf(grad, __shfl_down(grad, d, 64));
-  }
+  const auto fmax_fold = [](float a, float b) { return fmaxf(a, b); };
+  wave = wave_reduce(wave, 0.f, fmax_fold);  // both maxima are >= 0
+  grad = wave_reduce(grad, 0.f, fmax_fold);
   if ((threadIdx.x & 63u) == 0) {
     atomicMax(&bits[0], __float_as_uint(wave));
     atomicMax(&bits[1], __float_as_uint(grad));
@@ -318,13 +318,10 @@ __global__ __launch_bounds__(BLOCK) void k2_stats(const float* __restrict__ P, u
     strain = fmax(strain, .5 * sqrt(e0 * e0 + e1 * e1 + e2 * e2));
     q += (unsigned long long)(long long)(int32_t)rintf(ld2(P, cap, Q_MASS, p) * mass_scale);
   }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    jmin = fmin(jmin, __shfl_down(jmin, d, 64));
-    jmax = fmax(jmax, __shfl_down(jmax, d, 64));
-    strain = fmax(strain, __shfl_down(strain, d, 64));
-    q += __shfl_down(q, d, 64);
-  }
+  jmin = wave_reduce(jmin, (double)INFINITY, [](double a, double b) { return fmin(a, b); });
+  jmax = wave_reduce(jmax, -(double)INFINITY, [](double a, double b) { return fmax(a, b); });
+  strain = wave_reduce(strain, 0.0, [](double a, double b) { return fmax(a, b); });
+  q = wave_sum(q);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
     s[0][wave] = jmin;
