@@ -609,8 +609,12 @@ int mpm2_advance(mpm2_engine* e, const mpm2_settings* s, double dt, int32_t grav
     HIP2(e, hipMemsetAsync(e->counters, 0, sizeof(unsigned int), e->stream));
     HIP2(e, hipMemsetAsync(e->mass_q, 0, sizeof(unsigned long long), e->stream));
     const unsigned sblocks = std::min<unsigned>(RED2_BLOCKS, blocks2(e->n));
+    // The first round enqueues as many substeps as the previous frame took,
+    // plus one, so a steady run of frames costs one host round trip and few
+    // launches that find the loop finished; later rounds enqueue CHUNK.
+    int round = e->substeps ? std::min(CHUNK, (int)e->substeps + 1) : CHUNK;
     for (;;) {
-      for (int k = 0; k < CHUNK; ++k) {
+      for (int k = 0; k < round; ++k) {
         hipLaunchKernelGGL(k2_stable, dim3(sblocks), dim3(BLOCK), 0, e->stream, e->planes, e->cap, e->n, prm, e->loop,
                            &e->loop->wave_bits);
         hipLaunchKernelGGL(k2_begin, dim3(1), dim3(1), 0, e->stream, e->loop, e->counters, e->mass_q);
@@ -622,6 +626,7 @@ int mpm2_advance(mpm2_engine* e, const mpm2_settings* s, double dt, int32_t grav
       HIP2(e, hipMemcpyAsync(e->loop_host, e->loop, sizeof(Loop2), hipMemcpyDeviceToHost, e->stream));
       HIP2(e, hipStreamSynchronize(e->stream));
       if (!(e->loop_host->remaining > 1e-9) || e->loop_host->steps >= max_substeps) break;
+      round = CHUNK;
     }
     init = *e->loop_host;
   }
