@@ -90,13 +90,46 @@ __device__ __forceinline__ float step_dt(float dt, const Loop2* loop) { return l
 // the remainder; a particle outside the grid has clamped stencils whose
 // weights can be negative, and two's-complement sums stay exact), momentum
 // w (m v + A d) with A = m C - dt V0 4/DX^2 tau, and the pinned weight w m.
-__global__ __launch_bounds__(BLOCK) void k2_p2g(const float* __restrict__ P, const uint8_t* __restrict__ pinned,
+// Before the scatter of a frame-loop substep (pre_mode >= 0), the driver's
+// particle update of that substep (src/simulation.js:271-290): pinned
+// particles stop; the others take gravity (mode 1 uniform (0, -1), 2 towards
+// the centre -(x, y) / sqrt(x^2 + y^2 + 400)) times 27 strength over the
+// substep and are clamped to +-180 per axis. Returns the particle's velocity.
+__device__ __forceinline__ float2 frame_pre(float* __restrict__ P, uint32_t cap, uint32_t p, bool pin, int pre_mode,
+                                           float strength, float h) {
+  float vx = ld2(P, cap, Q_VX, p), vy = ld2(P, cap, Q_VY, p);
+  if (pre_mode < 0) return make_float2(vx, vy);
+  if (pin) {
+    vx = vy = 0.f;
+  } else {
+    float gx = 0.f, gy = 0.f;
+    if (pre_mode == 2) {
+      const float x = ld2(P, cap, Q_X, p), y = ld2(P, cap, Q_Y, p);
+      const float ir = 1.f / sqrtf(x * x + y * y + 400.f);
+      gx = -x * ir;
+      gy = -y * ir;
+    } else if (pre_mode == 1) {
+      gy = -1.f;
+    }
+    const float force = 27.f * strength;
+    vx = fminf(fmaxf(vx + gx * force * h, -180.f), 180.f);
+    vy = fminf(fmaxf(vy + gy * force * h, -180.f), 180.f);
+  }
+  st2(P, cap, Q_VX, p, vx);
+  st2(P, cap, Q_VY, p, vy);
+  return make_float2(vx, vy);
+}
+
+__global__ __launch_bounds__(BLOCK) void k2_p2g(float* __restrict__ P, const uint8_t* __restrict__ pinned,
                                                 uint32_t cap, uint32_t n, float dt, const Loop2* loop, Params2 prm,
+                                                int pre_mode, float strength,
                                                 unsigned long long* __restrict__ acc_m, float* __restrict__ acc_u,
                                                 float* __restrict__ acc_v, float* __restrict__ acc_fix) {
   const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
   const float h = step_dt(dt, loop);
   if (p >= n || h == 0.f) return;
+  const bool pin = pinned[p] != 0;
+  const float2 vel = frame_pre(P, cap, p, pin, pre_mode, strength, h);
   float fx, fy, wx[3], wy[3];
   const int bx = stencil_axis(ld2(P, cap, Q_X, p), fx, wx), by = stencil_axis(ld2(P, cap, Q_Y, p), fy, wy);
   const float m = ld2(P, cap, Q_MASS, p), V0 = AREA2 * ld2(P, cap, Q_VOL, p);
@@ -107,8 +140,7 @@ __global__ __launch_bounds__(BLOCK) void k2_p2g(const float* __restrict__ P, con
   const float factor = -h * V0 * 4.f / (DX2 * DX2);
   const float a00 = m * ld2(P, cap, Q_C0, p) + factor * t[0], a01 = m * ld2(P, cap, Q_C0 + 1, p) + factor * t[1];
   const float a10 = m * ld2(P, cap, Q_C0 + 2, p) + factor * t[2], a11 = m * ld2(P, cap, Q_C0 + 3, p) + factor * t[3];
-  const float mvx = m * ld2(P, cap, Q_VX, p), mvy = m * ld2(P, cap, Q_VY, p);
-  const bool pin = pinned[p] != 0;
+  const float mvx = m * vel.x, mvy = m * vel.y;
   const int32_t Q = (int32_t)rintf(m * prm.mass_scale);
   int32_t given = 0;
   int32_t part[9];
@@ -269,35 +301,6 @@ __global__ void k2_begin(Loop2* loop, unsigned int* active, unsigned long long* 
   loop->grad_bits = 0u;
 }
 
-// Before the MPM step of a substep (src/simulation.js:268-285): pinned
-// particles stop; the others take gravity (1 uniform (0, -1), 2 towards the
-// centre -(x, y) / sqrt(x^2 + y^2 + 400)) times 27 strength and are clamped to
-// +-180 per axis.
-__global__ __launch_bounds__(BLOCK) void k2_pre(float* __restrict__ P, const uint8_t* __restrict__ pinned,
-                                                uint32_t cap, uint32_t n, const Loop2* loop, int mode,
-                                                float strength) {
-  const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
-  const float h = (float)loop->h;
-  if (p >= n || h == 0.f) return;
-  if (pinned[p]) {
-    st2(P, cap, Q_VX, p, 0.f);
-    st2(P, cap, Q_VY, p, 0.f);
-    return;
-  }
-  float gx = 0.f, gy = 0.f;
-  if (mode == 2) {
-    const float x = ld2(P, cap, Q_X, p), y = ld2(P, cap, Q_Y, p);
-    const float ir = 1.f / sqrtf(x * x + y * y + 400.f);
-    gx = -x * ir;
-    gy = -y * ir;
-  } else if (mode == 1) {
-    gy = -1.f;
-  }
-  const float force = 27.f * strength;
-  const float vx = ld2(P, cap, Q_VX, p) + gx * force * h, vy = ld2(P, cap, Q_VY, p) + gy * force * h;
-  st2(P, cap, Q_VX, p, fminf(fmaxf(vx, -180.f), 180.f));
-  st2(P, cap, Q_VY, p, fminf(fmaxf(vy, -180.f), 180.f));
-}
 
 // Deformation statistics (src/solid-mpm.js:106-115): per-block min/max det F
 // and max Green strain norm; particle fixed-point mass total.
@@ -412,10 +415,13 @@ void wr2(void* p, int32_t dt, size_t i, double v) {
 }
 
 // One MPM step with a fixed dt, or one substep of the frame loop (loop set).
-void enqueue_step(mpm2_engine* e, const Params2& prm, float dt, const Loop2* loop) {
+// One step: P2G (after the frame loop's particle update when pre_mode >= 0),
+// grid, G2P.
+void enqueue_step(mpm2_engine* e, const Params2& prm, float dt, const Loop2* loop, int pre_mode = -1,
+                  float strength = 0.f) {
   const uint32_t n = e->n;
   hipLaunchKernelGGL(k2_p2g, dim3(blocks2(n)), dim3(BLOCK), 0, e->stream, e->planes, e->pinned, e->cap, n, dt, loop, prm,
-                     e->acc_m, e->acc_u, e->acc_v, e->acc_fix);
+                     pre_mode, strength, e->acc_m, e->acc_u, e->acc_v, e->acc_fix);
   hipLaunchKernelGGL(k2_grid, dim3(blocks2(NODES2)), dim3(BLOCK), 0, e->stream, e->acc_m, e->acc_u, e->acc_v,
                      e->acc_fix, e->vel, loop, prm, e->counters, e->mass_q);
   hipLaunchKernelGGL(k2_g2p, dim3(blocks2(n)), dim3(BLOCK), 0, e->stream, e->planes, e->pinned, e->cap, n, dt, loop, prm,
@@ -618,9 +624,7 @@ int mpm2_advance(mpm2_engine* e, const mpm2_settings* s, double dt, int32_t grav
         hipLaunchKernelGGL(k2_stable, dim3(sblocks), dim3(BLOCK), 0, e->stream, e->planes, e->cap, e->n, prm, e->loop,
                            &e->loop->wave_bits);
         hipLaunchKernelGGL(k2_begin, dim3(1), dim3(1), 0, e->stream, e->loop, e->counters, e->mass_q);
-        hipLaunchKernelGGL(k2_pre, dim3(blocks2(e->n)), dim3(BLOCK), 0, e->stream, e->planes, e->pinned, e->cap, e->n,
-                           e->loop, gravity_mode, strength);
-        enqueue_step(e, prm, 0.f, e->loop);
+        enqueue_step(e, prm, 0.f, e->loop, gravity_mode, strength);
       }
       HIP2(e, hipGetLastError());
       HIP2(e, hipMemcpyAsync(e->loop_host, e->loop, sizeof(Loop2), hipMemcpyDeviceToHost, e->stream));
