@@ -474,8 +474,13 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                              : 0xffffffffu - lane;
   const uint64_t below = (1ull << lane) - 1ull;
   const uint64_t valid_lanes = __ballot(valid);
-  int dest = -1, placed = 0;
-  {
+  // A wave already in cell order (the usual case soon after a re-bin; valid
+  // lanes are a prefix) keeps its lane order and skips the passes.
+  const uint32_t kleft = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, DPP_WAVE_SHR1, 0xf, 0xf, false);
+  const bool in_order = __ballot(valid && lane > 0u && kleft > key) == 0ull;
+  int dest = (int)lane, placed = 0;
+  if (!in_order) {
+    dest = -1;
     uint64_t rem = valid_lanes;
     for (int pass = 0; pass < SORT_PASSES && rem; ++pass) {
       const bool mine = (rem >> lane) & 1ull;
@@ -524,17 +529,20 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // box base of the lane placed at p (through the first 512 bytes of the
   // transpose rows, free until the first slab is written). Valid lanes take
   // positions [0, count).
-  int2* s_sort = reinterpret_cast<int2*>(wbuf);
-  s_sort[dest] = make_int2((int)key, lbase);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int2 at = s_sort[lane];
+  int2 at = make_int2((int)key, lbase);
+  if (!in_order) {
+    int2* s_sort = reinterpret_cast<int2*>(wbuf);
+    s_sort[dest] = at;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    at = s_sort[lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   const uint32_t skey = (uint32_t)at.x;
   const bool svalid = (int)lane < __popcll(valid_lanes);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   // Runs of equal cell in position order, cut into chunks of at most CHUNK
   // positions.
