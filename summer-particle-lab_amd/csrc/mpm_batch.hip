@@ -282,7 +282,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   alloc((void**)&d.acc_p, sizeof(float) * 3 * Nt);
   alloc((void**)&d.vel, sizeof(float4) * Nt);
   alloc((void**)&d.fixed, Nt);
-  alloc((void**)&d.pieces, Nt / 64);
+  alloc((void**)&d.pieces, piece_flag_bytes(Nt / 64));
   alloc((void**)&d.keys, sizeof(uint32_t) * cap);
   alloc((void**)&d.keys_alt, sizeof(uint32_t) * cap);
   alloc((void**)&d.vals, sizeof(uint32_t) * cap);
@@ -302,10 +302,12 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nt, b->stream);
-  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, Nt / 64, b->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, piece_flag_bytes(Nt / 64), b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.piece_counters, 0, sizeof(unsigned long long) * 3, b->stream);
-  b->grid_blocks = (uint32_t)std::max(1, std::min<int>(cus, (int)((Nt / 64 + 3) / 4)));
+  // Eight grid update workgroups per CU: a batch touches pieces of every
+  // scene each step.
+  b->grid_blocks = (uint32_t)std::max(1, std::min<int>(8 * cus, (int)((Nt / 64 + 3) / 4)));
   if (st == hipSuccess) st = hipMemsetAsync(d.planes, 0, sizeof(float) * NPLANES * (size_t)cap, b->stream);
   std::vector<uint32_t> block_scene(b->total_blocks);
   for (int32_t s = 0; s < nscenes; ++s) {

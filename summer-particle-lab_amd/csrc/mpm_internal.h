@@ -151,7 +151,7 @@ void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDim
 void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
 // Grid update: launch_compact lists the pieces P2G flagged (and clears the
 // grid sums), launch_grid / launch_grid_batch update the listed pieces with
-// `blocks` persistent workgroups.
+// `blocks` workgroups, each taking a contiguous range of the list.
 void launch_compact(hipStream_t s, const DevState& d, uint32_t npieces, uint32_t nscenes, LaunchEvents ev = {});
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
                  const int hi_idx[3], uint32_t blocks, LaunchEvents ev = {});
@@ -161,6 +161,9 @@ void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const Step
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode,
                   LaunchEvents ev = {});
 uint32_t grid_pieces(const GridDims& g);
+// Bytes of the piece flag array for `npieces` pieces: padded with zeros to a
+// multiple of 16, since the compaction reads 16 flags per 16-byte load.
+inline size_t piece_flag_bytes(size_t npieces) { return (npieces + 15) & ~(size_t)15; }
 // Batched launchers (mpm_kernels.hip, mpm_fused.hip): all scenes of a batch
 // in one launch; the particle launch runs one workgroup per chunk.
 void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b);

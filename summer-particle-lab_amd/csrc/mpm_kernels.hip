@@ -117,28 +117,64 @@ __device__ __forceinline__ void step_stress(const M3& F, int model, const DevMat
 // so no counter is ever reset; list slots are positions modulo the list
 // capacity (at most every piece is appended in one step).
 
-// Compaction of the piece flags. Block 0 also starts the step's list at the
+// Compaction of the piece flags, G flags per lane. G = 1: a ballot per wave.
+// G = 16, for large grids: one 16-byte load per lane (flags are 0 or 1, so a
+// word's popcount is its number of set flags) and a wave scan, so the single
+// append counter takes 16 times fewer atomics; the flag array is zero-padded
+// to whole 16-byte groups. Either way one counter atomic per wave reserves
+// the wave's list positions. Block 0 also starts the step's list at the
 // previous update's end and clears this step's grid sums.
+template <int G>
 __global__ __launch_bounds__(BLOCK) void k_compact_pieces(uint8_t* __restrict__ pieces, uint32_t npieces,
                                                           uint32_t* __restrict__ list,
                                                           unsigned long long* __restrict__ counters,
                                                           double* __restrict__ sums, uint32_t nsums) {
+  static_assert(G == 1 || G == 16, "1 or 16 flags per lane");
   if (blockIdx.x == 0) {
     for (uint32_t k = threadIdx.x; k < nsums; k += BLOCK) sums[k] = 0.0;
     if (threadIdx.x == 0) counters[1] = counters[2];
   }
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
-  bool act = false;
-  if (i < npieces) {
-    act = pieces[i] != 0;
-    if (act) pieces[i] = 0;
+  if constexpr (G == 1) {
+    bool act = false;
+    if (i < npieces) {
+      act = pieces[i] != 0;
+      if (act) pieces[i] = 0;
+    }
+    const uint64_t m = __ballot(act);
+    unsigned long long base = 0;
+    if (lane == 0 && m) base = atomicAdd(&counters[0], (unsigned long long)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (act) list[(base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))) % npieces] = i;
+  } else {
+    uint4* groups = reinterpret_cast<uint4*>(pieces);  // group i holds flags 16i .. 16i+15
+    uint4 f = make_uint4(0u, 0u, 0u, 0u);
+    if (i < (npieces + 15u) / 16u) f = groups[i];
+    const uint32_t cnt = __popc(f.x) + __popc(f.y) + __popc(f.z) + __popc(f.w);
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(inc, d, 64);
+      if ((int)lane >= d) inc += t;
+    }
+    const uint32_t total = __shfl(inc, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 0 && total) base = atomicAdd(&counters[0], (unsigned long long)total);
+    base = __shfl(base, 0, 64);
+    if (cnt) {
+      uint32_t slot = (uint32_t)((base + (inc - cnt)) % npieces);
+      const uint32_t w[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) {
+          list[slot] = i * 16u + (uint32_t)k;
+          if (++slot == npieces) slot = 0;
+        }
+      }
+      groups[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
   }
-  const uint64_t m = __ballot(act);
-  unsigned long long base = 0;
-  if (lane == 0 && m) base = atomicAdd(&counters[0], (unsigned long long)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (act) list[(base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))) % npieces] = i;
 }
 
 // One node of a listed piece (node index within its scene's grid).
@@ -242,10 +278,12 @@ __device__ __forceinline__ void flush_sums(double* __restrict__ sums, uint32_t s
   atomicAdd(s + 3, p2);
 }
 
-// Wave w of the launch updates the step's list entries w, w + W, ... (W waves
-// in all). Each wave keeps running sums for its current scene and flushes
-// them when the scene changes; at the end the waves of a workgroup combine
-// sums of equal scenes before the atomics.
+// Workgroup b updates a contiguous range of the step's list entries, at
+// least one per wave, its waves taking every (BLOCK / 64)-th entry of it;
+// workgroups past the end exit at once. Neighbouring entries mostly belong to
+// one scene, so a wave keeps running sums for its current scene and flushes
+// them only when the scene changes; at the end the waves of a workgroup
+// combine sums of equal scenes before the atomics.
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
                                                      uint8_t* __restrict__ fixed, float4* __restrict__ vel,
@@ -261,11 +299,15 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
   const unsigned long long start = counters[1], end = counters[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[2] = end;
   const uint32_t count = (uint32_t)(end - start);
-  const uint32_t W = gridDim.x * (BLOCK / 64);
+  constexpr uint32_t WAVES = BLOCK / 64;
+  const uint32_t per = max(WAVES, (count + gridDim.x - 1) / gridDim.x);
+  const uint32_t e0 = blockIdx.x * per;
+  if (e0 >= count) return;  // the same for the whole workgroup
+  const uint32_t e1 = min(count, e0 + per);
   uint32_t cur = 0xffffffffu;
   long long Mq = 0;
   double p0 = 0, p1 = 0, p2 = 0;
-  for (uint32_t e = blockIdx.x * (BLOCK / 64) + wave; e < count; e += W) {
+  for (uint32_t e = e0 + wave; e < e1; e += WAVES) {
     const uint32_t gp = list[(start + e) % npieces];  // piece index in the (concatenated) grid
     if constexpr (BATCH) {
       const uint32_t scene = block_scene[gp >> 2];
@@ -797,9 +839,18 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
 }
 
 void launch_compact(hipStream_t s, const DevState& d, uint32_t npieces, uint32_t nscenes, LaunchEvents ev) {
-  hipExtLaunchKernelGGL(k_compact_pieces, dim3(std::max<uint32_t>(1, (npieces + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
-                        s, ev.start, ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
-                        nscenes * SUM_SLOTS * 4);
+  // A ballot per 64 flags up to 2^17 pieces (at most 2048 counter atomics);
+  // 16 flags per lane above.
+  if (npieces <= (1u << 17)) {
+    hipExtLaunchKernelGGL(k_compact_pieces<1>, dim3(std::max<uint32_t>(1, (npieces + BLOCK - 1) / BLOCK)), dim3(BLOCK),
+                          0, s, ev.start, ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
+                          nscenes * SUM_SLOTS * 4);
+  } else {
+    const uint32_t groups = (npieces + 15) / 16;
+    hipExtLaunchKernelGGL(k_compact_pieces<16>, dim3((groups + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, ev.start,
+                          ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
+                          nscenes * SUM_SLOTS * 4);
+  }
 }
 
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
