@@ -10,7 +10,9 @@
 //
 // Per step: one particle launch per size bucket (scenes grouped by grid
 // edge; each bucket has its own particles-per-workgroup, chosen by the
-// autotuner; one workgroup per chunk) and one grid launch over all scenes. A
+// autotuner; one workgroup per chunk), or a single launch over the chunks of
+// every bucket when the buckets share a particles-per-workgroup (one tail
+// instead of one per bucket), and one grid launch over all scenes. A
 // whole re-bin period (re-bin, P2G, P-1 fused steps, grid, G2P) can be
 // captured once as a HIP graph and replayed; launches clear no memory with
 // memset.
@@ -64,6 +66,11 @@ struct mpm_batch {
     uint32_t nchunks = 0;
     uint32_t scenes = 0;
   } bucket[NBUCKETS];
+  // The chunks of all buckets in scene order, when every non-empty bucket
+  // has the same particles-per-workgroup `all_fb` (else n_all_chunks = 0).
+  uint2* d_all_chunks = nullptr;
+  uint32_t n_all_chunks = 0;
+  int all_fb = 0;
   uint32_t* d_block_scene = nullptr;
   uint32_t total_n = 0, total_nodes = 0, total_blocks = 0, key_bits = 0;
   uint32_t grid_blocks = 1;
@@ -112,12 +119,35 @@ void drop_graphs(mpm_batch* b) {
 int build_chunks(mpm_batch* b) {
   std::vector<uint2> bin;
   std::vector<uint2> per[NBUCKETS];
+  std::vector<uint2> all;
   for (int32_t s = 0; s < b->S; ++s) {
     const SceneDesc& sd = b->sc[s];
     for (uint32_t f = 0; f < sd.n; f += 256) bin.push_back(make_uint2((uint32_t)s, sd.p0 + f));
     const int k = bucket_of(sd.g.N);
     const uint32_t fb = (uint32_t)b->bucket[k].fb;
-    for (uint32_t f = 0; f < sd.n; f += fb) per[k].push_back(make_uint2((uint32_t)s, sd.p0 + f));
+    for (uint32_t f = 0; f < sd.n; f += fb) {
+      per[k].push_back(make_uint2((uint32_t)s, sd.p0 + f));
+      all.push_back(make_uint2((uint32_t)s, sd.p0 + f));
+    }
+  }
+  int all_fb = 0;
+  for (int k = 0; k < NBUCKETS; ++k) {
+    if (per[k].empty()) continue;
+    if (all_fb == 0) {
+      all_fb = b->bucket[k].fb;
+    } else if (all_fb != b->bucket[k].fb) {
+      all_fb = -1;
+    }
+  }
+  if (b->d_all_chunks) BHIP(b, hipFree(b->d_all_chunks));
+  b->d_all_chunks = nullptr;
+  b->n_all_chunks = 0;
+  b->all_fb = 0;
+  if (all_fb > 0) {
+    BHIP(b, hipMalloc((void**)&b->d_all_chunks, sizeof(uint2) * all.size()));
+    BHIP(b, hipMemcpy(b->d_all_chunks, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice));
+    b->n_all_chunks = (uint32_t)all.size();
+    b->all_fb = all_fb;
   }
   if (b->d_bin_chunks) BHIP(b, hipFree(b->d_bin_chunks));
   b->d_bin_chunks = nullptr;
@@ -158,6 +188,11 @@ void rebin(mpm_batch* b) {
 }
 
 void particles(mpm_batch* b, int mode) {
+  if (b->n_all_chunks) {
+    launch_fused_batch(b->stream, b->d, BatchArgs{b->d_scenes, b->d_all_chunks, b->n_all_chunks, b->total_nodes}, mode,
+                       b->all_fb);
+    return;
+  }
   for (int k = 0; k < NBUCKETS; ++k)
     if (b->bucket[k].nchunks)
       launch_fused_batch(b->stream, b->d, bucket_args(b, k), mode, b->bucket[k].fb);
@@ -347,6 +382,7 @@ int mpm_batch_destroy(mpm_batch* b) {
     if (p) (void)hipFree(p);
   for (auto& bk : b->bucket)
     if (bk.chunks) (void)hipFree(bk.chunks);
+  if (b->d_all_chunks) (void)hipFree(b->d_all_chunks);
   if (b->red_host) (void)hipHostFree(b->red_host);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;

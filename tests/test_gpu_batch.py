@@ -7,8 +7,8 @@ Bars:
     chunks; only the order of the FP32 node atomics differs);
   * graph replay of whole re-bin periods equals eager launches, also for an
     odd number of replayed periods (buffer swap tracking): same bar;
-  * every particles-per-workgroup choice (64, 128, 256) gives the same
-    trajectories: same bar;
+  * every particles-per-workgroup choice (64, 128, 256, equal or mixed
+    across size buckets) gives the same trajectories: same bar;
   * fixed-point grid mass equals the particle mass of every scene exactly;
   * the autotuner returns a valid configuration and the batch keeps stepping.
 """
@@ -72,11 +72,13 @@ def test_graph_replay_equals_eager(gpu, small):
         assert c["grid_mass_q"] == c["particle_mass_q"]
 
 
-@pytest.mark.parametrize("fb", [64, 128])
+@pytest.mark.parametrize("fb", [[64, 64, 64], [128, 128, 128], [64, 128, 256]])
 def test_particles_per_group_variants_agree(gpu, small, fb):
+    # Equal sizes take one particle launch over all buckets, mixed sizes one
+    # launch per bucket; the reference batch (256 everywhere) takes one.
     ref, var = small.batch(sort_every=8), small.batch(sort_every=8)
-    var.set_tuning(particles_per_group=[fb, fb, fb])
-    assert var.tuning()["particles_per_group"] == [fb, fb, fb]
+    var.set_tuning(particles_per_group=fb)
+    assert var.tuning()["particles_per_group"] == fb
     x0 = [ref.download(i)["positions"] for i in range(len(small.scenes))]
     ref.step(24)
     var.step(24)
