@@ -503,9 +503,9 @@ int mpm_surrogate_counts(mpm_engine* e, uint64_t* used, uint64_t* elastic, uint6
  *
  * Many independent scenes advanced together: one particle pool and one
  * concatenated grid on one GPU, one particle launch per size bucket (a
- * workgroup per chunk of 64-256 particles) and one grid launch per step for
- * all scenes, and optionally the
- * whole re-bin period replayed as a captured HIP graph. The reference runs
+ * workgroup per chunk of 64-256 particles; a single launch over all buckets
+ * when they share a chunk size) and one grid launch per step for all scenes,
+ * and optionally the whole re-bin period replayed as a captured HIP graph. The reference runs
  * one scene per Worker or Python process (gpu/server.py:34-40); these calls
  * batch what would be many MPM3 instances. Scenes share one material table;
  * each has its own domain (mpm_config; capacity and sort_every are ignored),
