@@ -369,6 +369,19 @@ int mpm_debug_bin(mpm_engine* e, uint32_t* keys_in, uint32_t* keys_out, uint32_t
  * the input positions. Used to test the binning sort on edge cases. */
 int mpm_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t key_bits, uint32_t* keys_out, uint32_t* perm_out);
 
+/* mpm_sort_pairs on device memory without any allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops). keys [n], keys_out [n] and
+ * perm_out [n] are device pointers (keys may alias keys_out); the sort is
+ * enqueued on `stream` (a hipStream_t; NULL is the null stream) and the
+ * caller synchronises. Scratch (second buffers, histograms, scan partials)
+ * is one static device array per GPU sized for MPM_SORT_NOALLOC_MAX keys;
+ * larger n gives MPM_ERR_CAPACITY. Calls that overlap on one GPU share that
+ * array and race. Same result as mpm_sort_pairs. */
+#define MPM_SORT_NOALLOC_MAX (1u << 20)
+int mpm_sort_pairs_noalloc(const uint32_t* keys, uint32_t n, uint32_t key_bits, uint32_t* keys_out,
+                           uint32_t* perm_out, void* stream);
+
 /* One P2G transfer of the resident state with step dt, leaving the dense
  * grid: mass [N^3] and momentum [3*N^3] as FP64 copies (node-major,
  * x fastest). Does not advance the state. */

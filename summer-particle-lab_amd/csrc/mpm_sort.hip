@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <mutex>
+
 #include "mpm_internal.h"
 
 namespace mpm {
@@ -218,3 +221,92 @@ void radix_sort_pairs(hipStream_t s, uint32_t*& keys, uint32_t*& vals, uint32_t*
 }
 
 }  // namespace mpm
+
+// ---- the sort without allocation (mpm_sort_pairs_noalloc) ----
+//
+// Scratch of the sort lives in one file-scope device array: the second key
+// and value buffers, the digit histogram table and the scan partials, sized
+// for MPM_SORT_NOALLOC_MAX keys (about 8.5 MiB, held on each device the
+// library runs on until the process ends). Every call overwrites all the
+// scratch it reads, so nothing has to be cleared between calls; calls that
+// overlap on one device share the array and race.
+namespace {
+
+using namespace mpm;
+
+constexpr size_t NA_MAX = MPM_SORT_NOALLOC_MAX;
+constexpr size_t NA_TILES = (NA_MAX + RS_TILE - 1) / RS_TILE;
+constexpr size_t NA_HIST = NA_TILES * RS_RADIX;
+constexpr size_t NA_PARTIALS = (NA_HIST + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
+constexpr size_t NA_WORDS = 2 * NA_MAX + NA_HIST + NA_PARTIALS;
+
+__device__ __align__(256) uint32_t scratch[NA_WORDS];
+
+// The array's device address per device. A host reference to `scratch` is a
+// host-side stand-in, not a device address: only these pointers may reach a
+// kernel or a copy.
+constexpr int NA_DEVICES = 64;
+uint32_t* g_scratch[NA_DEVICES] = {};
+std::mutex g_scratch_lock;
+
+// Resolves the array's address on the current device the first time the
+// device calls; later calls read the cached pointer.
+hipError_t scratch_on_device(uint32_t** out) {
+  int dev = 0;
+  hipError_t st = hipGetDevice(&dev);
+  if (st != hipSuccess) return st;
+  if (dev < 0 || dev >= NA_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_lock);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    st = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (st != hipSuccess) return st;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch[dev] = static_cast<uint32_t*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+
+// keys_out = keys_in, perm = 0..n-1 (keys_in may alias keys_out).
+__global__ __launch_bounds__(256) void k_na_init(const uint32_t* keys_in, uint32_t* keys_out,
+                                                 uint32_t* __restrict__ perm, uint32_t n) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    keys_out[i] = keys_in[i];
+    perm[i] = i;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_na_copy(const uint32_t* __restrict__ k_src, const uint32_t* __restrict__ v_src,
+                                                 uint32_t* __restrict__ k_dst, uint32_t* __restrict__ v_dst,
+                                                 uint32_t n) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    k_dst[i] = k_src[i];
+    v_dst[i] = v_src[i];
+  }
+}
+
+}  // namespace
+
+extern "C" int mpm_sort_pairs_noalloc(const uint32_t* keys, uint32_t n, uint32_t key_bits, uint32_t* keys_out,
+                                      uint32_t* perm_out, void* stream) {
+  if (key_bits > 32) return MPM_ERR_INVALID_ARGUMENT;
+  if (n > NA_MAX) return MPM_ERR_CAPACITY;
+  if (n == 0) return MPM_OK;
+  if (!keys || !keys_out || !perm_out) return MPM_ERR_INVALID_ARGUMENT;
+  uint32_t* base = nullptr;
+  if (scratch_on_device(&base) != hipSuccess) return MPM_ERR_HIP;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* k = keys_out;
+  uint32_t* v = perm_out;
+  uint32_t* k_alt = base;
+  uint32_t* v_alt = base + NA_MAX;
+  uint32_t* hist = base + 2 * NA_MAX;
+  uint32_t* partials = hist + NA_HIST;
+  const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 4096u);
+  hipLaunchKernelGGL(k_na_init, dim3(blocks), dim3(256), 0, s, keys, keys_out, perm_out, n);
+  radix_sort_pairs(s, k, v, k_alt, v_alt, hist, partials, n, key_bits);
+  if (k != keys_out)  // an odd number of passes leaves the result in the scratch buffers
+    hipLaunchKernelGGL(k_na_copy, dim3(blocks), dim3(256), 0, s, k, v, keys_out, perm_out, n);
+  return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP;
+}

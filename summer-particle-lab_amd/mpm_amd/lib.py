@@ -182,6 +182,7 @@ SIGNATURES = {
     "mpm_debug_bin": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "mpm_sort_pairs": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint32)]),
+    "mpm_sort_pairs_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_debug_p2g": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                 C.POINTER(C.c_int32)]),
     "mpm_return_map_batch": (C.c_int, [C.POINTER(MpmMaterial), C.c_uint32, C.c_float] + [C.POINTER(C.c_float)] * 6),
