@@ -33,7 +33,9 @@ extern "C" {
 
 /* Status codes. 7 and 8 mirror the reference GPU service's status codes for
  * "MPM CFL limit exceeded" and "GPU state became invalid"
- * (gpu/mixed_solver.py:381-396). */
+ * (gpu/mixed_solver.py:381-396); the state guard of mpm_config sets them
+ * (mpm_step returns 7 at once for a dt no state could allow; the rest
+ * surfaces at mpm_synchronize / mpm_batch_synchronize). */
 enum {
   MPM_OK = 0,
   MPM_ERR_INVALID_ARGUMENT = 1,
@@ -95,6 +97,21 @@ typedef struct {
                                 (reference: 1e-12) */
   int32_t sort_every;        /* re-bin particles every k steps (>= 1) */
   int32_t strict_domain;     /* 1: apply the reference's domain limits */
+  /* State guard: the checks of the reference GPU service's check_state
+   * (gpu/mixed_solver.py:381-388), made inside every step's particle kernel.
+   * CFL (status 7): the stable step of the state a step starts from,
+   * min(cfl_cap, cfl_number h / max(wave, 1), .08 / max(sum |C|, 1)) with
+   * wave = |v| + c max(1, |F|row |F|col, compression) (src/three/mpm.js:43-55;
+   * c = sqrt((lambda + 2 mu) hardening / rho)), is below 0.99999 dt.
+   * Invalid state (status 8): after G2P, det F outside [1e-6, 1e6] or not a
+   * number, a plastic strain that is not a number, a velocity component not
+   * finite or above 1e5 in magnitude, or a position not a number or more than
+   * one cell outside [lower, upper] (the walls act on nodes within one cell
+   * of a face, so a particle can settle up to half a cell past it). */
+  int32_t guard_off;         /* 1: no checks */
+  double cfl_number;         /* <= 0: the reference's 1.4/8 */
+  double cfl_cap;            /* > 0: the stable step's cap term (the reference's
+                                h/8/240 in its own units); <= 0: no cap */
 } mpm_config;
 
 /* One material. Strengths are normalised by E as in the reference records
@@ -214,8 +231,10 @@ int mpm_upload(mpm_engine* e, const mpm_particles* p);
 int mpm_download(mpm_engine* e, mpm_particles* p);
 
 /* Advance nsteps steps of size dt; replaces MPM3.step(s, dt)
- * (src/three/mpm.js:56-134). Asynchronous on the engine stream; errors of
- * the device state surface at the next synchronising call. */
+ * (src/three/mpm.js:56-134). Asynchronous on the engine stream. With the
+ * guard on, a dt above cfl_cap, cfl_number h or .08 (the stable step of any
+ * state) gives MPM_ERR_CFL at once and enqueues nothing; the state-dependent
+ * checks are made on the device and reported by mpm_synchronize. */
 int mpm_step(mpm_engine* e, double dt, int32_t nsteps);
 
 /* CFL-type stable step of the current state; replaces MPM3.stableStep(s)
@@ -304,7 +323,16 @@ int mpm_frame_packet(const char* json, uint32_t json_len, const float* data, uin
  * stress is [count] FP64. */
 int mpm_refresh_diagnostics(mpm_engine* e, double* stress);
 
-/* Wait for all work of the engine. */
+/* The Kirchhoff stress the next step applies to each particle, in the
+ * caller's particle order, 9 values row-major per particle (MPM3.tau,
+ * src/three/mpm.js:33-37): the cached return stress of models >= 2 when
+ * valid, else the model's stress of F, times (1 - damage) below failure. */
+int mpm_stress_tensors(mpm_engine* e, double* tau);
+
+/* Wait for all work of the engine, then report and clear the guard's
+ * findings since the last call: MPM_ERR_INVALID_STATE (8) when some step
+ * left an invalid state, else MPM_ERR_CFL (7) when some step started from a
+ * state whose stable step is below its dt (8 wins, as in check_state). */
 int mpm_synchronize(mpm_engine* e);
 
 /* Number of particles resident on the device. */
@@ -560,6 +588,8 @@ int mpm_batch_download(mpm_batch* b, int32_t scene, mpm_particles* p);
 int mpm_batch_set_dt(mpm_batch* b, const double* dt);
 /* Advance every scene nsteps steps (asynchronous on the batch stream). */
 int mpm_batch_step(mpm_batch* b, int32_t nsteps);
+/* Waits for the batch; reports and clears the guard's findings of every
+ * scene as mpm_synchronize does (the message names the first scene). */
 int mpm_batch_synchronize(mpm_batch* b);
 /* FP64 conservation report of one scene (as mpm_conservation_report). */
 int mpm_batch_conservation(mpm_batch* b, int32_t scene, mpm_conservation* out);

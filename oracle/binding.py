@@ -63,6 +63,7 @@ class OrcScene(C.Structure):
         ("N", C.c_int),
         ("gmass", PD), ("gv", PD), ("gfixed", PU8),
         ("vol_scale", D), ("grid_gravity", D * 3), ("wall_mode", C.c_int32 * 6), ("mass_eps", D),
+        ("press_enabled", C.c_int32), ("press_y", D), ("press_velocity", D), ("press_impulse", D),
     ]
 
 
@@ -404,6 +405,15 @@ class Scene:
         s.mass_eps = mass_eps
         self.s = s
         self.lower, self.upper = np.asarray(lower, float), np.asarray(upper, float)
+
+    def set_press(self, y, velocity, impulse=0.0):
+        """The platen of the reference's grid loop (s.press, src/three/mpm.js:97-101)."""
+        self.s.press_enabled, self.s.press_y, self.s.press_velocity = 1, float(y), float(velocity)
+        self.s.press_impulse = float(impulse)
+
+    @property
+    def press_impulse(self):
+        return self.s.press_impulse
 
     def step(self, dt, particle_gravity=None):
         if particle_gravity is not None:

@@ -227,6 +227,11 @@ typedef struct {
   double grid_gravity[3];
   int32_t wall_mode[6]; // faces -x,+x,-y,+y,-z,+z: 0 separating, 1 sticky, 2 slip
   double mass_eps;
+  // the platen (s.press, src/three/mpm.js:97-101): nodes at or above
+  // press_y - H/2 not held by a pinned particle move down at most at
+  // press_velocity; press_impulse accumulates (before - after) mass
+  int32_t press_enabled;
+  double press_y, press_velocity, press_impulse;
 } OrcScene;
 
 // -------------------------------------------------------------- sampling.js
@@ -815,6 +820,11 @@ int orc_step_to_grid(const OrcScene* s, double dt) {
       const bool hi_hit = q[a] > hi - H && (s->wall_mode[2 * a + 1] == 2 || v > 0);
       if (s->gfixed[node] || sticky || lo_hit || hi_hit) v = 0;
       s->gv[node * 3 + a] = v;
+    }
+    if (s->press_enabled && q[1] >= s->press_y - H * .5 && !s->gfixed[node]) {  // :97-101
+      const double before = s->gv[node * 3 + 1], after = std::min(before, s->press_velocity);
+      s->gv[node * 3 + 1] = after;
+      const_cast<OrcScene*>(s)->press_impulse += (before - after) * mass;
     }
   }
   return 0;

@@ -52,6 +52,7 @@ struct mpm_batch {
   hipStream_t stream = nullptr;
   int32_t S = 0;
   std::vector<SceneDesc> sc;
+  std::vector<mpm_config> cfg;  // per scene (the guard settings)
   std::vector<double> cell;
   std::vector<int64_t> mass_q;
   std::vector<char> uploaded;
@@ -83,6 +84,7 @@ struct mpm_batch {
   bool graph_ready = false;
   int graph_next = 0;
   double* red_host = nullptr;
+  uint32_t* status_host = nullptr;  // pinned copy of the per-scene guard words
   std::string err;
 };
 
@@ -238,7 +240,7 @@ int prepare_graphs(mpm_batch* b) {
   b->graph_after[1] = b->d;
   if (rc) return rc;
   if (b->d.planes != before.planes || b->d.keys != before.keys || b->d.vals != before.vals)
-    return bfail(b, MPM_ERR_INVALID_STATE, "graph pair does not return the buffers to their start");
+    return bfail(b, MPM_ERR_HIP, "graph pair does not return the buffers to their start");
   b->graph_ready = true;
   b->graph_next = 0;
   return MPM_OK;
@@ -260,6 +262,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   auto* b = new mpm_batch();
   b->S = nscenes;
   b->sc.resize(nscenes);
+  b->cfg.assign(scenes, scenes + nscenes);
   b->cell.resize(nscenes);
   b->mass_q.assign(nscenes, 0);
   b->uploaded.assign(nscenes, 0);
@@ -331,8 +334,12 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
   alloc((void**)&b->d_scenes, sizeof(SceneDesc) * nscenes);
   alloc((void**)&b->d_block_scene, sizeof(uint32_t) * b->total_blocks);
+  alloc((void**)&d.status, sizeof(uint32_t) * (size_t)nscenes);
   if (st == hipSuccess)
     st = hipHostMalloc((void**)&b->red_host, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes, hipHostMallocDefault);
+  if (st == hipSuccess)
+    st = hipHostMalloc((void**)&b->status_host, sizeof(uint32_t) * (size_t)nscenes, hipHostMallocDefault);
+  if (st == hipSuccess) st = hipMemsetAsync(d.status, 0, sizeof(uint32_t) * (size_t)nscenes, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nt, b->stream);
@@ -377,13 +384,14 @@ int mpm_batch_destroy(mpm_batch* b) {
   void* ptrs[] = {d.planes, d.planes_alt, d.ids,   d.ids_alt,  d.meta,     d.meta_alt, d.acc_m, d.acc_p,
                   d.vel,    d.fixed,      d.pieces, d.keys,    d.keys_alt, d.vals,     d.vals_alt, d.hist,
                   d.partials, d.red,      d.materials, b->d_scenes, b->d_bin_chunks, b->d_block_scene,
-                  d.piece_list, d.piece_counters};
+                  d.piece_list, d.piece_counters, d.status};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& bk : b->bucket)
     if (bk.chunks) (void)hipFree(bk.chunks);
   if (b->d_all_chunks) (void)hipFree(b->d_all_chunks);
   if (b->red_host) (void)hipHostFree(b->red_host);
+  if (b->status_host) (void)hipHostFree(b->status_host);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MPM_OK;
@@ -477,7 +485,16 @@ int mpm_batch_set_dt(mpm_batch* b, const double* dt) {
   if (!b || !dt) return bfail(b, MPM_ERR_INVALID_ARGUMENT, "dt array required");
   for (int32_t s = 0; s < b->S; ++s)
     if (!(dt[s] > 0) || !std::isfinite(dt[s])) return bfail(b, MPM_ERR_INVALID_ARGUMENT, "scene %d: bad dt", s);
-  for (int32_t s = 0; s < b->S; ++s) b->sc[s].sp.dt = (float)dt[s];
+  for (int32_t s = 0; s < b->S; ++s) {
+    StepParams sp = b->sc[s].sp;
+    if (guard_params(b->cfg[s], dt[s], sp))
+      return bfail(b, MPM_ERR_CFL, "scene %d: MPM CFL limit exceeded: dt %g is above the stable step of any state", s,
+                   dt[s]);
+  }
+  for (int32_t s = 0; s < b->S; ++s) {
+    b->sc[s].sp.dt = (float)dt[s];
+    guard_params(b->cfg[s], dt[s], b->sc[s].sp);
+  }
   return upload_scenes_table(b);
 }
 
@@ -529,7 +546,20 @@ int mpm_batch_step(mpm_batch* b, int32_t nsteps) {
 
 int mpm_batch_synchronize(mpm_batch* b) {
   if (!b) return MPM_ERR_INVALID_ARGUMENT;
+  BHIP(b, hipMemcpyAsync(b->status_host, b->d.status, sizeof(uint32_t) * (size_t)b->S, hipMemcpyDeviceToHost,
+                         b->stream));
+  launch_zero_u32(b->stream, b->d.status, (uint32_t)b->S);
+  BHIP(b, hipGetLastError());
   BHIP(b, hipStreamSynchronize(b->stream));
+  int first_cfl = -1;
+  for (int32_t s = 0; s < b->S; ++s) {
+    if (b->status_host[s] & STATUS_INVALID)
+      return bfail(b, MPM_ERR_INVALID_STATE,
+                   "scene %d: GPU state became invalid: det F outside [1e-6, 1e6], a value that is not a number, "
+                   "|v| above 1e5 or a particle outside the domain", s);
+    if ((b->status_host[s] & STATUS_CFL) && first_cfl < 0) first_cfl = s;
+  }
+  if (first_cfl >= 0) return bfail(b, MPM_ERR_CFL, "scene %d: MPM CFL limit exceeded; reduce dt", first_cfl);
   return MPM_OK;
 }
 

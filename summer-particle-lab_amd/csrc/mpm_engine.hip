@@ -37,6 +37,7 @@ struct mpm_engine {
   int steps_since_sort = 0;
   std::string err;
   double* red_host = nullptr;  // pinned
+  uint32_t* status_host = nullptr;  // pinned copy of the guard's status word
   uint32_t grid_blocks = 1;    // workgroups of the grid update
   // frame wrapper scratch (allocated on first use)
   float* frame_ox = nullptr;
@@ -287,11 +288,15 @@ int rebin(mpm_engine* e) {
   return MPM_OK;
 }
 
-StepParams params(const mpm_engine* e, double dt) {
+// Step parameters of a step of size dt; MPM_ERR_CFL in *rc when the guard
+// finds that no state can take dt.
+StepParams params(const mpm_engine* e, double dt, int* rc = nullptr) {
   StepParams p;
   p.dt = (float)dt;
   p.mass_scale = (float)e->mass_scale;
   p.inv_mass_scale = (float)(1.0 / e->mass_scale);
+  const int r = guard_params(e->cfg, dt, p);
+  if (rc) *rc = r;
   return p;
 }
 
@@ -379,8 +384,11 @@ int mpm_create(const mpm_config* c, mpm_engine** out) {
   alloc((void**)&d.piece_list, sizeof(uint32_t) * npieces);
   alloc((void**)&d.piece_counters, sizeof(unsigned long long) * 3);
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
+  alloc((void**)&d.status, sizeof(uint32_t));
   if (st == hipSuccess)
     st = hipHostMalloc((void**)&e->red_host, sizeof(double) * 4 * SUM_SLOTS, hipHostMallocDefault);
+  if (st == hipSuccess) st = hipHostMalloc((void**)&e->status_host, sizeof(uint32_t), hipHostMallocDefault);
+  if (st == hipSuccess) st = hipMemsetAsync(d.status, 0, sizeof(uint32_t), e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_m, 0, sizeof(unsigned long long) * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nn, e->stream);
@@ -418,10 +426,12 @@ int mpm_destroy(mpm_engine* e) {
   void* ptrs[] = {d.planes, d.planes_alt, d.ids,  d.ids_alt,  d.meta,     d.meta_alt, d.acc_m,
                   d.acc_p,  d.vel,        d.fixed, d.pieces,  d.keys,     d.keys_alt, d.vals,
                   d.vals_alt, d.hist,     d.partials, d.red,  d.materials, d.mlp, d.sur_counts,
-                  d.piece_list, d.piece_counters, e->frame_ox, e->frame_disp, e->frame_cell_start, e->frame_max};
+                  d.piece_list, d.piece_counters, d.status, e->frame_ox, e->frame_disp, e->frame_cell_start,
+                  e->frame_max};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (e->red_host) hipHostFree(e->red_host);
+  if (e->status_host) hipHostFree(e->status_host);
   for (auto& p : e->pending) {
     hipEventDestroy(p.start);
     hipEventDestroy(p.stop);
@@ -512,7 +522,9 @@ int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
   if (!(dt > 0) || !std::isfinite(dt) || nsteps < 0) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt or nsteps");
   if (e->split_pending) return fail(e, MPM_ERR_NOT_READY, "a split step waits for mpm_step_from_grid");
   if (nsteps == 0 || e->n == 0) return MPM_OK;
-  const StepParams sp = params(e, dt);
+  int guard = MPM_OK;
+  const StepParams sp = params(e, dt, &guard);
+  if (guard) return fail(e, guard, "MPM CFL limit exceeded: dt %g is above the stable step of any state", dt);
   if (e->steps_since_sort >= e->cfg.sort_every) {
     int rc = rebin(e);
     if (rc) return rc;
@@ -558,7 +570,9 @@ int mpm_step_to_grid(mpm_engine* e, double dt) {
   if (!(dt > 0) || !std::isfinite(dt)) return fail(e, MPM_ERR_INVALID_ARGUMENT, "bad dt");
   if (e->split_pending) return fail(e, MPM_ERR_NOT_READY, "a split step waits for mpm_step_from_grid");
   if (e->n == 0) return MPM_OK;
-  const StepParams sp = params(e, dt);
+  int guard = MPM_OK;
+  const StepParams sp = params(e, dt, &guard);
+  if (guard) return fail(e, guard, "MPM CFL limit exceeded: dt %g is above the stable step of any state", dt);
   if (e->steps_since_sort >= e->cfg.sort_every) {
     int rc = rebin(e);
     if (rc) return rc;
@@ -678,7 +692,16 @@ int mpm_step_from_grid(mpm_engine* e, double dt, const int32_t* material_slot, i
 
 int mpm_synchronize(mpm_engine* e) {
   if (!e) return MPM_ERR_INVALID_ARGUMENT;
+  HIP_OK(e, hipMemcpyAsync(e->status_host, e->d.status, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+  launch_zero_u32(e->stream, e->d.status, 1);
+  HIP_OK(e, hipGetLastError());
   HIP_OK(e, hipStreamSynchronize(e->stream));
+  const uint32_t status = *e->status_host;
+  if (status & STATUS_INVALID)
+    return fail(e, MPM_ERR_INVALID_STATE,
+                "GPU state became invalid: det F outside [1e-6, 1e6], a value that is not a number, |v| above 1e5 "
+                "or a particle outside the domain");
+  if (status & STATUS_CFL) return fail(e, MPM_ERR_CFL, "MPM CFL limit exceeded; reduce dt");
   return MPM_OK;
 }
 
@@ -742,6 +765,35 @@ int mpm_refresh_diagnostics(mpm_engine* e, double* stress) {
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "diagnostics: %s", hipGetErrorString(st));
   for (uint32_t s = 0; s < n; ++s) stress[ids[s]] = h[s];
+  return MPM_OK;
+}
+
+int mpm_stress_tensors(mpm_engine* e, double* tau) {
+  if (!e || !tau) return MPM_ERR_INVALID_ARGUMENT;
+  const uint32_t n = e->n;
+  if (!n) return MPM_OK;
+  float* dbuf = nullptr;
+  HIP_OK(e, hipMallocAsync((void**)&dbuf, sizeof(float) * 6 * (size_t)n, e->stream));
+  launch_von_mises(e->stream, e->d, n, nullptr, dbuf);
+  std::vector<float> h(6 * (size_t)n);
+  std::vector<uint32_t> ids(n);
+  hipError_t st = hipGetLastError();
+  if (st == hipSuccess)
+    st = hipMemcpyAsync(h.data(), dbuf, sizeof(float) * h.size(), hipMemcpyDeviceToHost, e->stream);
+  if (st == hipSuccess) st = hipMemcpyAsync(ids.data(), e->d.ids, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream);
+  hipFreeAsync(dbuf, e->stream);
+  if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+  if (st != hipSuccess) return fail(e, MPM_ERR_HIP, "stress tensors: %s", hipGetErrorString(st));
+  for (uint32_t s = 0; s < n; ++s) {
+    const float* t = &h[6 * (size_t)s];
+    double* o = tau + 9 * (size_t)ids[s];
+    o[0] = t[0];
+    o[4] = t[1];
+    o[8] = t[2];
+    o[1] = o[3] = t[3];
+    o[2] = o[6] = t[4];
+    o[5] = o[7] = t[5];
+  }
   return MPM_OK;
 }
 
@@ -953,7 +1005,8 @@ int mpm_debug_p2g(mpm_engine* e, double dt, double* grid_mass, double* grid_mome
   if (Nout) *Nout = N;
   if (!grid_mass && !grid_momentum) return MPM_OK;
   const size_t Nn = (size_t)N * N * N;
-  const StepParams sp = params(e, dt);
+  StepParams sp = params(e, dt);
+  sp.guard = 0;  // a transfer for inspection, not a step
   launch_fused(e->stream, e->d, e->n, e->g, sp, 0);
   std::vector<unsigned long long> m(Nn);
   std::vector<float> p(3 * Nn);

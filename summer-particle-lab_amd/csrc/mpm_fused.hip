@@ -168,6 +168,44 @@ __device__ __forceinline__ DevMaterial load_material(const DevMaterial* __restri
   return mats[id];
 }
 
+// The stable-step test of the reference's stableStep (src/three/mpm.js:43-55)
+// as the GPU service applies it after a step (gpu/mixed_solver.py:383): true
+// when this particle's wave speed |v| + c max(1, |F|row |F|col, compression),
+// c = sqrt((lambda + 2 mu) hardening / rho), or its sum |C| is above what the
+// step size allows (the limits of StepParams). A value that is not a number
+// fails too.
+__device__ __forceinline__ bool cfl_exceeded(const M3& F, const float C[9], const float v[3], float m, float vol,
+                                             const DevMaterial& mat, int model, float jp, const StepParams& sp) {
+  float row = 0.f, col = 0.f, gsum = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    row = fmaxf(row, fabsf(F[a * 3]) + fabsf(F[a * 3 + 1]) + fabsf(F[a * 3 + 2]));
+    col = fmaxf(col, fabsf(F[a]) + fabsf(F[3 + a]) + fabsf(F[6 + a]));
+    gsum += fabsf(C[a * 3]) + fabsf(C[a * 3 + 1]) + fabsf(C[a * 3 + 2]);
+  }
+  const float hard = model == 2 ? snow_hardening(jp) : 1.f;
+  float comp = 1.f;
+  if (model == 1 || model == 8 || model == 9) comp = 1.f / fmaxf(.05f, det3(F) * (model == 1 ? 1.f : jp));
+  const float c = sqrtf((mat.lambda + 2.f * mat.mu) * hard * vol / m);
+  const float wave = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + c * fmaxf(fmaxf(1.f, row * col), comp);
+  return !(wave <= sp.wave_limit) || !(gsum <= sp.grad_limit);
+}
+
+// The invalid state of the GPU service's check (gpu/mixed_solver.py:384-388):
+// det F outside [1e-6, 1e6] or not a number, plastic strain not a number, a
+// velocity component not finite or above 1e5 in magnitude, a position not a
+// number or more than one cell outside the domain box. Every comparison is
+// false for a NaN, so a NaN fails.
+__device__ __forceinline__ bool state_invalid(const M3& F, float ep, const float v[3], const float x[3],
+                                              const GridDims& g) {
+  const float J = det3(F);
+  bool ok = J >= 1e-6f && J <= 1e6f && ep == ep;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    ok = ok && fabsf(v[a]) <= 1e5f && x[a] >= g.lower[a] - g.H && x[a] <= g.upper[a] + g.H;
+  return !ok;
+}
+
 // The particle work of one chunk of FBK cell-sorted particles in slots
 // [first, min(first + FBK, end)). Grid pointers are those of the chunk's
 // scene; momentum planes are `pstride` floats apart.
@@ -179,7 +217,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                                             float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
                                             uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp,
                                             unsigned long long* __restrict__ sur_counts, unsigned char* s_raw,
-                                            int* s_box, const float4* __restrict__ deltas = nullptr,
+                                            int* s_box, uint32_t* __restrict__ status,
+                                            const float4* __restrict__ deltas = nullptr,
                                             const int32_t* __restrict__ delta_slot = nullptr) {
   using S = Shape<FBK>;
   const uint32_t tid = threadIdx.x;
@@ -418,12 +457,25 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         }
       }
     }
+    if (sp.guard) {
+      const bool bad = gather && state_invalid(F, ep, v, x, g);
+      if (__any(bad) && lane == 0u) atomicOr(status, STATUS_INVALID);
+    }
     if (MODE == 2) return;
   }
 
   // -------------------------------------------------------------------- P2G
   const float m = ld(P, cap, P_MASS, ic);
   const float vol = ld(P, cap, P_VOL, ic);
+  if (sp.guard) {
+    // The stable step of the state this step starts from.
+    bool slow = false;
+    if (valid) {
+      const float jpc = (model == 2 || model == 8 || model == 9) ? ld(P, cap, P_JP, ic) : 1.f;
+      slow = cfl_exceeded(F, Cm, v, m, vol, mat, model, jpc, sp);
+    }
+    if (__any(slow) && lane == 0u) atomicOr(status, STATUS_CFL);
+  }
   if (g.gravity_on_particles && !pinned) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] += g.gravity[a] * dt;
@@ -698,11 +750,12 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
     unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
     uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts,
-    const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
+    uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB>::LDS_BYTES];
   __shared__ int s_box[6 * Shape<FB>::WAVES];
   fused_chunk<MODE, SUR, FB, EXT>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
-                                  acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, deltas, delta_slot);
+                                  acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
+                                  delta_slot);
 }
 
 // Many scenes in one launch: one workgroup per chunk of the bucket's chunk
@@ -713,7 +766,7 @@ template <int MODE, int FBK, bool EXT>
 __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_batch(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-    uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces) {
+    uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FBK>::LDS_BYTES];
   __shared__ int s_box[6 * Shape<FBK>::WAVES];
   const uint32_t t = blockIdx.x;
@@ -730,7 +783,7 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const uint32_t end = __builtin_amdgcn_readfirstlane(sd->p0 + sd->n);
   fused_chunk<MODE, false, FBK, EXT>(P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0, acc_m + node0,
                                      acc_p + node0, fixed + node0, pieces + (node0 >> 6), nullptr, nullptr, s_raw,
-                                     s_box);
+                                     s_box, status + ch.x);
 }
 
 inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
@@ -744,7 +797,7 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
 #define MPM_FUSED(M, S, X)                                                                                      \
   hipExtLaunchKernelGGL((k_fused<M, S, X>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n, \
                         d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts,      \
-                        mode == 2 ? d.deltas : nullptr, d.delta_slot)
+                        d.status, mode == 2 ? d.deltas : nullptr, d.delta_slot)
 #define MPM_FUSED_X(M, S) \
   if (d.ext_laws)         \
     MPM_FUSED(M, S, true); \
@@ -771,7 +824,7 @@ void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, in
   const dim3 grid(b.nchunks);
 #define MPM_FUSED_B(M, K, X)                                                                                       \
   hipExtLaunchKernelGGL((k_fused_batch<M, K, X>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta,     \
-                        d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces)
+                        d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.status)
 #define MPM_FUSED_K(M, X)     \
   if (fb == 64)               \
     MPM_FUSED_B(M, 64, X);    \

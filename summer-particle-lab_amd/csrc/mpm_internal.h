@@ -49,7 +49,16 @@ struct StepParams {
   float dt;
   float mass_scale;      // fixed-point units per unit mass (2^k)
   float inv_mass_scale;
+  // State guard (0: off): a particle whose wave speed exceeds wave_limit or
+  // whose sum |C| exceeds grad_limit sets STATUS_CFL; an invalid state after
+  // G2P sets STATUS_INVALID.
+  uint32_t guard = 0;
+  float wave_limit = 3.0e38f;
+  float grad_limit = 3.0e38f;
 };
+
+// Bits of the guard's status word (one word per scene).
+enum : uint32_t { STATUS_CFL = 1u, STATUS_INVALID = 2u };
 
 // Device pointers of one engine.
 struct DevState {
@@ -81,6 +90,8 @@ struct DevState {
   // particle-step counters, SUR_SLOTS slots of {network, elastic, fallback, -}
   MlpDev* mlp;
   unsigned long long* sur_counts;
+  // guard status words: one per scene (STATUS_* bits)
+  uint32_t* status;
 };
 
 // Sort key of a base cell: cells are ordered by 4x4x4 block (blocks in
@@ -179,6 +190,8 @@ void launch_regularize(hipStream_t s, const DevState& d, uint32_t n, const GridD
 // Trained parameters packed into the device record (mpm_mlp.hip).
 void pack_mlp(const float* params, const MlpNorm& norm, MlpDev& out);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
+// Zeroes n words with a kernel (no memset node, so the launch may be captured).
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
 // Particle-side FP64 sums: particle_reduce_blocks() records of
 // particle_reduce_k() doubles each (see k_particle_reduce).
 int particle_reduce_k();
@@ -186,7 +199,9 @@ int particle_reduce_blocks();
 void launch_particle_reduce(hipStream_t s, const DevState& d, uint32_t n, float mass_scale, float D, double* out);
 // Per block {max wave speed, max sum |C|}; particle_reduce_blocks() records.
 void launch_stable(hipStream_t s, const DevState& d, uint32_t n, float* out);
-void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out);
+// Von Mises diagnostic per slot (out) and, when tau_out is set, the step's
+// Kirchhoff stress (6 per slot); either may be null.
+void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out, float* tau_out = nullptr);
 // Dense view of the last grid update (mass, 3 velocities per node, pinned
 // flag) over the listed pieces, and velocities written back into them.
 void launch_grid_export(hipStream_t s, const DevState& d, const GridDims& g, float* mass, float* v, uint8_t* fixed);

@@ -532,6 +532,11 @@ __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 // Particle-side FP64 sums, one record of RED_K doubles per block (finished on
 // the host): mass, fixed-point mass, momentum, angular momentum with the APIC
 // spin term, kinetic and affine energy, min J, max |v|, non-finite count.
@@ -675,9 +680,12 @@ __device__ __forceinline__ float diag_stress(const float* __restrict__ P, uint32
   return vm / fmaxf(1e-6f, J) / mat.E;
 }
 
+// The von Mises diagnostic per slot (out, may be null) and, when tau_out is
+// set, the Kirchhoff stress the next step applies: the step stress times
+// (1 - damage) below failure (src/three/mpm.js:36-37), 6 values per slot.
 __global__ __launch_bounds__(BLOCK) void k_von_mises(const float* __restrict__ P, const uint32_t* __restrict__ meta,
                                                      uint32_t cap, uint32_t n, const DevMaterial* __restrict__ mats,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out, float* __restrict__ tau_out) {
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const uint32_t mt = meta[i];
@@ -685,8 +693,16 @@ __global__ __launch_bounds__(BLOCK) void k_von_mises(const float* __restrict__ P
   M3 F;
 #pragma unroll
   for (int k = 0; k < 9; ++k) F[k] = plane(P, cap, P_F0 + k, i);
-  const int model = effective_model(mat, mat.fr_on ? plane(P, cap, P_DMG, i) : 0.f);
-  out[i] = diag_stress(P, cap, i, mt >> 8, mat, model, F);
+  const float dmg = mat.fr_on ? plane(P, cap, P_DMG, i) : 0.f;
+  const int model = effective_model(mat, dmg);
+  if (out) out[i] = diag_stress(P, cap, i, mt >> 8, mat, model, F);
+  if (tau_out) {
+    float t[6];
+    step_stress(F, model, mat, mt >> 8, P, cap, i, t);
+    const float f = dmg < 1.f ? 1.f - dmg : 1.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau_out[(size_t)i * 6 + k] = t[k] * f;
+  }
 }
 
 // Preview pick of one particle per block of `stride` consecutive original
@@ -900,6 +916,11 @@ void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {
   hipLaunchKernelGGL(k_fill_u8, dim3(blocks(n)), dim3(BLOCK), 0, s, p, n, v);
 }
 
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks(n)), dim3(BLOCK), 0, s, p, n);
+}
+
 void launch_particle_reduce(hipStream_t s, const DevState& d, uint32_t n, float mass_scale, float D, double* out) {
   hipLaunchKernelGGL(k_particle_reduce, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, d.planes, d.cap, n, mass_scale, D, out);
 }
@@ -941,9 +962,10 @@ void launch_grid_import(hipStream_t s, const DevState& d, const GridDims& g, con
                      d.piece_counters, Nn, v);
 }
 
-void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out) {
+void launch_von_mises(hipStream_t s, const DevState& d, uint32_t n, float* out, float* tau_out) {
   if (!n) return;
-  hipLaunchKernelGGL(k_von_mises, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, d.materials, out);
+  hipLaunchKernelGGL(k_von_mises, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.meta, d.cap, n, d.materials, out,
+                     tau_out);
 }
 
 }  // namespace mpm

@@ -67,7 +67,10 @@ def _ptr(a):
 
 
 def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
-                gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False):
+                gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False, guard=True,
+                cfl_number=0.0, cfl_cap=0.0):
+    """An mpm_config. The state guard (codes 7 and 8 at synchronize) is on by
+    default with the reference's Courant number 1.4/8 and no cap term."""
     cfg = L.MpmConfig()
     cfg.lower[:] = list(map(float, lower))
     cfg.upper[:] = list(map(float, upper))
@@ -79,6 +82,9 @@ def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, grav
     cfg.mass_eps = float(mass_eps)
     cfg.sort_every = int(sort_every)
     cfg.strict_domain = int(bool(strict_domain))
+    cfg.guard_off = 0 if guard else 1
+    cfg.cfl_number = float(cfl_number)
+    cfg.cfl_cap = float(cfl_cap)
     return cfg
 
 
@@ -156,11 +162,12 @@ class Engine:
     """One MLS-MPM engine on the current HIP device."""
 
     def __init__(self, lower, upper, cell_size, capacity, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
-                 gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False):
+                 gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False, guard=True,
+                 cfl_number=0.0, cfl_cap=0.0):
         self._lib = L.load()
         cfg = make_config(lower, upper, cell_size, capacity, wall_mode=wall_mode, gravity=gravity,
                           gravity_on_particles=gravity_on_particles, mass_eps=mass_eps, sort_every=sort_every,
-                          strict_domain=strict_domain)
+                          strict_domain=strict_domain, guard=guard, cfl_number=cfl_number, cfl_cap=cfl_cap)
         h = C.c_void_p()
         L.check(self._lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -272,6 +279,12 @@ class Engine:
         out = np.zeros(self.count)
         self._check(self._lib.mpm_refresh_diagnostics(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    def stress_tensors(self):
+        """The Kirchhoff stress the next step applies, (count, 3, 3) (MPM3.tau)."""
+        out = np.zeros(9 * self.count)
+        self._check(self._lib.mpm_stress_tensors(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(self.count, 3, 3)
 
     def profile(self, enable=True, reset=True):
         self._check(self._lib.mpm_profile(self.h, int(enable), int(reset)))

@@ -30,7 +30,8 @@ class MpmConfig(C.Structure):
         ("lower", C.c_double * 3), ("upper", C.c_double * 3), ("cell_size", C.c_double),
         ("capacity", C.c_uint32), ("wall_mode", C.c_int32 * 6), ("gravity", C.c_double * 3),
         ("gravity_on_particles", C.c_int32), ("mass_eps", C.c_double), ("sort_every", C.c_int32),
-        ("strict_domain", C.c_int32),
+        ("strict_domain", C.c_int32), ("guard_off", C.c_int32), ("cfl_number", C.c_double),
+        ("cfl_cap", C.c_double),
     ]
 
 
@@ -151,6 +152,7 @@ SIGNATURES = {
     "mpm_stable_dt": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(C.c_double)]),
     "mpm_conservation_report": (C.c_int, [_P, C.POINTER(MpmConservation)]),
     "mpm_refresh_diagnostics": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mpm_stress_tensors": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mpm_preview_layout": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "mpm_step_to_grid": (C.c_int, [_P, C.c_double]),
     "mpm_grid_download": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint8)]),

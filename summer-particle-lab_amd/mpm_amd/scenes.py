@@ -146,6 +146,9 @@ class Scene:
     gravity_on_particles: bool = False
     pinned: np.ndarray = None
     info: dict = field(default_factory=dict)
+    # Courant number of the state guard (mpm_config.cfl_number); 0: the
+    # reference's 1.4/8.
+    courant: float = 0.0
 
     @property
     def n(self):
@@ -154,7 +157,8 @@ class Scene:
     def engine(self, capacity=None, sort_every=1):
         from .engine import Engine
         e = Engine(self.lower, self.upper, self.h, capacity or self.n, wall_mode=self.wall_mode,
-                   gravity=self.gravity, gravity_on_particles=self.gravity_on_particles, sort_every=sort_every)
+                   gravity=self.gravity, gravity_on_particles=self.gravity_on_particles, sort_every=sort_every,
+                   cfl_number=self.courant)
         e.set_materials(self.materials)
         e.upload(self.x, self.v, self.mass, self.volume, temperature=self.temperature, material=self.material,
                  pinned=self.pinned)
@@ -211,6 +215,11 @@ def j2_cube(nu=0.3, E=1e5, rho=1000.0, seed=42, strength=0.2, T=(300.0, 900.0), 
 # P-wave CFL number c_p dt / h stays 0.27 at h = 1/128, dt = 2e-5.
 STEEL_THERMAL = dict(Tref=300.0, Thot=1200.0, hotRatio=0.1, A=1.25e-3, H=0.0, kelvin_offset=0.0)
 STEEL_RHO_SCALED = 2.5e7
+# Courant number of the state guard for the workloads that fix dt at a P-wave
+# CFL number of 0.25-0.27: the reference's 1.4/8 is below that, and its wave
+# measure max(1, |F|row |F|col) grows up to 3 under rotation alone, so these
+# scenes flag a wave that crosses a whole cell in one step.
+FIXED_DT_COURANT = 1.0
 
 
 def thermal_boxes(cells=128, nboxes=6, ppa=4, seed=6, side=(0.05, 0.2), T=(300.0, 900.0), vsigma=0.5,
@@ -247,7 +256,8 @@ def thermal_boxes(cells=128, nboxes=6, ppa=4, seed=6, side=(0.05, 0.2), T=(300.0
     mat = Material.from_E_nu(10, 2e11, 0.3, thermal=dict(STEEL_THERMAL))
     return Scene("thermal_boxes", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
                  np.concatenate(vols), temp, np.zeros(n, np.uint8), [mat], dt, steps,
-                 info=dict(boxes=[(c.tolist(), s) for c, s in boxes], rho=rho, spacing=spacing))
+                 info=dict(boxes=[(c.tolist(), s) for c, s in boxes], rho=rho, spacing=spacing),
+                 courant=FIXED_DT_COURANT)
 
 
 def large_scene(cells=256, ppa=2, seed=11, nspheres=32, radius=(0.03, 0.08), T=(300.0, 1200.0), dt=1e-5,
@@ -297,7 +307,8 @@ def large_scene(cells=256, ppa=2, seed=11, nspheres=32, radius=(0.03, 0.08), T=(
                  Material.from_E_nu(0, 2e11, 0.3)]
     return Scene("large_scene", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
                  np.concatenate(vols), temp, np.concatenate(mats), materials, dt, steps,
-                 info=dict(spheres=[(c.tolist(), r) for c, r in spheres], rho=rho, spacing=spacing, block=block))
+                 info=dict(spheres=[(c.tolist(), r) for c, r in spheres], rho=rho, spacing=spacing, block=block),
+                 courant=FIXED_DT_COURANT)
 
 
 @dataclass
@@ -320,7 +331,8 @@ class BatchScenes:
     def batch(self, sort_every=16, use_graph=False):
         from .engine import Batch, make_config
         cfgs = [make_config(sc.lower, sc.upper, sc.h, sc.n, wall_mode=sc.wall_mode, gravity=sc.gravity,
-                            gravity_on_particles=sc.gravity_on_particles) for sc in self.scenes]
+                            gravity_on_particles=sc.gravity_on_particles, cfl_number=sc.courant)
+                for sc in self.scenes]
         b = Batch(cfgs, self.counts, sort_every=sort_every, use_graph=use_graph)
         b.set_materials(self.materials)
         for i, sc in enumerate(self.scenes):
@@ -358,6 +370,6 @@ def batched_scenes(nscenes=256, seed=21, counts=(2_000, 200_000), grids=(32, 64,
         temp = uniform(seed + 9, np.arange(n) + 1_000_000 * k, 0, 0, 300.0, 900.0) if mat == 1 else np.full(n, 300.0)
         sc = Scene(f"batch_{k}", (0, 0, 0), (1, 1, 1), h, pts, np.tile(vel, (n, 1)), np.full(n, rho * vp),
                    np.full(n, vp), temp, np.full(n, mat, np.uint8), materials, cfl * h / cp, steps,
-                   info=dict(cells=cells, per_axis=per_axis, side=side, material=mat))
+                   info=dict(cells=cells, per_axis=per_axis, side=side, material=mat), courant=FIXED_DT_COURANT)
         out.append(sc)
     return BatchScenes(out, materials, steps, info=dict(seed=seed, counts=counts, grids=grids, cp=cp))

@@ -18,6 +18,20 @@
 // Option `resident: true` keeps particle state on the device between steps
 // (upload once, call `download(s)` to read it back); the default copies state
 // in and out on every step like the reference's in-place arrays.
+//
+// Option `guard: true` turns on the device state guard with the reference's
+// stable-step constants (cap h/8/240, Courant number 1.4/8): `checkState()`
+// then returns the GPU service's status code, 7 (CFL exceeded) or 8 (invalid
+// state), or 0 (gpu/mixed_solver.py:381-396). Off by default, because the
+// reference's MPM3.step never throws.
+//
+// Scene fields honoured besides the reference step's inputs: `s.press` (the
+// platen on the grid, src/three/mpm.js:97-101, through a split step),
+// `s.profile` (device kernel times added to its `transfer` and `grid`
+// buckets; the constitutive update is fused into the transfer kernel, so
+// `constitutive` gains nothing). `s.energyObserver` and
+// `s.mechanicalDiagnostics` need the host-side grid of the reference step
+// (unforced momenta, per-node wall work) and are refused with an error.
 const native = require('./mpm_napi.node');
 
 const DEFAULT_DOMAIN = Object.freeze({lower: [-64, -48, -64], upper: [64, 72, 64], cellSize: 8, particlesPerAxis: 2});
@@ -35,7 +49,7 @@ function det(a) {
 class MPM3 {
   constructor(capacity, options = {}) {
     this.capacity = capacity;
-    this.options = Object.assign({sortEvery: 8, resident: false}, options);
+    this.options = Object.assign({sortEvery: 8, resident: false, guard: false}, options);
     this.F = new Float64Array(capacity * 9); this.C = new Float64Array(capacity * 9);
     this.Jp = new Float64Array(capacity); this.dilation = new Float64Array(capacity);
     this.stress = new Float64Array(capacity); this.plastic = new Float64Array(capacity);
@@ -56,9 +70,14 @@ class MPM3 {
     if (this.engine) native.destroy(this.engine);
     this.engine = native.create({lower: domain.lower, upper: domain.upper, cellSize: domain.cellSize,
       capacity: this.capacity, wallMode: [0, 0, 0, 0, 0, 0], gravity: [0, 0, 0], gravityOnParticles: 0,
-      massEps: 1e-12, sortEvery: this.options.sortEvery, strictDomain: 0});
+      massEps: 1e-12, sortEvery: this.options.sortEvery, strictDomain: 0,
+      guardOff: this.options.guard ? 0 : 1, cflNumber: 1.4 / 8, cflCap: domain.cellSize / 8 / 240});
     this.onDevice = 0;
   }
+
+  // The state guard's finding since the last call: 0, 7 (CFL exceeded) or 8
+  // (invalid state), as the GPU service reports them.
+  checkState() { return native.checkState(this.engine); }
 
   reset() { this.C.fill(0); this.F.fill(0); for (let p = 0; p < this.F.length / 9; p++) this.resetParticle(p); }
 
@@ -185,11 +204,67 @@ class MPM3 {
   }
 
   step(s, dt) {
-    if (s.separatePartGrids) { this._stepParts(s, dt); return; }
-    if (!this.options.resident || !this.onDevice) this.upload(s);
-    if (s.externalGridContact || s.contactGridDeltas) this._splitStep(s, dt);
-    else native.step(this.engine, dt, 1);
-    if (!this.options.resident) this.download(s);
+    if (s.energyObserver) {
+      throw new Error('MPM3 on the MI355X engine: s.energyObserver needs the host grid of the reference step ' +
+        '(unforced momenta); it is not supported');
+    }
+    if (s.mechanicalDiagnostics) {
+      throw new Error('MPM3 on the MI355X engine: s.mechanicalDiagnostics needs the per-node wall work of the ' +
+        'reference grid loop; it is not supported');
+    }
+    const profiled = !!s.profile;
+    if (profiled) native.profile(this.engine, 1, 1);
+    if (s.separatePartGrids) {
+      this._stepParts(s, dt);
+    } else {
+      if (!this.options.resident || !this.onDevice) this.upload(s);
+      // The stress this step applies (src/three/mpm.js:37), as the reference
+      // leaves it in this.tau; resident mode keeps it on the device.
+      if (!this.options.resident) this._readTau();
+      if (s.externalGridContact || s.contactGridDeltas || (s.press && s.press.enabled)) this._splitStep(s, dt);
+      else native.step(this.engine, dt, 1);
+      if (!this.options.resident) this.download(s);
+    }
+    if (profiled) this._addProfile(s.profile);
+  }
+
+  // this.tau of the particles on the device (9 values each, reference units).
+  _readTau() {
+    const ids = this._ids;
+    if (!ids || !ids.length) return;
+    const t = new Float64Array(ids.length * 9);
+    native.stressTensors(this.engine, t);
+    for (let k = 0; k < ids.length; k++) this.tau.set(t.subarray(k * 9, k * 9 + 9), ids[k] * 9);
+  }
+
+  // Device kernel times into the reference's profile buckets (milliseconds,
+  // src/three/mpm.js:62,86,105,124,133).
+  _addProfile(profile) {
+    const engines = this._children ? [...this._children.values()].map((c) => c.engine) : [this.engine];
+    for (const e of engines) {
+      const t = native.kernelTimes(e);
+      native.profile(e, 0, 1);
+      for (const [name, ms] of Object.entries(t)) {
+        const bucket = name === 'grid' || name === 'pieces' ? 'grid' : 'transfer';
+        profile[bucket] = (profile[bucket] || 0) + ms;
+      }
+    }
+  }
+
+  // The platen of the reference's grid loop (src/three/mpm.js:97-101) on a
+  // downloaded grid {mass, v, fixed}: nodes at or above press.y - H/2 that no
+  // pinned particle holds move down at most at press.velocity, and the
+  // velocity taken away, times the node mass, adds to press.impulse.
+  _applyPress(s, f) {
+    const press = s.press, N = this.N, H = this.H, O1 = this.O[1];
+    const y0 = press.y - H * .5;
+    for (let n = 0; n < f.mass.length; n++) {
+      if (!(f.mass[n] > 1e-12) || f.fixed[n]) continue;
+      if (O1 + (Math.floor(n / N) % N) * H < y0) continue;
+      const before = f.v[n * 3 + 1], after = Math.min(before, press.velocity);
+      f.v[n * 3 + 1] = after;
+      press.impulse += (before - after) * f.mass[n];
+    }
   }
 
   // One engine per part for s.separatePartGrids (src/three/mpm.js:58,69):
@@ -215,7 +290,16 @@ class MPM3 {
     for (let p = 0; p < s.count; p++) if (!s.isSolid || s.isSolid(p)) parts.add(s.lab && s.lab.part ? s.lab.part[p] : 0);
     const kids = [...parts].sort((a, b) => a - b).map((k) => this._child(k));
     const n3 = this.N * this.N * this.N;
-    for (const c of kids) { c.upload(s); native.stepToGrid(c.engine, dt); }
+    for (const c of kids) { c.upload(s); c._readTau(); native.stepToGrid(c.engine, dt); }
+    const pressed = !!(s.press && s.press.enabled);
+    if (pressed && !s.externalGridContact) {
+      const f = {mass: new Float64Array(n3), v: new Float64Array(3 * n3), fixed: new Uint8Array(n3)};
+      for (const c of kids) {
+        native.gridDownload(c.engine, f.mass, f.v, f.fixed);
+        this._applyPress(s, f);
+        native.gridUploadVelocity(c.engine, f.v);
+      }
+    }
     if (s.externalGridContact) {
       const count = Math.max((s.projectPartCount || 0) + 1, ...kids.map((c) => c._part + 1));
       if (!this.fields || this.fields.length !== count || !this.fields[0] || this.fields[0].mass.length !== n3)
@@ -225,7 +309,11 @@ class MPM3 {
         this.mass = new Float64Array(n3); this.v = new Float64Array(3 * n3); this.fixed = new Uint8Array(n3);
       }
       for (const f of this.fields) { f.mass.fill(0); f.v.fill(0); f.fixed.fill(0); }
-      for (const c of kids) { const f = this.fields[c._part]; native.gridDownload(c.engine, f.mass, f.v, f.fixed); }
+      for (const c of kids) {
+        const f = this.fields[c._part];
+        native.gridDownload(c.engine, f.mass, f.v, f.fixed);
+        if (pressed) this._applyPress(s, f);
+      }
       for (let p = 0; p < s.count; p++) {
         const x = [s.x[p], s.y[p], s.z[p]];
         for (let a = 0; a < 3; a++) {
@@ -251,18 +339,22 @@ class MPM3 {
   _splitStep(s, dt) {
     native.stepToGrid(this.engine, dt);
     const n3 = this.N * this.N * this.N;
-    if (s.externalGridContact) {
+    const pressed = !!(s.press && s.press.enabled);
+    if (s.externalGridContact || pressed) {
       if (!this.mass || this.mass.length !== n3) {
         this.mass = new Float64Array(n3); this.v = new Float64Array(3 * n3); this.fixed = new Uint8Array(n3);
       }
       native.gridDownload(this.engine, this.mass, this.v, this.fixed);
-      for (let p = 0; p < s.count; p++) {  // the step's stencil bases (src/three/mpm.js:25-28)
-        const x = [s.x[p], s.y[p], s.z[p]];
-        for (let a = 0; a < 3; a++) {
-          this.base[p * 3 + a] = Math.max(0, Math.min(this.N - 3, Math.floor((x[a] - this.O[a]) / this.H - .5)));
+      if (pressed) this._applyPress(s, this);
+      if (s.externalGridContact) {
+        for (let p = 0; p < s.count; p++) {  // the step's stencil bases (src/three/mpm.js:25-28)
+          const x = [s.x[p], s.y[p], s.z[p]];
+          for (let a = 0; a < 3; a++) {
+            this.base[p * 3 + a] = Math.max(0, Math.min(this.N - 3, Math.floor((x[a] - this.O[a]) / this.H - .5)));
+          }
         }
+        s.externalGridContact(this, dt);
       }
-      s.externalGridContact(this, dt);
       native.gridUploadVelocity(this.engine, this.v);
     }
     const slots = new Int32Array(64).fill(-1), arrays = [];
@@ -338,6 +430,7 @@ class MPM3 {
     native.refreshDiagnostics(this.engine, out);
     this.stress.fill(0, 0, s.count);
     for (let k = 0; k < this._ids.length; k++) this.stress[this._ids[k]] = out[k];
+    this._readTau();  // updateStress also leaves this.tau (src/three/mpm.js:37)
   }
 
   minJ(s) {

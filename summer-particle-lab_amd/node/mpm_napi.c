@@ -105,6 +105,9 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   c.mass_eps = get_num(env, argv[0], "massEps", 1e-12);
   c.sort_every = (int32_t)get_num(env, argv[0], "sortEvery", 8);
   c.strict_domain = (int32_t)get_num(env, argv[0], "strictDomain", 0);
+  c.guard_off = (int32_t)get_num(env, argv[0], "guardOff", 0);
+  c.cfl_number = get_num(env, argv[0], "cflNumber", 0);
+  c.cfl_cap = get_num(env, argv[0], "cflCap", 0);
   mpm_engine* e = NULL;
   int rc = mpm_create(&c, &e);
   if (rc) return throw_mpm(env, rc, NULL);
@@ -872,6 +875,69 @@ static napi_value js_refresh_diagnostics(napi_env env, napi_callback_info info) 
   return NULL;
 }
 
+/* stressTensors(engine, tau Float64Array [9 count]): MPM3.tau of the resident
+ * state (mpm_stress_tensors). */
+static napi_value js_stress_tensors(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 2 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  uint32_t n = 0;
+  mpm_count(e, &n);
+  void* data = NULL;
+  if (typed(env, argv[1], napi_float64_array, 9 * (size_t)n, &data)) return NULL;
+  int rc = mpm_stress_tensors(e, (double*)data);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+/* checkState(engine) -> 0, 7 or 8: waits for the engine and returns the state
+ * guard's finding as the GPU service's status code (mpm_synchronize); other
+ * failures throw. */
+static napi_value js_check_state(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], r;
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  const int rc = mpm_synchronize(e);
+  if (rc && rc != MPM_ERR_CFL && rc != MPM_ERR_INVALID_STATE) return throw_mpm(env, rc, e);
+  CHECK_NAPI(env, napi_create_int32(env, rc, &r));
+  return r;
+}
+
+/* profile(engine, enable, reset): per-kernel timing on or off (mpm_profile). */
+static napi_value js_profile(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  int32_t enable = 1, reset = 1;
+  if (argc >= 2) napi_get_value_int32(env, argv[1], &enable);
+  if (argc >= 3) napi_get_value_int32(env, argv[2], &reset);
+  int rc = mpm_profile(e, enable, reset);
+  if (rc) return throw_mpm(env, rc, e);
+  return NULL;
+}
+
+/* kernelTimes(engine) -> {name: milliseconds} accumulated while profiling. */
+static napi_value js_kernel_times(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  mpm_engine* e = argc >= 1 ? get_engine(env, argv[0]) : NULL;
+  if (!e) return NULL;
+  mpm_kernel_time t[32];
+  int32_t n = 0;
+  int rc = mpm_kernel_times(e, t, 32, &n);
+  if (rc) return throw_mpm(env, rc, e);
+  CHECK_NAPI(env, napi_create_object(env, &out));
+  for (int32_t k = 0; k < n && k < 32; ++k) set_num(env, out, t[k].name, t[k].total_ms);
+  return out;
+}
+
 static napi_value js_last_error(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], r;
@@ -897,6 +963,10 @@ static napi_value init(napi_env env, napi_value exports) {
              {"stableDt", js_stable_dt},
              {"conservation", js_conservation},
              {"refreshDiagnostics", js_refresh_diagnostics},
+             {"stressTensors", js_stress_tensors},
+             {"checkState", js_check_state},
+             {"profile", js_profile},
+             {"kernelTimes", js_kernel_times},
              {"snapshot", js_snapshot},
              {"framePacket", js_frame_packet},
              {"stepToGrid", js_step_to_grid},

@@ -22,7 +22,10 @@ if (input.pinned) for (const i of input.pinned) s.pinned[i] = 1;
 if (input.part) for (let i = 0; i < n; i++) s.lab.part[i] = input.part[i];
 if (input.settings) Object.assign(s.settings, input.settings);
 if (input.boundsLo) { s.boundsLo = input.boundsLo; s.boundsHi = input.boundsHi; }
-const mpm = new MPM3(n, {sortEvery: 1});
+const mpm = new MPM3(n, {sortEvery: 1, guard: !!input.guard});
+// The platen (src/three/mpm.js:97-101) and the profile buckets (:62-133).
+if (input.press) s.press = Object.assign({enabled: true, impulse: 0}, input.press);
+if (input.profile) s.profile = {constitutive: 0, transfer: 0, grid: 0};
 // Grid hooks (src/three/mpm.js:104,114): a rigid plate below plateY that
 // stops nodes moving into it, and a constant velocity delta for one part.
 if (input.plateY !== undefined) {
@@ -46,6 +49,9 @@ const dtStable = mpm.stableStep(s);
 const frames = [];
 if (input.frames) for (let k = 0; k < input.frames; k++) frames.push(mpm.advanceFrame(s, input.dt));
 else for (let k = 0; k < input.steps; k++) mpm.step(s, input.dt);
+// this.tau as the last step left it: the stress of its start state.
+const tauStep = Array.from(mpm.tau.subarray(0, n * 9));
+const status = input.guard ? mpm.checkState() : null;
 mpm.refreshDiagnostics(s);
 let snapshot = null;
 if (input.snapshot) {
@@ -56,6 +62,7 @@ const out = {dtStable, x: Array.from(s.x), y: Array.from(s.y), z: Array.from(s.z
   vy: Array.from(s.vy), vz: Array.from(s.vz), F: Array.from(mpm.F.subarray(0, n * 9)),
   plastic: Array.from(mpm.plastic.subarray(0, n)), stress: Array.from(mpm.stress.subarray(0, n)),
   jp: Array.from(mpm.Jp.subarray(0, n)), dilation: Array.from(mpm.dilation.subarray(0, n)),
-  minJ: mpm.minJ(s), conservation: mpm.conservation(), frames, snapshot};
+  minJ: mpm.minJ(s), conservation: mpm.conservation(), frames, snapshot, tauStep,
+  tau: Array.from(mpm.tau.subarray(0, n * 9)), press: s.press || null, profile: s.profile || null, status};
 mpm.destroy();
 console.log(JSON.stringify(out));

@@ -40,7 +40,8 @@ console.log(JSON.stringify({{names, code, typeErr}}));
                                    "stableDt", "conservation", "refreshDiagnostics", "lastError", "advanceFrame",
                                    "snapshot", "framePacket", "create2", "destroy2", "upload2", "download2",
                                    "step2", "stableDt2", "advance2", "stats2", "stepToGrid", "gridDownload",
-                                   "gridUploadVelocity", "stepFromGrid"])
+                                   "gridUploadVelocity", "stepFromGrid", "stressTensors", "checkState", "profile",
+                                   "kernelTimes"])
     assert out["code"] == "MPM1"
     assert "engine handle" in out["typeErr"]
 
@@ -107,6 +108,87 @@ def test_drop_in_mpm3_matches_oracle_on_reference_scene(gpu, tmp_path):
     assert got["conservation"]["gridMassQ"] == got["conservation"]["particleMassQ"]
     orc.refresh_diagnostics()
     np.testing.assert_allclose(np.array(got["stress"]), orc.stress, atol=3e-5, rtol=5e-3)
+    # MPM3.tau (src/three/mpm.js:37): the Kirchhoff stress of the current
+    # state after refreshDiagnostics, in the reference's units
+    tau_scale = np.abs(orc.tau).max()
+    assert tau_scale > 1.0
+    np.testing.assert_allclose(np.array(got["tau"]), orc.tau, atol=2e-3 * tau_scale, rtol=5e-3)
+
+
+def run_scene(tmp_path, inp, name="scene.json"):
+    path = tmp_path / name
+    path.write_text(json.dumps(inp))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "mpm3_scene.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    return r
+
+
+@pytest.mark.gpu
+def test_drop_in_tau_after_a_step_is_the_stress_the_step_applied(gpu, tmp_path):
+    """After MPM3.step, this.tau holds the stress of the state the step
+    started from (src/three/mpm.js:61 then :71): one step of the 4-block
+    scene from the preset F, against the oracle's tau of the same step."""
+    pts, models = four_blocks()
+    F = [.98, .1, 0, 0, .95, 0, 0, 0, 1.02]
+    r = run_scene(tmp_path, dict(points=pts.tolist(), velocity=[0, -3, 0], models=models.tolist(), F=F,
+                                 dt=1 / 240, steps=1, profile=True, guard=True))
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout)
+    orc = o.Scene(pts, (0, -3, 0), 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8, model=models)
+    orc.F[:] = np.tile(F, len(pts))
+    orc.step(1 / 240)
+    scale = np.abs(orc.tau).max()
+    np.testing.assert_allclose(np.array(got["tauStep"]), orc.tau, atol=1e-4 * scale, rtol=1e-4)
+    # s.profile: the device kernels land in the transfer and grid buckets
+    assert got["profile"]["transfer"] > 0 and got["profile"]["grid"] > 0 and got["profile"]["constitutive"] == 0
+    assert got["status"] == 0  # the guard found nothing at the reference's stable step
+
+
+@pytest.mark.gpu
+def test_drop_in_press_matches_oracle(gpu, tmp_path):
+    """The platen (s.press, src/three/mpm.js:97-101) on a block moving up
+    into it and one pressed down by it, 12 steps: positions to the reference's
+    bar, the accumulated impulse to 1e-3, and the impulse equal to the
+    momentum the block lost (no gravity, walls out of reach)."""
+    pts, models = four_blocks()
+    pts = pts[:27].copy()
+    top = pts[:, 1].max()
+    for press, v in ((dict(y=top + 4.0, velocity=0.0), (0, 40, 0)), (dict(y=top + 2.0, velocity=-20.0), (0, 0, 0))):
+        r = run_scene(tmp_path, dict(points=pts.tolist(), velocity=list(v), models=[0] * 27, dt=1 / 480, steps=12,
+                                     press=press))
+        assert r.returncode == 0, r.stderr[-2000:]
+        got = json.loads(r.stdout)
+        orc = o.Scene(pts, v, 1.1, 1.0, lower=(-64, -48, -64), upper=(64, 72, 64), h=8, model=0)
+        orc.set_press(press["y"], press["velocity"])
+        p0 = (orc.mass * orc.vy).sum()
+        for _ in range(12):
+            orc.step(1 / 480)
+        assert orc.press_impulse != 0
+        assert orc.press_impulse == pytest.approx(p0 - (orc.mass * orc.vy).sum(), rel=1e-9)
+        x = np.stack([got["x"], got["y"], got["z"]], 1)
+        np.testing.assert_allclose(x, orc.positions(), atol=.003, rtol=.001)
+        assert got["press"]["impulse"] == pytest.approx(orc.press_impulse, rel=1e-3)
+
+
+def test_drop_in_refuses_the_observers_it_cannot_honour():
+    """s.energyObserver and s.mechanicalDiagnostics need the reference's host
+    grid; the drop-in throws before touching the engine instead of ignoring
+    them."""
+    r = node(f"""
+const {{MPM3}} = require({json.dumps(os.path.join(NODE_DIR, 'index.js'))});
+const proto = MPM3.prototype;
+const out = [];
+for (const key of ['energyObserver', 'mechanicalDiagnostics']) {{
+  const s = {{count: 0}};
+  s[key] = {{}};
+  try {{ proto.step.call({{}}, s, 1e-3); out.push(null); }} catch (e) {{ out.push(e.message); }}
+}}
+console.log(JSON.stringify(out));
+""")
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert "energyObserver" in out[0] and "not supported" in out[0]
+    assert "mechanicalDiagnostics" in out[1] and "not supported" in out[1]
 
 
 @pytest.mark.gpu
