@@ -100,7 +100,7 @@ typedef struct {
   /* State guard: the checks of the reference GPU service's check_state
    * (gpu/mixed_solver.py:381-388), made inside every step's particle kernel.
    * CFL (status 7): the stable step of the state a step starts from,
-   * min(cfl_cap, cfl_number h / max(wave, 1), .08 / max(sum |C|, 1)) with
+   * min(cfl_cap, cfl_number h / max(wave, 1), grad_number / max(sum |C|, 1)) with
    * wave = |v| + c max(1, |F|row |F|col, compression) (src/three/mpm.js:43-55;
    * c = sqrt((lambda + 2 mu) hardening / rho)), is below 0.99999 dt.
    * Invalid state (status 8): after G2P, det F outside [1e-6, 1e6] or not a
@@ -112,6 +112,8 @@ typedef struct {
   double cfl_number;         /* <= 0: the reference's 1.4/8 */
   double cfl_cap;            /* > 0: the stable step's cap term (the reference's
                                 h/8/240 in its own units); <= 0: no cap */
+  double grad_number;        /* bound of dt sum |C| (the deformation of one
+                                step); <= 0: the reference's .08 */
 } mpm_config;
 
 /* One material. Strengths are normalised by E as in the reference records
@@ -232,7 +234,7 @@ int mpm_download(mpm_engine* e, mpm_particles* p);
 
 /* Advance nsteps steps of size dt; replaces MPM3.step(s, dt)
  * (src/three/mpm.js:56-134). Asynchronous on the engine stream. With the
- * guard on, a dt above cfl_cap, cfl_number h or .08 (the stable step of any
+ * guard on, a dt above cfl_cap, cfl_number h or grad_number (the stable step of any
  * state) gives MPM_ERR_CFL at once and enqueues nothing; the state-dependent
  * checks are made on the device and reported by mpm_synchronize. */
 int mpm_step(mpm_engine* e, double dt, int32_t nsteps);
@@ -445,8 +447,11 @@ int mpm_return_map_state_batch(const mpm_material* m, uint32_t n, float dt, cons
  * packed W1[64x7] b1 W2[64x64] b2 W3[64x64] b3 W4[6x64] b4, weights
  * output-major) maps the trial elastic logarithmic strain eps (xx, yy, zz,
  * xy, xz, yz) and the temperature T to the deviatoric part of the
- * return-mapped Kirchhoff stress. Inputs are eps * in_scale and
- * (T - T_center) * T_scale; outputs are in units of out_scale * E. It replaces
+ * return-mapped Kirchhoff stress, both in the units of the yield point at T:
+ * inputs are eps sqrt(6) (mu/E) / y(T) * in_scale and (T - T_center) *
+ * T_scale, outputs are in units of out_scale * y(T) * E, with y(T) the
+ * E-normalised yield stress (a perfectly plastic J2 return is then the same
+ * map at every temperature). It replaces
  * the plastic return of projectPlastic (src/three/constitutive.js:52-78) for
  * rate-independent J2 models without hardening (model 4; model 10 with
  * H = 0), where the return depends on eps and T only. */
@@ -463,21 +468,24 @@ typedef struct {
   float dt;            /* step passed to the return map */
   float eps_max;       /* envelope half-width of every eps component; <= 0: 4 sigma */
   float near_yield_fraction; /* share of samples drawn near the yield surface:
-                                |dev eps| uniform in [0.8, 2.5] times the yield
-                                strain at T, random direction, volumetric
-                                part N(0, sigma); the rest is N(0, sigma) */
+                                |dev eps| = 1 + 1.5 u^2 (u uniform) times the
+                                yield strain at T, random direction, volumetric
+                                part N(0, sigma); the rest is N(0, sigma).
+                                Draws inside the yield surface get their
+                                deviatoric part stretched to 1 + 1.5 u times
+                                the yield strain: all samples are plastic */
 } mpm_mlp_train_config;
 
 typedef struct {
   float in_scale, T_center, T_scale;
-  float out_scale;     /* output unit in units of E (the yield ratio A);
-                          the stress is output * out_scale * E */
+  float out_scale;     /* output unit in units of y(T) E; the deviatoric
+                          stress is output * out_scale * y(T) * E */
   float eps_max;       /* envelope: |eps_ij| <= eps_max */
   float T_lo, T_hi;    /* envelope: T_lo <= T <= T_hi */
   float min_yield_ratio; /* envelope: the network takes trial states with
-                            q >= min_yield_ratio * yield; states just past
-                            yield (where a 64-wide network is least
-                            accurate) take the plain return */
+                            q >= min_yield_ratio * yield (1: every state past
+                            yield); the ones between yield and that take
+                            the plain return */
   int32_t material;    /* set by mpm_set_surrogate */
   /* the training material, in E-normalised form: a network applies to any
    * material with the same model, mu/E, lambda/E, yield ratio and thermal
@@ -486,9 +494,9 @@ typedef struct {
   float mu_over_E, lambda_over_E, A, Tref, Thot, hotRatio;
 } mpm_mlp_norm;
 
-/* Normalisation for material m and a training config: in_scale 1/sigma,
- * T mapped to [-1, 1], out_scale = A (the cold yield stress over E),
- * min_yield_ratio 1.2, and m's E-normalised fingerprint. */
+/* Normalisation for material m and a training config: in_scale 1/2, T
+ * mapped to [-1, 1], out_scale 1, min_yield_ratio 1, and m's E-normalised
+ * fingerprint. */
 int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, mpm_mlp_norm* out);
 
 /* MPM_OK when material m matches norm's training fingerprint (E-normalised

@@ -179,15 +179,16 @@ inline void finish_conservation(const double* h, int K, int nb, const double* gr
 // (mpm_config's guard fields): a particle violates the stable step when
 // cfl h / max(wave, 1) < 0.99999 dt, i.e. wave > cfl h / (0.99999 dt) (the
 // floor of 1 cannot save it once that limit is >= 1), or when
-// .08 / max(sum |C|, 1) < 0.99999 dt. Returns MPM_ERR_CFL when no state can
-// pass: the cap term, or a limit below the floor of 1.
+// grad / max(sum |C|, 1) < 0.99999 dt (grad = grad_number, the reference's .08
+// by default). Returns MPM_ERR_CFL when no state can pass: the cap term, or
+// a limit below the floor of 1.
 inline int guard_params(const mpm_config& c, double dt, StepParams& sp) {
   sp.guard = c.guard_off ? 0u : 1u;
   sp.wave_limit = sp.grad_limit = 3.0e38f;
   if (!sp.guard) return MPM_OK;
   const double lim = dt * 0.99999;
   const double cfl = c.cfl_number > 0 ? c.cfl_number : 1.4 / 8;
-  const double wave = cfl * c.cell_size / lim, grad = .08 / lim;
+  const double wave = cfl * c.cell_size / lim, grad = (c.grad_number > 0 ? c.grad_number : .08) / lim;
   if ((c.cfl_cap > 0 && c.cfl_cap < lim) || wave < 1 || grad < 1) return MPM_ERR_CFL;
   sp.wave_limit = (float)std::min(wave, 3.0e38);
   sp.grad_limit = (float)std::min(grad, 3.0e38);

@@ -40,11 +40,18 @@ __device__ __forceinline__ void sym_eig(const float e[6], float V[9], float lam[
 
 // One training sample: T uniform in [T_lo, T_hi]; log strain eps ~ N(0,
 // sigma) per component, or (for a share near of the samples) a deviatoric
-// part of norm uniform in [0.8, 2.5] times the yield strain at T in a random
-// direction plus a volumetric part N(0, sigma); F = exp(eps) (the trial
-// state with no rotation); the plain return map of the step gives the
+// part of norm 1 + 1.5 u^2 (u uniform: denser at the yield surface, where the
+// trial states of a flowing scene sit) times the yield strain at T in a
+// random direction plus a volumetric part N(0, sigma). Every sample lies on the
+// plastic side of the yield surface: a N(0, sigma) draw inside it has its
+// deviatoric part stretched to 1 + 1.5 u times the yield strain. The network
+// serves only trial states past yield (the elastic ones take the closed
+// form), and without the elastic side the map it fits has no kink at the
+// yield surface, where the states just past yield crowd. F = exp(eps) (the
+// trial state with no rotation); the plain return map of the step gives the
 // Kirchhoff stress; the target is its deviatoric part in units of
-// out_scale * E. Inputs and targets feature-major.
+// out_scale * y(T) * E, the input the strain in units of the yield strain at
+// T (eps sqrt(6) G / y(T)) times in_scale. Inputs and targets feature-major.
 __global__ __launch_bounds__(TB) void k_mlp_samples(uint32_t seed, uint32_t n, float sigma, float T_lo, float T_hi,
                                                     float dt, float near, DevMaterial mat, MlpNorm nrm,
                                                     float* __restrict__ X, float* __restrict__ Y) {
@@ -54,20 +61,33 @@ __global__ __launch_bounds__(TB) void k_mlp_samples(uint32_t seed, uint32_t n, f
   float e[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) e[c] = gauss(seed, i, c);
+  // yield strain at T: |dev eps| at which q = sqrt(6) G |dev eps| reaches y
+  const float ey = surrogate_yield(T, mat) / (2.4494897427831781f * mat.G);
   if (uhash(seed, i, 101) < near) {
     const float m = (e[0] + e[1] + e[2]) * (1.f / 3.f);
     e[0] -= m;
     e[1] -= m;
     e[2] -= m;
     const float len = sym_norm(e);
-    const float ey = surrogate_yield(T, mat) / (2.4494897427831781f * mat.G);
-    const float r = ey * (0.8f + 1.7f * uhash(seed, i, 102)) / fmaxf(len, 1e-20f);
+    const float u = uhash(seed, i, 102);
     const float v = sigma * gauss(seed, i, 7);
+    // J = exp(3 v) divides q: the yield strain of this volume is ey J
+    const float r = ey * expf(3.f * v) * (1.f + 1.5f * u * u) / fmaxf(len, 1e-20f);
 #pragma unroll
     for (int c = 0; c < 6; ++c) e[c] = e[c] * r + (c < 3 ? v : 0.f);
   } else {
 #pragma unroll
     for (int c = 0; c < 6; ++c) e[c] *= sigma;
+    const float m = (e[0] + e[1] + e[2]) * (1.f / 3.f);
+    float d[6] = {e[0] - m, e[1] - m, e[2] - m, e[3], e[4], e[5]};
+    const float len = sym_norm(d);
+    // J = exp(tr eps) divides q: the yield strain of this volume is ey J
+    const float eyJ = ey * expf(3.f * m);
+    if (len < eyJ) {
+      const float r = eyJ * (1.f + 1.5f * uhash(seed, i, 103)) / fmaxf(len, 1e-20f);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) e[c] = d[c] * r + (c < 3 ? m : 0.f);
+    }
   }
   float V[9], lam[3];
   sym_eig(e, V, lam);
@@ -78,10 +98,12 @@ __global__ __launch_bounds__(TB) void k_mlp_samples(uint32_t seed, uint32_t n, f
   plastic_project(F, 0.f, dt, T, mat, tau);
   const float p = (tau[0] + tau[1] + tau[2]) * (1.f / 3.f);
   const float dev[6] = {tau[0] - p, tau[1] - p, tau[2] - p, tau[3], tau[4], tau[5]};
+  const float ys = surrogate_yield(T, mat);
+  const float kin = nrm.in_scale * 2.4494897427831781f * mat.G / ys;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) X[(size_t)c * n + i] = e[c] * nrm.in_scale;
+  for (int c = 0; c < 6; ++c) X[(size_t)c * n + i] = e[c] * kin;
   X[(size_t)6 * n + i] = (T - nrm.T_center) * nrm.T_scale;
-  const float inv = 1.f / (nrm.out_scale * mat.E);
+  const float inv = 1.f / (nrm.out_scale * ys * mat.E);
 #pragma unroll
   for (int c = 0; c < 6; ++c) Y[(size_t)c * n + i] = dev[c] * inv;
 }
@@ -371,20 +393,19 @@ extern "C" {
 
 int mpm_mlp_default_norm(const mpm_material* m, const mpm_mlp_train_config* c, mpm_mlp_norm* out) {
   if (!m || !c || !out || !(c->strain_sigma > 0) || !(c->T_hi > c->T_lo)) return MPM_ERR_INVALID_ARGUMENT;
-  out->in_scale = 1.f / c->strain_sigma;
+  out->in_scale = 0.5f;
   out->T_center = 0.5f * (c->T_lo + c->T_hi);
   out->T_scale = 2.f / (c->T_hi - c->T_lo);
-  const float A = (m->model >= 10) ? m->th_A : m->pl_A;
-  out->out_scale = A;
+  out->out_scale = 1.f;
   out->eps_max = c->eps_max > 0 ? c->eps_max : 4.f * c->strain_sigma;
   out->T_lo = c->T_lo;
   out->T_hi = c->T_hi;
-  out->min_yield_ratio = 1.2f;
+  out->min_yield_ratio = 1.f;
   out->material = -1;
   out->model = m->model;
   out->mu_over_E = m->mu / m->E;
   out->lambda_over_E = m->lambda / m->E;
-  out->A = A;
+  out->A = (m->model >= 10) ? m->th_A : m->pl_A;
   out->Tref = m->th_Tref;
   out->Thot = m->th_Thot;
   out->hotRatio = m->th_hotRatio;

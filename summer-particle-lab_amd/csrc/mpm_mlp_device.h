@@ -31,9 +31,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Input/output scaling and the training envelope.
 struct MlpNorm {
-  float in_scale;   // 1 / sigma of the strain components
+  float in_scale;   // scale of the strain in units of the yield strain at T
   float T_center, T_scale;
-  float out_scale;  // output unit over E (the yield ratio A)
+  float out_scale;  // output unit over y(T) E (y the E-normalised yield stress)
   float eps_max;    // envelope: |eps_ij| <= eps_max
   float T_lo, T_hi; // envelope: T_lo <= T <= T_hi
   float min_yield_ratio;  // envelope: q >= min_yield_ratio * yield
@@ -290,11 +290,16 @@ __device__ __forceinline__ int surrogate_return(const MlpDev* __restrict__ M, bo
   float dtau[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (__any(use)) {
     float xin[7], yout[6];
+    // the strain in units of the yield strain at T, the output in units of
+    // the yield stress at T: the return of a perfectly plastic J2 material is
+    // the same map at every temperature in these units
+    const float ys = surrogate_yield(T, mat);
+    const float kin = nm.in_scale * 2.4494897427831781f * mat.G / ys;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xin[k] = eps[k] * nm.in_scale;
+    for (int k = 0; k < 6; ++k) xin[k] = eps[k] * kin;
     xin[6] = (T - nm.T_center) * nm.T_scale;
     mlp_forward_wave(M, use, xin, yout, wave_lds);
-    const float unit = nm.out_scale * mat.E;
+    const float unit = nm.out_scale * ys * mat.E;
 #pragma unroll
     for (int k = 0; k < 6; ++k) dtau[k] = yout[k] * unit;
   }

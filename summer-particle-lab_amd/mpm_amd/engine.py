@@ -68,9 +68,10 @@ def _ptr(a):
 
 def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
                 gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False, guard=True,
-                cfl_number=0.0, cfl_cap=0.0):
+                cfl_number=0.0, cfl_cap=0.0, grad_number=0.0):
     """An mpm_config. The state guard (codes 7 and 8 at synchronize) is on by
-    default with the reference's Courant number 1.4/8 and no cap term."""
+    default with the reference's Courant number 1.4/8, gradient bound .08 and
+    no cap term."""
     cfg = L.MpmConfig()
     cfg.lower[:] = list(map(float, lower))
     cfg.upper[:] = list(map(float, upper))
@@ -85,6 +86,7 @@ def make_config(lower, upper, cell_size, capacity=0, *, wall_mode=(0,) * 6, grav
     cfg.guard_off = 0 if guard else 1
     cfg.cfl_number = float(cfl_number)
     cfg.cfl_cap = float(cfl_cap)
+    cfg.grad_number = float(grad_number)
     return cfg
 
 
@@ -163,11 +165,12 @@ class Engine:
 
     def __init__(self, lower, upper, cell_size, capacity, *, wall_mode=(0,) * 6, gravity=(0.0, 0.0, 0.0),
                  gravity_on_particles=False, mass_eps=1e-12, sort_every=1, strict_domain=False, guard=True,
-                 cfl_number=0.0, cfl_cap=0.0):
+                 cfl_number=0.0, cfl_cap=0.0, grad_number=0.0):
         self._lib = L.load()
         cfg = make_config(lower, upper, cell_size, capacity, wall_mode=wall_mode, gravity=gravity,
                           gravity_on_particles=gravity_on_particles, mass_eps=mass_eps, sort_every=sort_every,
-                          strict_domain=strict_domain, guard=guard, cfl_number=cfl_number, cfl_cap=cfl_cap)
+                          strict_domain=strict_domain, guard=guard, cfl_number=cfl_number, cfl_cap=cfl_cap,
+                          grad_number=grad_number)
         h = C.c_void_p()
         L.check(self._lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h

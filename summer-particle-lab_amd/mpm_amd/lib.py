@@ -31,7 +31,7 @@ class MpmConfig(C.Structure):
         ("capacity", C.c_uint32), ("wall_mode", C.c_int32 * 6), ("gravity", C.c_double * 3),
         ("gravity_on_particles", C.c_int32), ("mass_eps", C.c_double), ("sort_every", C.c_int32),
         ("strict_domain", C.c_int32), ("guard_off", C.c_int32), ("cfl_number", C.c_double),
-        ("cfl_cap", C.c_double),
+        ("cfl_cap", C.c_double), ("grad_number", C.c_double),
     ]
 
 

@@ -22,16 +22,19 @@ _M32 = 0xFFFFFFFF
 
 def random(seed, index, attempt, axis):
     """Seeded uint32 hash mapped to [-1, 1); vectorised over numpy inputs."""
-    seed = np.asarray(seed, np.uint64)
+    # uint32 arithmetic wraps modulo 2^32 as the reference's Math.imul does.
+    u32 = np.uint32
+    seed = np.asarray(np.asarray(seed, np.int64) & _M32, u32)
     index = np.asarray(index, np.int64)
     attempt = np.asarray(attempt, np.int64)
     axis = np.asarray(axis, np.int64)
-    a = ((index + 1).astype(np.uint64) & _M32) * np.uint64(0x9E3779B9)
-    b = ((attempt * 3 + axis + 1).astype(np.uint64) & _M32) * np.uint64(0x85EBCA6B)
-    h = (seed + (a & _M32) + (b & _M32)) & _M32
-    h = ((h ^ (h >> np.uint64(16))) * np.uint64(0x7FEB352D)) & _M32
-    h = ((h ^ (h >> np.uint64(15))) * np.uint64(0x846CA68B)) & _M32
-    h = (h ^ (h >> np.uint64(16))) & _M32
+    with np.errstate(over="ignore"):
+        a = ((index + 1) & _M32).astype(u32) * u32(0x9E3779B9)
+        b = ((attempt * 3 + axis + 1) & _M32).astype(u32) * u32(0x85EBCA6B)
+        h = seed + a + b
+        h = (h ^ (h >> u32(16))) * u32(0x7FEB352D)
+        h = (h ^ (h >> u32(15))) * u32(0x846CA68B)
+        h = h ^ (h >> u32(16))
     return h.astype(np.float64) / 4294967296.0 * 2 - 1
 
 
@@ -91,25 +94,25 @@ def jitter(points, lo, hi, shape, seed, strength, spacing, fixed=None):
     fixed = np.zeros(n, bool) if fixed is None else np.asarray(fixed, bool)
     amplitude = spacing * strength
     out = p.copy()
-    done = np.zeros(n, bool)
-    idx = np.arange(n)
+    idx = np.arange(n)  # points still without an accepted attempt
     for attempt in range(8):
+        if idx.size == 0:
+            break
         d = np.stack([random(seed, idx, attempt, a) for a in range(3)], 1)
-        d[fixed, 0] = 0
+        d[fixed[idx], 0] = 0
         scale = amplitude / np.maximum(1.0, np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]))
-        c = p + d * scale[:, None]
+        c = p[idx] + d * scale[:, None]
         c = np.where(c < lo, 2 * lo - c, c)
         c = np.where(c > hi, 2 * hi - c, c)
         q = (c - (lo + hi) / 2) / ((hi - lo) / 2)
         if shape == "box":
-            ok = np.ones(n, bool)
+            ok = np.ones(idx.size, bool)
         elif shape == "ellipsoid":
             ok = q[:, 0] ** 2 + q[:, 1] ** 2 + q[:, 2] ** 2 <= 1 + 1e-12
         else:
             ok = q[:, 0] ** 2 + q[:, 2] ** 2 <= 1 + 1e-12
-        take = ok & ~done
-        out[take] = c[take]
-        done |= ok
+        out[idx[ok]] = c[ok]
+        idx = idx[~ok]
     return out
 
 
@@ -146,9 +149,10 @@ class Scene:
     gravity_on_particles: bool = False
     pinned: np.ndarray = None
     info: dict = field(default_factory=dict)
-    # Courant number of the state guard (mpm_config.cfl_number); 0: the
-    # reference's 1.4/8.
+    # Courant number and gradient bound of the state guard
+    # (mpm_config.cfl_number, grad_number); 0: the reference's 1.4/8 and .08.
     courant: float = 0.0
+    gradient: float = 0.0
 
     @property
     def n(self):
@@ -158,7 +162,7 @@ class Scene:
         from .engine import Engine
         e = Engine(self.lower, self.upper, self.h, capacity or self.n, wall_mode=self.wall_mode,
                    gravity=self.gravity, gravity_on_particles=self.gravity_on_particles, sort_every=sort_every,
-                   cfl_number=self.courant)
+                   cfl_number=self.courant, grad_number=self.gradient)
         e.set_materials(self.materials)
         e.upload(self.x, self.v, self.mass, self.volume, temperature=self.temperature, material=self.material,
                  pinned=self.pinned)
@@ -215,20 +219,30 @@ def j2_cube(nu=0.3, E=1e5, rho=1000.0, seed=42, strength=0.2, T=(300.0, 900.0), 
 # P-wave CFL number c_p dt / h stays 0.27 at h = 1/128, dt = 2e-5.
 STEEL_THERMAL = dict(Tref=300.0, Thot=1200.0, hotRatio=0.1, A=1.25e-3, H=0.0, kelvin_offset=0.0)
 STEEL_RHO_SCALED = 2.5e7
-# Courant number of the state guard for the workloads that fix dt at a P-wave
-# CFL number of 0.25-0.27: the reference's 1.4/8 is below that, and its wave
-# measure max(1, |F|row |F|col) grows up to 3 under rotation alone, so these
-# scenes flag a wave that crosses a whole cell in one step.
+# Guard constants of the workloads that fix dt at a P-wave CFL number of
+# 0.25-0.27: the reference's Courant number 1.4/8 is below that, and its wave
+# measure max(1, |F|row |F|col) grows up to 3 under rotation alone; its bound
+# .08 on dt sum |C| is passed where the batched scenes hit the walls at
+# 1 m/s. These scenes flag a wave that crosses a whole cell in one step and a
+# step that changes F by 100% (summed over the components).
 FIXED_DT_COURANT = 1.0
+FIXED_DT_GRADIENT = 1.0
 
 
 def thermal_boxes(cells=128, nboxes=6, ppa=4, seed=6, side=(0.05, 0.2), T=(300.0, 900.0), vsigma=0.5,
-                  dt=2e-5, steps=500, rho=STEEL_RHO_SCALED):
+                  dt=2e-5, steps=500, rho=STEEL_RHO_SCALED, per_particle=False, squeeze=0.0):
     """Thermal-J2 workload: 6 seeded non-overlapping boxes (sides uniform in
     [0.05, 0.2]) in the unit domain at 128^3 cells, 4 particles per axis per
     cell (~2M particles), per-box velocity N(0, 0.5 m/s) per component,
     per-particle temperature uniform in [300, 900] K, gravity -9.81,
-    separating walls, dt = 2e-5, 500 steps."""
+    separating walls, dt = 2e-5, 500 steps. With per_particle, every particle
+    draws its own velocity N(0, vsigma) per component (seeded hash normals):
+    neighbours then move apart and together; the grid transfer averages most
+    of that noise away, so only a few particles yield. With squeeze > 0 every
+    box also gets the vertical squeeze v_y = -squeeze (y - y_c) / (side/2)
+    (a uniaxial strain rate of 2 squeeze / side, 5-20/s at 0.5 m/s): the steel
+    yields within a few steps and flows plastically for the rest (the
+    flowing variant)."""
     h = 1.0 / cells
     spacing = h / ppa
     boxes = []
@@ -246,26 +260,35 @@ def thermal_boxes(cells=128, nboxes=6, ppa=4, seed=6, side=(0.05, 0.2), T=(300.0
     for k, (c, s) in enumerate(boxes):
         pts, vp = geometry(c, (s, s, s), "box", spacing, seed + k, 0.2)
         vel = np.array([vsigma * normal(seed, 10_000 + k, 0, a) for a in range(3)], float)
+        v = np.array(np.broadcast_to(vel, pts.shape))
+        if squeeze:
+            v[:, 1] -= squeeze * (pts[:, 1] - c[1]) / (s / 2)
         xs.append(pts)
-        vs.append(np.broadcast_to(vel, pts.shape))
+        vs.append(v)
         ms.append(np.full(len(pts), rho * vp))
         vols.append(np.full(len(pts), vp))
     x = np.concatenate(xs)
     n = len(x)
     temp = uniform(seed + 7, np.arange(n), 0, 0, T[0], T[1])
     mat = Material.from_E_nu(10, 2e11, 0.3, thermal=dict(STEEL_THERMAL))
-    return Scene("thermal_boxes", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
+    if per_particle:
+        ids = np.arange(n)
+        vs = [np.stack([vsigma * normal(seed + 11, ids, 0, a) for a in range(3)], 1)]
+    name = "thermal_boxes" + ("_particle_v" if per_particle else "") + ("_squeezed" if squeeze else "")
+    return Scene(name, (0, 0, 0), (1, 1, 1), h, x,
+                 np.concatenate(vs), np.concatenate(ms),
                  np.concatenate(vols), temp, np.zeros(n, np.uint8), [mat], dt, steps,
                  info=dict(boxes=[(c.tolist(), s) for c, s in boxes], rho=rho, spacing=spacing),
-                 courant=FIXED_DT_COURANT)
+                 courant=FIXED_DT_COURANT, gradient=FIXED_DT_GRADIENT)
 
 
 def large_scene(cells=256, ppa=2, seed=11, nspheres=32, radius=(0.03, 0.08), T=(300.0, 1200.0), dt=1e-5,
-                steps=1000, rho=STEEL_RHO_SCALED, block=(0.84, 0.14, 0.84), drop=1.0):
+                steps=1000, rho=STEEL_RHO_SCALED, block=(0.84, 0.14, 0.84), drop=1.0, gap=2.0):
     """Large single scene: 256^3 cells, 2 particles per axis per cell; a
     seeded dense thermal-J2 block (STEEL_THERMAL, material 0) on the floor
     plus `nspheres` fixed-corotated elastic spheres (material 1, radii
-    uniform in `radius`, non-overlapping, falling at `drop` m/s) above it;
+    uniform in `radius`, non-overlapping, at least `gap` cells above the
+    block, falling at `drop` m/s) above it;
     temperature from T[0] at x = 0 to T[1] at x = 1 with seeded noise of 30 K
     (clipped to T); gravity -9.81, separating walls, dt = 1e-5, 1000 steps.
     The density is scaled as in thermal_boxes (P-wave CFL 0.27 at h = 1/256,
@@ -287,7 +310,7 @@ def large_scene(cells=256, ppa=2, seed=11, nspheres=32, radius=(0.03, 0.08), T=(
         k = len(spheres)
         r = uniform(seed, k, attempt, 0, radius[0], radius[1])
         c = np.array([uniform(seed, k, attempt, 1, 0.05 + r, 0.95 - r),
-                      uniform(seed, k, attempt, 2, top + 2 * h + r, 0.95 - r),
+                      uniform(seed, k, attempt, 2, top + gap * h + r, 0.95 - r),
                       uniform(seed, k, attempt, 3, 0.05 + r, 0.95 - r)])
         if all(np.linalg.norm(c - c2) > r + r2 + 2 * h for c2, r2 in spheres):
             spheres.append((c, float(r)))
@@ -308,7 +331,7 @@ def large_scene(cells=256, ppa=2, seed=11, nspheres=32, radius=(0.03, 0.08), T=(
     return Scene("large_scene", (0, 0, 0), (1, 1, 1), h, x, np.concatenate(vs), np.concatenate(ms),
                  np.concatenate(vols), temp, np.concatenate(mats), materials, dt, steps,
                  info=dict(spheres=[(c.tolist(), r) for c, r in spheres], rho=rho, spacing=spacing, block=block),
-                 courant=FIXED_DT_COURANT)
+                 courant=FIXED_DT_COURANT, gradient=FIXED_DT_GRADIENT)
 
 
 @dataclass
@@ -331,7 +354,8 @@ class BatchScenes:
     def batch(self, sort_every=16, use_graph=False):
         from .engine import Batch, make_config
         cfgs = [make_config(sc.lower, sc.upper, sc.h, sc.n, wall_mode=sc.wall_mode, gravity=sc.gravity,
-                            gravity_on_particles=sc.gravity_on_particles, cfl_number=sc.courant)
+                            gravity_on_particles=sc.gravity_on_particles, cfl_number=sc.courant,
+                            grad_number=sc.gradient)
                 for sc in self.scenes]
         b = Batch(cfgs, self.counts, sort_every=sort_every, use_graph=use_graph)
         b.set_materials(self.materials)
@@ -370,6 +394,7 @@ def batched_scenes(nscenes=256, seed=21, counts=(2_000, 200_000), grids=(32, 64,
         temp = uniform(seed + 9, np.arange(n) + 1_000_000 * k, 0, 0, 300.0, 900.0) if mat == 1 else np.full(n, 300.0)
         sc = Scene(f"batch_{k}", (0, 0, 0), (1, 1, 1), h, pts, np.tile(vel, (n, 1)), np.full(n, rho * vp),
                    np.full(n, vp), temp, np.full(n, mat, np.uint8), materials, cfl * h / cp, steps,
-                   info=dict(cells=cells, per_axis=per_axis, side=side, material=mat), courant=FIXED_DT_COURANT)
+                   info=dict(cells=cells, per_axis=per_axis, side=side, material=mat), courant=FIXED_DT_COURANT,
+                   gradient=FIXED_DT_GRADIENT)
         out.append(sc)
     return BatchScenes(out, materials, steps, info=dict(seed=seed, counts=counts, grids=grids, cp=cp))

@@ -25,16 +25,16 @@ NPARAM = L.MLP_NPARAM
 class TrainConfig:
     samples: int = 4_000_000
     batch: int = 4096
-    epochs: int = 20
+    epochs: int = 150
     seed: int = 7
     lr: float = 2e-3
-    lr_final: float = 2e-5
+    lr_final: float = 1e-5
     strain_sigma: float = 0.02
     T_lo: float = 300.0
     T_hi: float = 900.0
     dt: float = 1e-4
     eps_max: float = 0.0  # <= 0: 4 sigma
-    near_yield_fraction: float = 0.3
+    near_yield_fraction: float = 0.7
 
     def to_c(self):
         c = L.MpmMlpTrainConfig()

@@ -108,6 +108,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   c.guard_off = (int32_t)get_num(env, argv[0], "guardOff", 0);
   c.cfl_number = get_num(env, argv[0], "cflNumber", 0);
   c.cfl_cap = get_num(env, argv[0], "cflCap", 0);
+  c.grad_number = get_num(env, argv[0], "gradNumber", 0);
   mpm_engine* e = NULL;
   int rc = mpm_create(&c, &e);
   if (rc) return throw_mpm(env, rc, NULL);

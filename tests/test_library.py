@@ -150,9 +150,9 @@ def test_surrogate_entry_points_validate_on_the_host():
                             T_lo=300.0, T_hi=900.0, dt=1e-4, eps_max=0.0, near_yield_fraction=0.3)
     n = L.MpmMlpNorm()
     assert lib.mpm_mlp_default_norm(C.byref(m), C.byref(c), C.byref(n)) == L.MPM_OK
-    assert n.in_scale == pytest.approx(50.0) and n.T_center == 600.0 and n.T_scale == pytest.approx(1 / 300)
-    assert n.out_scale == pytest.approx(0.02) and n.eps_max == pytest.approx(0.08) and n.material == -1
-    assert (n.T_lo, n.T_hi) == (300.0, 900.0) and n.min_yield_ratio == pytest.approx(1.2)
+    assert n.in_scale == pytest.approx(0.5) and n.T_center == 600.0 and n.T_scale == pytest.approx(1 / 300)
+    assert n.out_scale == pytest.approx(1.0) and n.eps_max == pytest.approx(0.08) and n.material == -1
+    assert (n.T_lo, n.T_hi) == (300.0, 900.0) and n.min_yield_ratio == pytest.approx(1.0)
     # the fingerprint is E-normalised: a material with another E but the same
     # ratios matches, one with another yield ratio does not
     assert lib.mpm_mlp_check_material(C.byref(m), C.byref(n)) == L.MPM_OK
