@@ -55,6 +55,7 @@ struct mpm_batch {
   std::vector<mpm_config> cfg;  // per scene (the guard settings)
   std::vector<double> cell;
   std::vector<int64_t> mass_q;
+  std::vector<uint64_t> occupied;  // distinct base cells per scene at upload
   std::vector<char> uploaded;
   std::vector<mpm_material> materials;
   DevState d{};
@@ -265,6 +266,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   b->cfg.assign(scenes, scenes + nscenes);
   b->cell.resize(nscenes);
   b->mass_q.assign(nscenes, 0);
+  b->occupied.assign(nscenes, 0);
   b->uploaded.assign(nscenes, 0);
   uint64_t p0 = 0, node0 = 0, block0 = 0;
   for (int32_t s = 0; s < nscenes; ++s) {
@@ -444,6 +446,14 @@ int mpm_batch_upload(mpm_batch* b, int32_t scene, const mpm_particles* p) {
   int64_t q = 0;
   for (uint32_t i = 0; i < n; ++i) q += (int64_t)std::nearbyint(planes[(size_t)P_MASS * n + i] * (float)scale);
   b->mass_q[scene] = q;
+  b->occupied[scene] = occupied_cells(planes.data(), n, n, sd.g);
+  uint64_t cells = 0, particles = 0;  // over the scenes uploaded so far: one launch serves all
+  for (int32_t s = 0; s < b->S; ++s) {
+    if (s != scene && !b->uploaded[s]) continue;
+    cells += b->occupied[s];
+    particles += b->sc[s].n;
+  }
+  b->d.narrow = narrow_chunks(particles, cells);
   sd.sp.mass_scale = (float)scale;
   sd.sp.inv_mass_scale = (float)(1.0 / scale);
   if (n) {

@@ -487,6 +487,7 @@ int mpm_upload(mpm_engine* e, const mpm_particles* p) {
   for (uint32_t i = 0; i < n; ++i) q += (int64_t)std::nearbyint(planes[(size_t)P_MASS * cap + i] * sf);
   e->particle_mass_q = q;
   e->n = n;
+  e->d.narrow = narrow_chunks(n, occupied_cells(planes.data(), cap, n, e->g));
   HIP_OK(e, hipMemcpyAsync(e->d.planes, planes.data(), sizeof(float) * planes.size(), hipMemcpyHostToDevice,
                            e->stream));
   if (n) {

@@ -38,15 +38,37 @@ namespace {
 constexpr int FB = 256;            // particles per chunk of the single-scene launch
 constexpr int TR_ROWS = 9;         // per-wave transpose: one row per node of a slab, {mass, momentum} per lane
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 entry: rows start in different banks
-constexpr int CHUNK = 16;          // lanes per scatter chunk (run pieces summed by one lane)
+constexpr int CHUNK_MAX = 16;      // lanes per scatter chunk (run pieces summed by one lane), at most
 constexpr int TR_TAIL = 16;        // zero entries after the last row
 constexpr int SORT_PASSES = 8;     // cells of a wave grouped by minimum passes before P2G
 constexpr int TR_ENTRIES = TR_ROWS * TR_STRIDE + TR_TAIL;
 constexpr int WAVE_BYTES = TR_ENTRIES * 16 + 64 * 16;
 static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0, "surrogate activations must fit the wave buffer");
-static_assert((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK <= TR_ENTRIES,
+static_assert((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK_MAX <= TR_ENTRIES,
               "the run-sum reads past a row's last lane stay inside the rows and the zero tail");
 static_assert(TR_ROWS + TR_TAIL <= 64, "one store per lane zeroes the row pads and the tail");
+static_assert(CHUNK_MAX <= TR_TAIL, "the zero tail covers a chunk read past the last row");
+static_assert(TR_ROWS + TR_TAIL + CHUNK_MAX <= 64,
+              "one store per lane zeroes the pads, the tail and the row past the passes");
+
+// Shape of the scatter: chunks of CH lanes and R nodes transposed per pass.
+// Dense cells (long runs) take 16-lane chunks over 9 rows; cells of about 8
+// particles take 8-lane chunks, whose task reads carry half the dead entries
+// (7 rows per pass keep a pass's tasks within one round of the wave up to 9
+// chunks). Measured at 64 and 8 particles per cell: 16 lanes 171 vs 214 us,
+// 8 lanes 1696 vs 1799 us of the fused kernel.
+template <int CH>
+struct Scatter {
+  static constexpr int CHUNK = CH;
+  static constexpr int R_PASS = CH == 8 ? 7 : 9;
+  static constexpr int NPASS = (27 + R_PASS - 1) / R_PASS;
+  static_assert(CH % 8 == 0 && CH <= CHUNK_MAX && R_PASS <= TR_ROWS, "scatter pass shape");
+};
+
+// Node g of the scatter order: the 26 nodes other than the centre (n = 13,
+// offset (1, 1, 1)) in index order, then the centre, which takes the
+// remainder of the fixed-point mass. Node index n = ii + 3 j + 9 k.
+__host__ __device__ constexpr int scatter_node(int g) { return g == 26 ? 13 : g + (g >= 13 ? 1 : 0); }
 
 // LDS plan of a workgroup of FBK particles (one per lane): a node box for the
 // scatter (20 B per node, 2 nodes per particle) and one transpose buffer per
@@ -209,7 +231,7 @@ __device__ __forceinline__ bool state_invalid(const M3& F, float ep, const float
 // The particle work of one chunk of FBK cell-sorted particles in slots
 // [first, min(first + FBK, end)). Grid pointers are those of the chunk's
 // scene; momentum planes are `pstride` floats apart.
-template <int MODE, bool SUR, int FBK, bool EXT>
+template <int MODE, bool SUR, int FBK, bool EXT, int CH>
 __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
                                             uint32_t first, uint32_t end, const DevMaterial* __restrict__ mats,
                                             const GridDims& g, const StepParams& sp, size_t pstride,
@@ -221,6 +243,9 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                                             const float4* __restrict__ deltas = nullptr,
                                             const int32_t* __restrict__ delta_slot = nullptr) {
   using S = Shape<FBK>;
+  constexpr int CHUNK = Scatter<CH>::CHUNK;
+  constexpr int R_PASS = Scatter<CH>::R_PASS;
+  constexpr int NPASS = Scatter<CH>::NPASS;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t i = first + tid;
@@ -605,9 +630,20 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const uint64_t chunk_starts = __ballot(((int)lane - run_start) % CHUNK == 0);
   const int nchunks = __popcll(chunk_starts);
 
-  if (lane < (uint32_t)(TR_ROWS + TR_TAIL))
-    s_tr[lane < (uint32_t)TR_ROWS ? lane * TR_STRIDE + 64 : TR_ROWS * TR_STRIDE + (lane - TR_ROWS)] =
-        make_float4(0.f, 0.f, 0.f, 0.f);
+  // Zeroed once per launch: the pad entry of every row, the tail, and, when
+  // a pass uses fewer rows than the buffer holds, the first CHUNK entries of
+  // the first row no pass writes (the run sums of the last row in use read
+  // into it).
+  {
+    constexpr int NZ_EXTRA = R_PASS < TR_ROWS ? CHUNK : 0;
+    const int l = (int)lane;
+    if (l < TR_ROWS + TR_TAIL + NZ_EXTRA) {
+      const int idx = l < TR_ROWS ? l * TR_STRIDE + 64
+                                  : (l < TR_ROWS + TR_TAIL ? TR_ROWS * TR_STRIDE + (l - TR_ROWS)
+                                                           : R_PASS * TR_STRIDE + (l - TR_ROWS - TR_TAIL));
+      s_tr[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
   if ((chunk_starts >> lane) & 1ull) {
     const int c = __popcll(chunk_starts & (upto >> 1));  // chunks before this position
     const uint64_t later = chunk_starts & ~upto;
@@ -615,15 +651,16 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     s_chunk[c] = make_int4((int)lane, next - 1, at.y, svalid ? (int)skey : -1);
   }
 
-  // Sum of one (node, chunk) task of slab k, added to the node box (or to
-  // the grid when the box does not fit). All CHUNK entries are read
-  // unconditionally, so the loads issue back to back; entries past the
-  // chunk's last lane (at most 15 past the row, still inside this wave's
-  // buffer) are masked out by the bits of `in`. Mass parts are below 2^23 in
-  // magnitude, so the 24-bit multiply-add sums them exactly.
-  auto run_task = [&](int nn, const int4 ch, int k) {
+  // Sum of one (row, chunk) task of a pass, for node n, added to the node
+  // box (or to the grid when the box does not fit). All CHUNK entries are
+  // read unconditionally, so the loads issue back to back; entries past the
+  // chunk's last lane (at most CHUNK - 1 past the row: the row's pad, the
+  // next row, or the zero tail, all finite) are masked out by the bits of
+  // `in`. Mass parts are below 2^23 in magnitude, so the 24-bit multiply-add
+  // sums them exactly.
+  auto run_task = [&](int row, int n, const int4 ch) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const f4* src = reinterpret_cast<const f4*>(s_tr + nn * TR_STRIDE + ch.x);
+    const f4* src = reinterpret_cast<const f4*>(s_tr + row * TR_STRIDE + ch.x);
     const uint32_t in = (2u << (ch.y - ch.x)) - 1u;
     int sq = 0;
     float px = 0.f, py = 0.f, pz = 0.f;
@@ -643,7 +680,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       }
     }
     if (ch.w < 0) return;
-    const int ii = nn % 3, j = nn / 3;
+    const int k = n / 9, j = (n / 3) % 3, ii = n % 3;
     const unsigned long long mq = (unsigned long long)(long long)sq;
     if (staged) {
       const int ln = ch.z + (k * ey + j) * ex + ii;
@@ -660,12 +697,12 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       pieces[node >> 6] = 1;
     }
   };
-  // The chunk of this lane's first task is the same in every slab: read once.
-  const int ntask = TR_ROWS * nchunks;
+  // The chunk of this lane's first task is the same in every full pass:
+  // read once.
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int4 ch0 = s_chunk[lane / TR_ROWS];
+  const int4 ch0 = s_chunk[min((int)lane / R_PASS, 63)];
 
   // Nodes a pinned particle touches with weight above 1e-8 are pinned
   // (src/three/mpm.js:83).
@@ -680,41 +717,50 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
             if (w[0][ii] * wyz[j][k] > 1e-8f) fixed[(size_t)gbase + ((size_t)k * N + j) * N + ii] = 1;
     }
   }
-  // Transpose through LDS: each lane writes its particle's contributions to
-  // the 9 nodes of one z-slab as a column; then every lane sums whole
-  // (node, chunk) tasks and adds each chunk sum to the node box with one LDS
-  // atomic per channel. The centre slab goes last: its centre node takes
-  // the remainder of the fixed-point mass so the 27 parts sum exactly to Q.
+  // Transpose through LDS: in each pass, each lane writes its particle's
+  // contributions to R_PASS nodes (in scatter_node order) as a column; then
+  // every lane sums whole (row, chunk) tasks and adds each chunk sum to the
+  // node box with one LDS atomic per channel. R_PASS rows of chunks of CHUNK
+  // lanes keep a pass's tasks within one round of the wave's 64 lanes up to
+  // 64 / R_PASS chunks (9 at 7 rows: a wave of 8 particles per cell). The
+  // centre node goes last and takes the remainder of the fixed-point mass,
+  // so the 27 parts sum exactly to Q.
   int qsum = 0;
 #pragma unroll
-  for (int si = 0; si < 3; ++si) {
-    const int k = si == 0 ? 0 : (si == 1 ? 2 : 1);
-    if (si == 0) __syncthreads();  // node box cleared before any atomic
+  for (int p = 0; p < NPASS; ++p) {
+    constexpr int R_LAST = 27 - (NPASS - 1) * R_PASS;
+    const int rows = p == NPASS - 1 ? R_LAST : R_PASS;
+    if (p == 0) __syncthreads();  // node box cleared before any atomic
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      // In the centre slab the centre node (nn = 4) is visited last.
-      const int nn = k != 1 ? r : (r < 4 ? r : (r < 8 ? r + 1 : 4));
-      const int ii = nn % 3, j = nn / 3;
-      const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
-      int qm;
-      if (k == 1 && nn == 4) {
-        qm = Q - qsum;
-      } else {
-        qm = __float2int_rn(wt * Qf);
-        qsum += qm;
+    for (int r = 0; r < R_PASS; ++r) {
+      const int g = p * R_PASS + r;
+      if (g < 27) {
+        const int n = scatter_node(g);
+        const int k = n / 9, j = (n / 3) % 3, ii = n % 3;
+        const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
+        int qm;
+        if (g == 26) {
+          qm = Q - qsum;
+        } else {
+          qm = __float2int_rn(wt * Qf);
+          qsum += qm;
+        }
+        float pc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          pc[c] = wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
+        s_tr[r * TR_STRIDE + dest] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
       }
-      float pc[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) pc[c] = wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
-      s_tr[nn * TR_STRIDE + dest] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int ntask = rows * nchunks;
     for (int t = (int)lane; t < ntask; t += 64) {
       int4 ch = ch0;
-      if (t >= 64) ch = s_chunk[t / TR_ROWS];
-      run_task(t % TR_ROWS, ch, k);
+      if (t >= 64 || rows != R_PASS) ch = s_chunk[t / rows];
+      const int row = t % rows;
+      run_task(row, scatter_node(p * R_PASS + row), ch);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -744,7 +790,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
 // per CU is the LDS limit (3 x 52.8 KB of 160 KB); the register budget is
 // held to the same occupancy so the surrogate variant's MFMA operands do not
 // cost a wave.
-template <int MODE, bool SUR, bool EXT>
+template <int MODE, bool SUR, bool EXT, int CH>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, uint32_t n,
     const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
@@ -753,16 +799,16 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB>::LDS_BYTES];
   __shared__ int s_box[6 * Shape<FB>::WAVES];
-  fused_chunk<MODE, SUR, FB, EXT>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
-                                  acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
-                                  delta_slot);
+  fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
+                                      acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
+                                      delta_slot);
 }
 
 // Many scenes in one launch: one workgroup per chunk of the bucket's chunk
 // list (chunks never straddle scenes). A persistent form (workgroups taking
 // chunks from a counter) kept the loop's invariants live across chunks and
 // needed about 45 more VGPRs, which spilled at three waves per SIMD.
-template <int MODE, int FBK, bool EXT>
+template <int MODE, int FBK, bool EXT, int CH>
 __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_batch(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
@@ -781,9 +827,9 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const StepParams sp = to_scalar(sd->sp);
   const uint32_t node0 = __builtin_amdgcn_readfirstlane(sd->node0);
   const uint32_t end = __builtin_amdgcn_readfirstlane(sd->p0 + sd->n);
-  fused_chunk<MODE, false, FBK, EXT>(P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0, acc_m + node0,
-                                     acc_p + node0, fixed + node0, pieces + (node0 >> 6), nullptr, nullptr, s_raw,
-                                     s_box, status + ch.x);
+  fused_chunk<MODE, false, FBK, EXT, CH>(P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0,
+                                         acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> 6), nullptr,
+                                         nullptr, s_raw, s_box, status + ch.x);
 }
 
 inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
@@ -794,15 +840,21 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
                   LaunchEvents ev) {
   if (!n) return;
   const dim3 grid(fblocks(n)), block(FB);
-#define MPM_FUSED(M, S, X)                                                                                      \
-  hipExtLaunchKernelGGL((k_fused<M, S, X>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n, \
-                        d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts,      \
+#define MPM_FUSED(M, S, X, C)                                                                                      \
+  hipExtLaunchKernelGGL((k_fused<M, S, X, C>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n, \
+                        d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts,         \
                         d.status, mode == 2 ? d.deltas : nullptr, d.delta_slot)
-#define MPM_FUSED_X(M, S) \
-  if (d.ext_laws)         \
-    MPM_FUSED(M, S, true); \
-  else                    \
-    MPM_FUSED(M, S, false)
+#define MPM_FUSED_C(M, S, X)  \
+  if (d.narrow)               \
+    MPM_FUSED(M, S, X, 8);    \
+  else                        \
+    MPM_FUSED(M, S, X, 16)
+#define MPM_FUSED_X(M, S)        \
+  if (d.ext_laws) {              \
+    MPM_FUSED_C(M, S, true);     \
+  } else {                       \
+    MPM_FUSED_C(M, S, false);    \
+  }
   const bool sur = d.mlp != nullptr;
   if (mode == 0) {
     MPM_FUSED_X(0, false);
@@ -816,27 +868,34 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
     MPM_FUSED_X(2, false);
   }
 #undef MPM_FUSED_X
+#undef MPM_FUSED_C
 #undef MPM_FUSED
 }
 
 void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, int mode, int fb, LaunchEvents ev) {
   if (!b.nchunks) return;
   const dim3 grid(b.nchunks);
-#define MPM_FUSED_B(M, K, X)                                                                                       \
-  hipExtLaunchKernelGGL((k_fused_batch<M, K, X>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta,     \
+#define MPM_FUSED_B(M, K, X, C)                                                                                     \
+  hipExtLaunchKernelGGL((k_fused_batch<M, K, X, C>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta,   \
                         d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.status)
-#define MPM_FUSED_K(M, X)     \
-  if (fb == 64)               \
-    MPM_FUSED_B(M, 64, X);    \
-  else if (fb == 128)         \
-    MPM_FUSED_B(M, 128, X);   \
-  else                        \
-    MPM_FUSED_B(M, 256, X)
-#define MPM_FUSED_M(M)        \
-  if (d.ext_laws) {           \
-    MPM_FUSED_K(M, true);     \
-  } else {                    \
-    MPM_FUSED_K(M, false);    \
+#define MPM_FUSED_K(M, X, C)     \
+  if (fb == 64)                  \
+    MPM_FUSED_B(M, 64, X, C);    \
+  else if (fb == 128)            \
+    MPM_FUSED_B(M, 128, X, C);   \
+  else                           \
+    MPM_FUSED_B(M, 256, X, C)
+#define MPM_FUSED_C(M, X)        \
+  if (d.narrow) {                \
+    MPM_FUSED_K(M, X, 8);        \
+  } else {                       \
+    MPM_FUSED_K(M, X, 16);       \
+  }
+#define MPM_FUSED_M(M)           \
+  if (d.ext_laws) {              \
+    MPM_FUSED_C(M, true);        \
+  } else {                       \
+    MPM_FUSED_C(M, false);       \
   }
   if (mode == 0) {
     MPM_FUSED_M(0);
@@ -846,6 +905,7 @@ void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, in
     MPM_FUSED_M(2);
   }
 #undef MPM_FUSED_M
+#undef MPM_FUSED_C
 #undef MPM_FUSED_K
 #undef MPM_FUSED_B
 }

@@ -175,6 +175,37 @@ inline void finish_conservation(const double* h, int K, int nb, const double* gr
   out->max_speed = maxv;
 }
 
+// Number of distinct base cells (the cell of a particle's first stencil node,
+// as the kernels compute it) over particles [0, n) of FP32 planes of stride
+// cap. With the particle count it gives the mean particles per occupied
+// cell, which picks the scatter's chunk width.
+inline uint64_t occupied_cells(const float* planes, uint32_t cap, uint32_t n, const GridDims& g) {
+  if (!n) return 0;
+  const int M = g.N - 2;  // base cells per axis
+  std::vector<uint64_t> bits(((size_t)M * M * M + 63) / 64, 0ull);
+  uint64_t count = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    size_t cell = 0;
+    for (int a = 2; a >= 0; --a) {
+      const float q = (planes[(size_t)(P_X + a) * cap + i] - g.O[a]) * g.invH;
+      int b = (int)std::floor(q - 0.5f);
+      b = b < 0 ? 0 : (b > g.N - 3 ? g.N - 3 : b);
+      cell = cell * M + (size_t)b;
+    }
+    const uint64_t bit = 1ull << (cell & 63);
+    if (!(bits[cell >> 6] & bit)) {
+      bits[cell >> 6] |= bit;
+      ++count;
+    }
+  }
+  return count;
+}
+
+// Scatter chunk width from the particle and occupied-cell counts: 8-lane
+// chunks below a mean of 12 particles per cell (cells of 8 or fewer
+// particles), 16 otherwise.
+inline bool narrow_chunks(uint64_t particles, uint64_t cells) { return cells && particles < 12 * cells; }
+
 // Guard thresholds of a step of size dt for a scene of configuration c
 // (mpm_config's guard fields): a particle violates the stable step when
 // cfl h / max(wave, 1) < 0.99999 dt, i.e. wave > cfl h / (0.99999 dt) (the

@@ -64,6 +64,7 @@ enum : uint32_t { STATUS_CFL = 1u, STATUS_INVALID = 2u };
 struct DevState {
   float* planes;      // NPLANES * cap
   bool ext_laws;      // some material uses damage or a pressure model (8, 9)
+  bool narrow;        // sparse cells (under 12 particles each): 8-lane scatter chunks
   const float4* deltas;      // per-slot grid velocity deltas of a split step (N^3 each), or null
   const int32_t* delta_slot; // per material: delta slot or -1 [64]
   float* planes_alt;  // permutation target
