@@ -5,8 +5,9 @@
 // Layout: one particle pool (slots of scene s are [p0_s, p0_s + n_s), p0 the
 // prefix sum of the counts) and one concatenated grid (scene s's dense N_s^3
 // block starts at node0_s, a multiple of one 256-node grid block). One radix
-// sort per re-bin orders the whole pool by node0_s + cell key, so every
-// scene stays in its own slot range, cell-sorted, ties by original id.
+// sort per re-bin orders the whole pool by key0_s + blocked cell key, so
+// every scene stays in its own slot range, cell-sorted in 4x4x4 blocks, ties
+// by original id.
 //
 // Per step: one particle launch per size bucket (scenes grouped by grid
 // edge; each bucket has its own particles-per-workgroup, chosen by the
@@ -268,7 +269,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   b->mass_q.assign(nscenes, 0);
   b->occupied.assign(nscenes, 0);
   b->uploaded.assign(nscenes, 0);
-  uint64_t p0 = 0, node0 = 0, block0 = 0;
+  uint64_t p0 = 0, node0 = 0, block0 = 0, key0 = 0;
   for (int32_t s = 0; s < nscenes; ++s) {
     SceneDesc& sd = b->sc[s];
     const int rc = grid_geometry(&scenes[s], sd.g, sd.lo_idx, sd.hi_idx);
@@ -281,6 +282,8 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
     sd.n = counts[s];
     sd.node0 = (uint32_t)node0;
     sd.block0 = (uint32_t)block0;
+    sd.key0 = (uint32_t)key0;
+    key0 += blocked_key_count(sd.g.N);
     const uint64_t Nn = (uint64_t)sd.g.N * sd.g.N * sd.g.N;
     const uint64_t blocks = (Nn + GRID_BLOCK_NODES - 1) / GRID_BLOCK_NODES;
     p0 += counts[s];
@@ -288,7 +291,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
     block0 += blocks;
     b->cell[s] = scenes[s].cell_size;
   }
-  if (p0 > (1ull << 31) || node0 > (1ull << 31)) {
+  if (p0 > (1ull << 31) || node0 > (1ull << 31) || key0 > (1ull << 32)) {
     delete b;
     return MPM_ERR_CAPACITY;
   }
@@ -296,7 +299,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   b->total_nodes = (uint32_t)node0;
   b->total_blocks = (uint32_t)block0;
   b->key_bits = 8;
-  while (b->key_bits < 32 && (1ull << b->key_bits) < node0) b->key_bits += 8;
+  while (b->key_bits < 32 && (1ull << b->key_bits) < key0) b->key_bits += 8;
   if (opt) {
     if (opt->sort_every >= 1) b->sort_every = opt->sort_every;
     b->use_graph = opt->use_graph != 0;

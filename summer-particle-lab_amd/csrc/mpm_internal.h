@@ -123,14 +123,17 @@ constexpr int SUR_SLOTS = 64;
 constexpr int SUM_SLOTS = 16;  // slots per scene of the grid sums (spreads the atomics)
 
 // One scene of a batch: its grid and step parameters, its particle slot
-// range [p0, p0 + n) and the first node / grid block of its grid in the
-// concatenated grid arrays (node0 is a multiple of 256, one grid block).
+// range [p0, p0 + n), the first node / grid block of its grid in the
+// concatenated grid arrays (node0 is a multiple of 256, one grid block) and
+// the first of its sort keys (key0 + blocked_cell_key; scenes take
+// consecutive ranges of blocked_key_count(N) keys).
 struct SceneDesc {
   GridDims g;
   StepParams sp;
   uint32_t p0, n;
   uint32_t node0, block0;
   int lo_idx[3], hi_idx[3];
+  uint32_t key0;
 };
 
 // Particle-kernel arguments of a batched launch: the scene table, the chunk

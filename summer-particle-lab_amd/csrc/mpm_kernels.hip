@@ -55,9 +55,9 @@ __global__ __launch_bounds__(BLOCK) void k_bin_keys(const float* __restrict__ P,
   vals[id] = i;
 }
 
-// Batched keys: chunk -> scene; the key is the base cell's node index in the
-// concatenated grid (node0 + cell), so one sort orders scenes by node0 and
-// cells within each scene.
+// Batched keys: chunk -> scene; the key is the scene's first key plus the
+// base cell's blocked key, so one sort orders scenes by slot range and the
+// cells of each scene in 4x4x4 blocks, as the single-scene binning does.
 __global__ __launch_bounds__(BLOCK) void k_bin_keys_batch(const float* __restrict__ P,
                                                           const uint32_t* __restrict__ ids, uint32_t cap,
                                                           BatchArgs b, uint32_t chunk_size,
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_keys_batch(const float* __restric
   const int by = base_of(plane(P, cap, P_Y, i), g.O[1], g.invH, g.N, t);
   const int bz = base_of(plane(P, cap, P_Z, i), g.O[2], g.invH, g.N, t);
   const uint32_t id = ids[i];
-  keys[id] = sd->node0 + ((uint32_t)bz * (uint32_t)g.N + (uint32_t)by) * (uint32_t)g.N + (uint32_t)bx;
+  keys[id] = sd->key0 + blocked_cell_key(bx, by, bz, g.N);
   vals[id] = i;
 }
 
