@@ -613,13 +613,41 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   static_assert(FBK <= 256, "the fixed-point bound counts at most 256 particles per node");
   const int qexp = min(126, 24 - frexp_exponent(bmax));
   const float qs = __int_as_float((qexp + 127) << 23), inv_qs = __int_as_float((127 - qexp) << 23);
-  const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
-  const int cnt = ex * ey * ez;
-  const bool staged = box_inside(lo, hi, N) && cnt <= S::PATCH_ACC && bmax <= 3.0e38f;
-  int* s_q = reinterpret_cast<int*>(s_raw);                                 // momentum x, y, z planes
-  uint32_t* s_mq = reinterpret_cast<uint32_t*>(s_raw + 12 * S::PATCH_ACC);  // fixed-point mass
+  int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
+  int cnt = ex * ey * ez;
+  bool staged = box_inside(lo, hi, N) && cnt <= S::PATCH_ACC && bmax <= 3.0e38f;
+  // A block whose box does not fit gives each wave a box of its own in a
+  // quarter of the buffer: when the block's particles straddle a jump of the
+  // cell order (the next row of 4x4x4 blocks), usually one wave holds the
+  // jump and only that wave takes the global atomics.
+  constexpr int WAVE_ACC = S::PATCH_ACC / S::WAVES;
+  const bool per_wave = S::WAVES > 1 && !staged && bmax <= 3.0e38f;  // uniform over the block
+  int box0 = 0;
+  if (per_wave) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = wave_extreme<false>(valid ? b[a] : 0x7fffffff);
+      hi[a] = wave_extreme<true>(valid ? b[a] : -1);
+    }
+    if (hi[0] < 0) {  // no particle in this wave: an empty box
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = 0;
+        hi[a] = -1;
+      }
+    }
+    ex = hi[0] - lo[0] + 3;
+    ey = hi[1] - lo[1] + 3;
+    ez = hi[2] - lo[2] + 3;
+    cnt = ex * ey * ez;
+    staged = box_inside(lo, hi, N) && cnt <= WAVE_ACC;  // uniform over the wave
+    box0 = (int)(tid >> 6) * WAVE_ACC;
+  }
+  const int box_first = per_wave ? (int)lane : (int)tid, box_stride = per_wave ? 64 : FBK;
+  int* s_q = reinterpret_cast<int*>(s_raw) + box0;                                  // momentum x, y, z planes
+  uint32_t* s_mq = reinterpret_cast<uint32_t*>(s_raw + 12 * S::PATCH_ACC) + box0;  // fixed-point mass
   if (staged) {
-    for (int p = tid; p < cnt; p += FBK) {
+    for (int p = box_first; p < cnt; p += box_stride) {
       s_mq[p] = 0u;
       s_q[p] = 0;
       s_q[S::PATCH_ACC + p] = 0;
@@ -806,10 +834,10 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (staged) {
+  if (staged || per_wave) {  // uniform over the block
     __syncthreads();
     const BoxIndex at(ex, ey);
-    for (int p = tid; p < cnt; p += FBK) {
+    for (int p = box_first; p < (staged ? cnt : 0); p += box_stride) {
       const unsigned long long mq = s_mq[p];
       const int qx = s_q[p], qy = s_q[S::PATCH_ACC + p], qz = s_q[2 * S::PATCH_ACC + p];
       if (mq != 0ull || qx != 0 || qy != 0 || qz != 0) {
