@@ -36,14 +36,21 @@ namespace mpm {
 namespace {
 
 constexpr int FB = 256;            // particles per chunk of the single-scene launch
-constexpr int TR_ROWS = 9;         // per-wave transpose: one row per node of a slab, {mass, momentum} per lane
+constexpr int TR_ROWS = 5;         // per-wave transpose: one row per node of a pass, {mass, momentum} per lane
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 entry: rows start in different banks
 constexpr int CHUNK_MAX = 16;      // lanes per scatter chunk (run pieces summed by one lane), at most
 constexpr int TR_TAIL = 16;        // zero entries after the last row
 constexpr int SORT_PASSES = 8;     // cells of a wave grouped by minimum passes before P2G
 constexpr int TR_ENTRIES = TR_ROWS * TR_STRIDE + TR_TAIL;
-constexpr int WAVE_BYTES = TR_ENTRIES * 16 + 64 * 16;
-static_assert(WAVE_BYTES >= MLP_WAVE_LDS && WAVE_BYTES % 16 == 0, "surrogate activations must fit the wave buffer");
+constexpr int TR_BYTES = TR_ENTRIES * 16 + 64 * 16;  // transpose rows, zero tail, chunk table
+static_assert(TR_BYTES % 16 == 0, "wave buffers must stay 16-byte aligned");
+
+// Bytes of one wave's LDS buffer: the transpose, or in the surrogate variant
+// at least the network's activations, which reuse it.
+template <bool SUR>
+constexpr int wave_bytes() {
+  return SUR && MLP_WAVE_LDS > TR_BYTES ? (MLP_WAVE_LDS + 15) / 16 * 16 : TR_BYTES;
+}
 static_assert((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK_MAX <= TR_ENTRIES,
               "the run-sum reads past a row's last lane stay inside the rows and the zero tail");
 static_assert(TR_ROWS + TR_TAIL <= 64, "one store per lane zeroes the row pads and the tail");
@@ -52,15 +59,16 @@ static_assert(TR_ROWS + TR_TAIL + CHUNK_MAX <= 64,
               "one store per lane zeroes the pads, the tail and the row past the passes");
 
 // Shape of the scatter: chunks of CH lanes and R nodes transposed per pass.
-// Dense cells (long runs) take 16-lane chunks over 9 rows; cells of about 8
-// particles take 8-lane chunks, whose task reads carry half the dead entries
-// (7 rows per pass keep a pass's tasks within one round of the wave up to 9
-// chunks). Measured at 64 and 8 particles per cell: 16 lanes 171 vs 214 us,
-// 8 lanes 1696 vs 1799 us of the fused kernel.
+// Dense cells (long runs) take 16-lane chunks; cells of about 8 particles
+// take 8-lane chunks, whose task reads carry half the dead entries (measured
+// at 8 particles per cell: 1214 against 1314 us of the fused kernel; at 64,
+// 16 lanes 132.7 against 136.9 us). The number of rows per pass, 4 to 9,
+// did not change the time; 5 rows keep a workgroup's LDS small enough for
+// four workgroups per CU.
 template <int CH>
 struct Scatter {
   static constexpr int CHUNK = CH;
-  static constexpr int R_PASS = CH == 8 ? 7 : 9;
+  static constexpr int R_PASS = TR_ROWS;
   static constexpr int NPASS = (27 + R_PASS - 1) / R_PASS;
   static_assert(CH % 8 == 0 && CH <= CHUNK_MAX && R_PASS <= TR_ROWS, "scatter pass shape");
 };
@@ -72,13 +80,14 @@ __host__ __device__ constexpr int scatter_node(int g) { return g == 26 ? 13 : g 
 
 // LDS plan of a workgroup of FBK particles (one per lane): a node box for the
 // scatter (32-bit fixed-point mass and momentum x, y, z planes, 16 B per
-// node, 2.5 nodes per particle) and one transpose buffer per wave; the
-// gather box (16 B per node) reuses the whole buffer.
-template <int FBK>
+// node, 2.5 nodes per particle) and one buffer per wave; the gather box
+// (16 B per node) reuses the whole buffer.
+template <int FBK, bool SUR>
 struct Shape {
   static constexpr int WAVES = FBK / 64;
   static constexpr int PATCH_ACC = 5 * FBK / 2;
   static constexpr int ACC_BYTES = 16 * PATCH_ACC;
+  static constexpr int WAVE_BYTES = wave_bytes<SUR>();
   static constexpr int LDS_BYTES = ACC_BYTES + WAVES * WAVE_BYTES;
   static constexpr int PATCH_VEL = LDS_BYTES / 16;
   static_assert(ACC_BYTES % 16 == 0, "wave buffers must stay 16-byte aligned");
@@ -257,7 +266,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                                             int* s_box, uint32_t* __restrict__ status,
                                             const float4* __restrict__ deltas = nullptr,
                                             const int32_t* __restrict__ delta_slot = nullptr) {
-  using S = Shape<FBK>;
+  using S = Shape<FBK, SUR>;
   constexpr int CHUNK = Scatter<CH>::CHUNK;
   constexpr int R_PASS = Scatter<CH>::R_PASS;
   constexpr int NPASS = Scatter<CH>::NPASS;
@@ -444,7 +453,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         Tp = ld(P, cap, P_TEMP, ic);
         ep = ld(P, cap, P_EP, ic);
       }
-      const int br = surrogate_return(mlp, mine, Fn, ep, Tp, mat, tau, s_raw + S::ACC_BYTES + (tid >> 6) * WAVE_BYTES);
+      const int br = surrogate_return(mlp, mine, Fn, ep, Tp, mat, tau, s_raw + S::ACC_BYTES + (tid >> 6) * S::WAVE_BYTES);
       if (br == SUR_NETWORK || br == SUR_ELASTIC) {
         new_flags |= FLAG_CACHE;
         F = Fn;
@@ -666,7 +675,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // a finite value: the pads and the tail are zeroed here (the buffer held
   // the gather box or surrogate activations before), the lanes are written
   // in every slab.
-  unsigned char* wbuf = s_raw + S::ACC_BYTES + (tid >> 6) * WAVE_BYTES;
+  unsigned char* wbuf = s_raw + S::ACC_BYTES + (tid >> 6) * S::WAVE_BYTES;
   float4* s_tr = reinterpret_cast<float4*>(wbuf);
   int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ENTRIES * 16);
 
@@ -855,19 +864,22 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   }
 }
 
-// Single scene: one chunk of FB particles per workgroup. Three workgroups
-// per CU is the LDS limit (3 x 52.8 KB of 160 KB); the register budget is
-// held to the same occupancy so the surrogate variant's MFMA operands do not
-// cost a wave.
+// Single scene: one chunk of FB particles per workgroup. Four workgroups per
+// CU fit the LDS (4 x 36.3 KB of 160 KB) and the register budget is held to
+// the same four waves per SIMD (the compiler then fits 116 VGPRs without
+// spills; left at three waves it took about 148, and four waves ran the large
+// scene's fused step 13% faster). The surrogate variant's wave buffer holds
+// the network's activations (47.2 KB per workgroup) and its MFMA operands
+// need about 160 VGPRs: three waves.
 template <int MODE, bool SUR, bool EXT, int CH>
-__global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused(
+__global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(SUR ? 3 : 4, SUR ? 3 : 4))) void k_fused(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, uint32_t n,
     const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
     unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
     uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts,
     uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
-  __shared__ __align__(16) unsigned char s_raw[Shape<FB>::LDS_BYTES];
-  __shared__ int s_box[7 * Shape<FB>::WAVES];
+  __shared__ __align__(16) unsigned char s_raw[Shape<FB, SUR>::LDS_BYTES];
+  __shared__ int s_box[7 * Shape<FB, SUR>::WAVES];
   fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
                                       acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
                                       delta_slot);
@@ -876,14 +888,15 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 // Many scenes in one launch: one workgroup per chunk of the bucket's chunk
 // list (chunks never straddle scenes). A persistent form (workgroups taking
 // chunks from a counter) kept the loop's invariants live across chunks and
-// needed about 45 more VGPRs, which spilled at three waves per SIMD.
+// needed about 45 more VGPRs, which spilled at three waves per SIMD. Four
+// waves per SIMD, as the single-scene kernel.
 template <int MODE, int FBK, bool EXT, int CH>
-__global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_batch(
+__global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_fused_batch(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
     uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status) {
-  __shared__ __align__(16) unsigned char s_raw[Shape<FBK>::LDS_BYTES];
-  __shared__ int s_box[7 * Shape<FBK>::WAVES];
+  __shared__ __align__(16) unsigned char s_raw[Shape<FBK, false>::LDS_BYTES];
+  __shared__ int s_box[7 * Shape<FBK, false>::WAVES];
   const uint32_t t = blockIdx.x;
   if (t >= b.nchunks) return;
   // The chunk and its scene's parameters are the same for every lane, but
