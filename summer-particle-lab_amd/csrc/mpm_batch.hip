@@ -108,6 +108,11 @@ int bfail(mpm_batch* b, int code, const char* fmt, ...) {
     if (_st != hipSuccess) return bfail((b), MPM_ERR_HIP, "%s: %s", #call, hipGetErrorString(_st)); \
   } while (0)
 
+// The buffer assignment that a re-bin swaps.
+bool same_buffers(const DevState& a, const DevState& c) {
+  return a.planes == c.planes && a.ids == c.ids && a.meta == c.meta && a.keys == c.keys && a.vals == c.vals;
+}
+
 void drop_graphs(mpm_batch* b) {
   for (auto& g : b->graph)
     if (g) {
@@ -183,11 +188,18 @@ BatchArgs bucket_args(const mpm_batch* b, int k) {
   return BatchArgs{b->d_scenes, bk.chunks, bk.nchunks, b->total_nodes};
 }
 
-void rebin(mpm_batch* b) {
+// Bin, sort and permute every scene's particles into cell order. With
+// `state_only` set the re-bin sits between a grid update and the fused step
+// that follows it, whose G2P rebuilds v, C and the stress: only the planes
+// that carry over a step move.
+void rebin(mpm_batch* b, bool state_only = false) {
   launch_bin_keys_batch(b->stream, b->d, bin_args(b));
   radix_sort_pairs(b->stream, b->d.keys, b->d.vals, b->d.keys_alt, b->d.vals_alt, b->d.hist, b->d.partials,
                    b->total_n, b->key_bits);
-  launch_permute(b->stream, b->d, b->total_n, b->d.vals);
+  if (state_only)
+    launch_permute_state(b->stream, b->d, b->total_n, b->d.vals);
+  else
+    launch_permute(b->stream, b->d, b->total_n, b->d.vals);
   b->steps_since_sort = 0;
 }
 
@@ -207,21 +219,48 @@ void grid(mpm_batch* b) {
   launch_grid_batch(b->stream, b->d, b->d_scenes, b->d_block_scene, b->total_nodes, b->grid_blocks);
 }
 
-// One re-bin period of P steps: the launch sequence a graph captures.
-void period(mpm_batch* b, int P) {
-  rebin(b);
+// The steps of a call: the first P2G reads the stored state, every step but
+// the last is a grid update and a fused step (G2P of this step, P2G of the
+// next), and the last step's G2P writes the whole state back. A re-bin is
+// due before the fused step once sort_every steps have passed since the
+// last one.
+void begin_steps(mpm_batch* b) {
+  if (b->needs_rebin || b->steps_since_sort >= b->sort_every) rebin(b);
   particles(b, 0);
-  for (int s = 0; s < P; ++s) {
+}
+
+void fused_step(mpm_batch* b) {
+  grid(b);
+  if (++b->steps_since_sort >= b->sort_every) rebin(b, true);
+  particles(b, 1);
+}
+
+void end_steps(mpm_batch* b) {
+  grid(b);
+  particles(b, 2);
+  b->steps_since_sort++;
+}
+
+// One re-bin period of P fused steps, starting with the re-bin that is due:
+// the launch sequence a graph captures. The grid holds a P2G on entry and on
+// exit, and the next re-bin is due P steps later.
+void period(mpm_batch* b, int P) {
+  grid(b);
+  rebin(b, true);
+  particles(b, 1);
+  for (int s = 1; s < P; ++s) {
     grid(b);
-    particles(b, s == P - 1 ? 2 : 1);
+    particles(b, 1);
   }
-  b->steps_since_sort = P;
+  b->steps_since_sort = P - 1;
 }
 
 int capture_period(mpm_batch* b, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
+  const int since = b->steps_since_sort;
   BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   period(b, b->sort_every);
+  b->steps_since_sort = since;
   BHIP(b, hipStreamEndCapture(b->stream, &g));
   hipError_t st = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -241,8 +280,8 @@ int prepare_graphs(mpm_batch* b) {
   if (!rc) rc = capture_period(b, &b->graph[1]);
   b->graph_after[1] = b->d;
   if (rc) return rc;
-  if (b->d.planes != before.planes || b->d.keys != before.keys || b->d.vals != before.vals)
-    return bfail(b, MPM_ERR_HIP, "graph pair does not return the buffers to their start");
+  if (!same_buffers(b->d, before) || same_buffers(b->graph_after[0], before))
+    return bfail(b, MPM_ERR_HIP, "graph pair does not alternate between two buffer assignments");
   b->graph_ready = true;
   b->graph_next = 0;
   return MPM_OK;
@@ -423,10 +462,15 @@ int mpm_batch_set_materials(mpm_batch* b, const mpm_material* mats, int32_t coun
     dm[i] = to_dev_material(mats[i]);
   }
   b->materials.assign(mats, mats + count);
-  bool ext = false;
-  for (int i = 0; i < count; ++i) ext |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
-  if (ext != b->d.ext_laws) drop_graphs(b);  // captured launches name the kernel variant
+  bool ext = false, history = false;
+  for (int i = 0; i < count; ++i) {
+    ext |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
+    history |= keeps_history(mats[i]);
+  }
+  // captured launches name the kernel variant and the planes a re-bin moves
+  if (ext != b->d.ext_laws || history != b->d.history) drop_graphs(b);
   b->d.ext_laws = ext;
+  b->d.history = history;
   BHIP(b, hipMemcpyAsync(b->d.materials, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   return MPM_OK;
@@ -519,40 +563,30 @@ int mpm_batch_step(mpm_batch* b, int32_t nsteps) {
   }
   if (nsteps == 0 || b->total_n == 0) return MPM_OK;
   const int P = b->sort_every;
-  int left = nsteps;
-  if (b->use_graph) {
-    while (left >= P) {
+  begin_steps(b);
+  b->needs_rebin = false;
+  int left = nsteps - 1;  // fused steps; the last step is G2P only
+  while (left > 0) {
+    if (b->use_graph && left >= P && b->steps_since_sort + 1 >= P) {
+      // Re-bins outside the captured periods swap the buffers too: take the
+      // graph of the pair that starts from the current assignment.
+      if (b->graph_ready && !same_buffers(b->d, b->graph_after[b->graph_next ^ 1])) b->graph_next ^= 1;
+      if (b->graph_ready && !same_buffers(b->d, b->graph_after[b->graph_next ^ 1])) drop_graphs(b);
       if (!b->graph_ready) {
         const int rc = prepare_graphs(b);
         if (rc) return rc;
       }
       BHIP(b, hipGraphLaunch(b->graph[b->graph_next], b->stream));
       b->d = b->graph_after[b->graph_next];
-      b->steps_since_sort = P;
+      b->steps_since_sort = P - 1;
       b->graph_next ^= 1;
       left -= P;
+      continue;
     }
+    fused_step(b);
+    --left;
   }
-  // Eager steps: the same period structure, cut at the end of the call.
-  while (left > 0) {
-    const int steps = std::min(left, P);
-    if (steps == P && !b->use_graph) {
-      period(b, P);
-    } else {
-      rebin(b);
-      particles(b, 0);
-      for (int s = 0; s < steps; ++s) {
-        grid(b);
-        particles(b, s == steps - 1 ? 2 : 1);
-      }
-      if (b->use_graph) {
-        // the buffers moved: the captured pair no longer matches
-        drop_graphs(b);
-      }
-    }
-    left -= steps;
-  }
-  b->needs_rebin = false;
+  end_steps(b);
   BHIP(b, hipGetLastError());
   return MPM_OK;
 }
@@ -694,9 +728,12 @@ int mpm_batch_tune(mpm_batch* b, int32_t trial_steps, mpm_batch_tuning* out) {
   double best_ms = 1e300;
   for (int P : {4, 8, 16, 32, 64}) {
     const int steps = std::max(P, (trial_steps + P - 1) / P * P);
+    rebin(b);
+    particles(b, 0);
     BHIP(b, hipEventRecord(e0, b->stream));
     for (int s = 0; s < steps; s += P) period(b, P);
     BHIP(b, hipEventRecord(e1, b->stream));
+    end_steps(b);
     BHIP(b, hipEventSynchronize(e1));
     float ms = 0;
     BHIP(b, hipEventElapsedTime(&ms, e0, e1));

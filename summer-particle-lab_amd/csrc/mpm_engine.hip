@@ -267,8 +267,11 @@ namespace {
 
 bool supported_model(int m) { return m >= 0 && m <= 11; }
 
-// Bin, sort and permute the resident particles into cell order.
-int rebin(mpm_engine* e) {
+// Bin, sort and permute the resident particles into cell order. With
+// `state_only` set the re-bin sits between a grid update and the fused step
+// that follows it, whose G2P rebuilds v, C and the stress: only the planes
+// that carry over a step move.
+int rebin(mpm_engine* e, bool state_only = false) {
   if (e->n == 0) return MPM_OK;
   {
     Timed t(e, "bin_keys");
@@ -281,7 +284,10 @@ int rebin(mpm_engine* e) {
   }
   {
     Timed t(e, "permute");
-    launch_permute(e->stream, e->d, e->n, e->d.vals);
+    if (state_only)
+      launch_permute_state(e->stream, e->d, e->n, e->d.vals);
+    else
+      launch_permute(e->stream, e->d, e->n, e->d.vals);
   }
   HIP_OK(e, hipGetLastError());
   e->steps_since_sort = 0;
@@ -455,8 +461,11 @@ int mpm_set_materials(mpm_engine* e, const mpm_material* mats, int32_t count) {
   }
   e->materials.assign(mats, mats + count);
   e->d.ext_laws = false;
-  for (int i = 0; i < count; ++i)
+  e->d.history = false;
+  for (int i = 0; i < count; ++i) {
     e->d.ext_laws |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
+    e->d.history |= keeps_history(mats[i]);
+  }
   if (e->d.mlp) {  // a new table switches the surrogate off
     HIP_OK(e, hipStreamSynchronize(e->stream));
     HIP_OK(e, hipFree(e->d.mlp));
@@ -544,20 +553,16 @@ int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
       launch_grid(e->stream, e->d, e->g, sp, e->lo_idx, e->hi_idx, e->grid_blocks, t.ev());
     }
     e->steps_since_sort++;
-    const bool last = s == nsteps - 1;
-    const bool resort = !last && e->steps_since_sort >= e->cfg.sort_every;
-    if (last || resort) {
-      {
-        Timed t(e, "g2p", true);
-        launch_fused(e->stream, e->d, e->n, e->g, sp, 2, t.ev());
-      }
-      if (resort) {
-        int rc = rebin(e);
-        if (rc) return rc;
-        Timed t(e, "p2g", true);
-        launch_fused(e->stream, e->d, e->n, e->g, sp, 0, t.ev());
-      }
+    if (s == nsteps - 1) {
+      Timed t(e, "g2p", true);
+      launch_fused(e->stream, e->d, e->n, e->g, sp, 2, t.ev());
     } else {
+      // A re-bin due here sorts the positions this grid was built from,
+      // which the next G2P gathers at.
+      if (e->steps_since_sort >= e->cfg.sort_every) {
+        int rc = rebin(e, true);
+        if (rc) return rc;
+      }
       Timed t(e, "g2p2g", true);
       launch_fused(e->stream, e->d, e->n, e->g, sp, 1, t.ev());
     }

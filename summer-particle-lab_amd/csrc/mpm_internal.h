@@ -65,6 +65,7 @@ struct DevState {
   float* planes;      // NPLANES * cap
   bool ext_laws;      // some material uses damage or a pressure model (8, 9)
   bool narrow;        // sparse cells (under 12 particles each): 8-lane scatter chunks
+  bool history;       // some material keeps Jp, dilation or damage (models 2, 3, 8, 9, fracture)
   const float4* deltas;      // per-slot grid velocity deltas of a split step (N^3 each), or null
   const int32_t* delta_slot; // per material: delta slot or -1 [64]
   float* planes_alt;  // permutation target
@@ -164,6 +165,9 @@ struct LaunchEvents {
 // Kernel launchers (mpm_kernels.hip)
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
 void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
+// The planes that carry over a step only (every plane of a pinned particle):
+// for a re-bin between two fused steps, whose G2P rebuilds v, C and stress.
+void launch_permute_state(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
 // Grid update: launch_compact lists the pieces P2G flagged (and clears the
 // grid sums), launch_grid / launch_grid_batch update the listed pieces with
 // `blocks` workgroups, each taking a contiguous range of the list.
@@ -221,6 +225,11 @@ void launch_return_batch(hipStream_t s, const DevMaterial& m, uint32_t n, float 
                          float* jp_out, float* dil_out, float* tau_out);
 void launch_stress_batch(hipStream_t s, const DevMaterial& m, uint32_t n, const float* F, float* tau_out);
 DevMaterial to_dev_material(const mpm_material& m);
+// A material whose particles carry Jp (snow, the pressure models, and a
+// fractured particle's sand state), dilation (sand) or damage (fracture).
+inline bool keeps_history(const mpm_material& m) {
+  return m.model == 2 || m.model == 3 || m.model == 8 || m.model == 9 || m.fracture != 0;
+}
 
 // Radix sort (mpm_sort.hip): stable LSD sort of (keys, vals) pairs. On return
 // the sorted pairs are in (keys, vals) (the buffers are swapped as needed).
