@@ -84,6 +84,8 @@ def parse():
     p.add_argument("--workload-steps", type=int, default=500)
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--batched-scenes", type=int, default=256)
+    p.add_argument("--batched-only", action="store_true",
+                   help="only the batched leg, without the one-by-one pass (for profiling)")
     p.add_argument("--no-2d", action="store_true", help="skip the 2D SolidMPM leg")
     p.add_argument("--frames-2d", type=int, default=120)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -394,6 +396,8 @@ def batched_leg(args):
     el_e = time.perf_counter() - t0
     out["eager"] = dict(value=round(total * steps / el_e, 1), ms_per_step=round(el_e / steps * 1e3, 4))
     e.close()
+    if args.batched_only:
+        return out
     # the same scenes, one single-scene engine each, stepped one after another
     engines = [sc.engine(sort_every=out["tuning"]["sort_every"]) for sc in bs.scenes]
     for eng, sc in zip(engines, bs.scenes):
@@ -516,6 +520,11 @@ def main():
         del a
     if args.headline_only:
         args.no_cpu = args.no_surrogate = args.no_workloads = args.no_batched = args.no_2d = True
+    if args.batched_only:
+        batched = batched_leg(args)
+        log(f"batched: {batched['ms_per_step']} ms/step")
+        print(json.dumps({"batched": batched}))
+        return
     K, W = args.steps, args.warmup
     period = period_for(K, args.sort_every)
     t = time.perf_counter()

@@ -551,11 +551,14 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     c2[a] = A[a * 3 + 2] * H;
     q0[a] = m * v[a] - (c0[a] * t[0] + c1[a] * t[1] + c2[a] * t[2]);
   }
+  // Weights of a lane past the end are zero, so all its parts are zero.
   float wyz[3][3];
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
+  for (int k = 0; k < 3; ++k) {
+    const float wz = valid ? w[2][k] : 0.f;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) wyz[j][k] = w[1][j] * w[2][k];
+    for (int j = 0; j < 3; ++j) wyz[j][k] = w[1][j] * wz;
+  }
   // Particle mass in fixed-point units, at most 2^24: a part is at most
   // 0.75^3 of it (below 2^23) and the centre remainder at least 0.125 Q - 13,
   // so every part fits a signed 24-bit factor and every sum of up to 64 parts
@@ -746,8 +749,16 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       f4 vals[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) vals[q] = src[h + q];
+      int qfirst = 0;
+      if (h == 0) {  // a chunk holds at least its first entry
+        sq = __float_as_int(vals[0].x);
+        px = vals[0].y;
+        py = vals[0].z;
+        pz = vals[0].w;
+        qfirst = 1;
+      }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
+      for (int q = qfirst; q < 8; ++q) {
         const uint32_t mi = (in >> (h + q)) & 1u;
         const float mf = (float)mi;
         sq += __mul24(__float_as_int(vals[q].x), (int)mi);
@@ -802,6 +813,13 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // 64 / R_PASS chunks (9 at 7 rows: a wave of 8 particles per cell). The
   // centre node goes last and takes the remainder of the fixed-point mass,
   // so the 27 parts sum exactly to Q.
+  float c0x2[3], c1x2[3], c2x2[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    c0x2[c] = 2.f * c0[c];
+    c1x2[c] = 2.f * c1[c];
+    c2x2[c] = 2.f * c2[c];
+  }
   int qsum = 0;
 #pragma unroll
   for (int p = 0; p < NPASS; ++p) {
@@ -814,18 +832,27 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       if (g < 27) {
         const int n = scatter_node(g);
         const int k = n / 9, j = (n / 3) % 3, ii = n % 3;
-        const float wt = valid ? w[0][ii] * wyz[j][k] : 0.f;
+        const float wt = w[0][ii] * wyz[j][k];
         int qm;
         if (g == 26) {
           qm = Q - qsum;
         } else {
-          qm = __float2int_rn(wt * Qf);
+          // rint(wt Qf) for 0 <= wt Qf < 2^23: the sum with 2^23 rounds to
+          // the nearest integer and its low mantissa bits hold it.
+          qm = __float_as_int(fmaf(wt, Qf, 8388608.f)) - 0x4b000000;
           qsum += qm;
         }
+        // q0 + ii c0 + j c1 + k c2 without multiplying by zero offsets (the
+        // row part is shared by the three nodes of a row).
         float pc[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          pc[c] = wt * (q0[c] + (float)ii * c0[c] + (float)j * c1[c] + (float)k * c2[c]);
+        for (int c = 0; c < 3; ++c) {
+          float rowv = q0[c];
+          if (j) rowv += j == 1 ? c1[c] : c1x2[c];
+          if (k) rowv += k == 1 ? c2[c] : c2x2[c];
+          if (ii) rowv += ii == 1 ? c0[c] : c0x2[c];
+          pc[c] = wt * rowv;
+        }
         s_tr[r * TR_STRIDE + dest] = make_float4(__int_as_float(qm), pc[0], pc[1], pc[2]);
       }
     }
