@@ -278,15 +278,18 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const int N = g.N;
   const float H = g.H, invH = g.invH, dt = sp.dt;
 
-  const uint32_t mt = meta[ic];
-  const uint32_t flags = mt >> 8;
-  const bool pinned = (flags & FLAG_PINNED) != 0;
-  const DevMaterial mat = load_material(mats, mt & 255u);
-
+  // The loads that need no material go first, so they are in flight
+  // together with the meta word.
   float x[3] = {ld(P, cap, P_X, ic), ld(P, cap, P_Y, ic), ld(P, cap, P_Z, ic)};
   M3 F;
 #pragma unroll
   for (int k = 0; k < 9; ++k) F[k] = ld(P, cap, P_F0 + k, ic);
+  const float m = MODE != 2 ? ld(P, cap, P_MASS, ic) : 0.f;
+  const float vol = MODE != 2 ? ld(P, cap, P_VOL, ic) : 0.f;
+  const uint32_t mt = meta[ic];
+  const uint32_t flags = mt >> 8;
+  const bool pinned = (flags & FLAG_PINNED) != 0;
+  const DevMaterial mat = load_material(mats, mt & 255u);
   float v[3] = {0.f, 0.f, 0.f}, Cm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float tau[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float ep = 0.f, jp = 1.f, dil = 0.f;
@@ -300,6 +303,11 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     dmg = mat.fr_on ? ld(P, cap, P_DMG, ic) : 0.f;
     model = effective_model(mat, dmg);
   }
+  // Plastic strain and temperature of a return in G2P (models 2 and up, and
+  // a particle that may fail into sand), loaded before the gather.
+  const bool hist = MODE != 0 && (model >= 2 || (EXT && mat.fr_on));
+  const float ep0 = hist ? ld(P, cap, P_EP, ic) : 0.f;
+  const float temp0 = hist ? ld(P, cap, P_TEMP, ic) : 0.f;
 
   if (MODE == 0 || pinned) {
     // State as stored: stress of this step from F (or the cached return).
@@ -332,12 +340,23 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     const bool staged = box_inside(lo, hi, N) && ex * ey * ez <= S::PATCH_VEL;
     float4* s_vel = reinterpret_cast<float4*>(s_raw);
     if (staged) {
+      // Up to four nodes per lane and round, their loads issued together.
       const int cnt = ex * ey * ez;
       const BoxIndex at(ex, ey);
-      for (int p = tid; p < cnt; p += FBK) {
-        int lx, ly, lz;
-        at(p, lx, ly, lz);
-        s_vel[p] = vel[((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx)];
+      for (int p0 = tid; p0 < cnt; p0 += 4 * FBK) {
+        float4 nv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int p = p0 + u * FBK;
+          if (p < cnt) {
+            int lx, ly, lz;
+            at(p, lx, ly, lz);
+            nv[u] = vel[((size_t)(lo[2] + lz) * N + (lo[1] + ly)) * N + (lo[0] + lx)];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (p0 + u * FBK < cnt) s_vel[p0 + u * FBK] = nv[u];
       }
     }
     __syncthreads();
@@ -450,8 +469,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       const bool mine = gather && (int)(mt & 255u) == mlp->norm.material && (model == 4 || model == 10);
       float Tp = 0.f;
       if (mine) {
-        Tp = ld(P, cap, P_TEMP, ic);
-        ep = ld(P, cap, P_EP, ic);
+        Tp = temp0;
+        ep = ep0;
       }
       const int br = surrogate_return(mlp, mine, Fn, ep, Tp, mat, tau, s_raw + S::ACC_BYTES + (tid >> 6) * S::WAVE_BYTES);
       if (br == SUR_NETWORK || br == SUR_ELASTIC) {
@@ -470,12 +489,12 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     }
     if (gather && !done) {
       if (model >= 2) {
-        ep = ld(P, cap, P_EP, ic);
+        ep = ep0;
         if (!failed) {
           if (model == 2 || model == 8 || model == 9) jp = ld(P, cap, P_JP, ic);
           if (model == 3) dil = ld(P, cap, P_DIL, ic);
         }
-        ep += plastic_project<EXT>(Fn, model, ep, jp, dil, dt, ld(P, cap, P_TEMP, ic), mat, tau);
+        ep += plastic_project<EXT>(Fn, model, ep, jp, dil, dt, temp0, mat, tau);
         new_flags |= FLAG_CACHE;
       } else if (MODE == 1) {
         elastic_tau(Fn, model, mat, 1.f, tau);
@@ -514,8 +533,6 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   }
 
   // -------------------------------------------------------------------- P2G
-  const float m = ld(P, cap, P_MASS, ic);
-  const float vol = ld(P, cap, P_VOL, ic);
   if (sp.guard) {
     // The stable step of the state this step starts from.
     bool slow = false;
