@@ -462,16 +462,21 @@ int mpm_batch_set_materials(mpm_batch* b, const mpm_material* mats, int32_t coun
     dm[i] = to_dev_material(mats[i]);
   }
   b->materials.assign(mats, mats + count);
-  bool ext = false, history = false;
+  bool ext = false, history = false, returns = false;
   for (int i = 0; i < count; ++i) {
     ext |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
     history |= keeps_history(mats[i]);
+    returns |= returns_in_g2p(mats[i]);
   }
+  b->d.returns = returns;
+  for (auto& sd : b->sc) sd.sp.returns = returns ? 1u : 0u;
   // captured launches name the kernel variant and the planes a re-bin moves
   if (ext != b->d.ext_laws || history != b->d.history) drop_graphs(b);
   b->d.ext_laws = ext;
   b->d.history = history;
   BHIP(b, hipMemcpyAsync(b->d.materials, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice, b->stream));
+  const int rc = upload_scenes_table(b);
+  if (rc) return rc;
   BHIP(b, hipStreamSynchronize(b->stream));
   return MPM_OK;
 }

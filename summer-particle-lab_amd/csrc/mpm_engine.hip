@@ -301,6 +301,7 @@ StepParams params(const mpm_engine* e, double dt, int* rc = nullptr) {
   p.dt = (float)dt;
   p.mass_scale = (float)e->mass_scale;
   p.inv_mass_scale = (float)(1.0 / e->mass_scale);
+  p.returns = e->d.returns ? 1u : 0u;
   const int r = guard_params(e->cfg, dt, p);
   if (rc) *rc = r;
   return p;
@@ -462,9 +463,11 @@ int mpm_set_materials(mpm_engine* e, const mpm_material* mats, int32_t count) {
   e->materials.assign(mats, mats + count);
   e->d.ext_laws = false;
   e->d.history = false;
+  e->d.returns = false;
   for (int i = 0; i < count; ++i) {
     e->d.ext_laws |= mats[i].model == 8 || mats[i].model == 9 || mats[i].fracture != 0;
     e->d.history |= keeps_history(mats[i]);
+    e->d.returns |= returns_in_g2p(mats[i]);
   }
   if (e->d.mlp) {  // a new table switches the surrogate off
     HIP_OK(e, hipStreamSynchronize(e->stream));

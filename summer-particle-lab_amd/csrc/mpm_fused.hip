@@ -303,9 +303,11 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     dmg = mat.fr_on ? ld(P, cap, P_DMG, ic) : 0.f;
     model = effective_model(mat, dmg);
   }
-  // Plastic strain and temperature of a return in G2P (models 2 and up, and
-  // a particle that may fail into sand), loaded before the gather.
-  const bool hist = MODE != 0 && (model >= 2 || (EXT && mat.fr_on));
+  // Plastic strain and temperature of a return in G2P, loaded with the
+  // first loads when some material of the scene returns (models 2 and up, a
+  // particle that may fail into sand): the material record arrives while the
+  // gather runs.
+  const bool hist = MODE != 0 && sp.returns;
   const float ep0 = hist ? ld(P, cap, P_EP, ic) : 0.f;
   const float temp0 = hist ? ld(P, cap, P_TEMP, ic) : 0.f;
 

@@ -55,6 +55,9 @@ struct StepParams {
   uint32_t guard = 0;
   float wave_limit = 3.0e38f;
   float grad_limit = 3.0e38f;
+  // Some material returns in G2P (models 2 and up, or fracture): the fused
+  // kernel then loads ep and T with the particle's first loads.
+  uint32_t returns = 1;
 };
 
 // Bits of the guard's status word (one word per scene).
@@ -66,6 +69,7 @@ struct DevState {
   bool ext_laws;      // some material uses damage or a pressure model (8, 9)
   bool narrow;        // sparse cells (under 12 particles each): 8-lane scatter chunks
   bool history;       // some material keeps Jp, dilation or damage (models 2, 3, 8, 9, fracture)
+  bool returns;       // some material returns in G2P (models 2 and up, fracture)
   const float4* deltas;      // per-slot grid velocity deltas of a split step (N^3 each), or null
   const int32_t* delta_slot; // per material: delta slot or -1 [64]
   float* planes_alt;  // permutation target
@@ -230,6 +234,8 @@ DevMaterial to_dev_material(const mpm_material& m);
 inline bool keeps_history(const mpm_material& m) {
   return m.model == 2 || m.model == 3 || m.model == 8 || m.model == 9 || m.fracture != 0;
 }
+// A material whose particles take a plastic return (and read ep and T) in G2P.
+inline bool returns_in_g2p(const mpm_material& m) { return m.model >= 2 || m.fracture != 0; }
 
 // Radix sort (mpm_sort.hip): stable LSD sort of (keys, vals) pairs. On return
 // the sorted pairs are in (keys, vals) (the buffers are swapped as needed).
