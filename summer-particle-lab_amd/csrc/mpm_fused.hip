@@ -119,7 +119,8 @@ __device__ __forceinline__ int wave_extreme(int x) {
 // lanes with `use` set. Returns false when no lane is in use. With `top`
 // set, *top becomes the block's maximum of every lane's *top in the same
 // pass (s_box holds 7 words per wave). Called with every lane of the block
-// active.
+// active, at most once per s_box: no barrier guards s_box against a
+// previous call's readers.
 template <int WAVES>
 __device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], int hi[3], int* s_box,
                                           int* top = nullptr) {
@@ -131,7 +132,6 @@ __device__ __forceinline__ bool block_box(const int b[3], bool use, int lo[3], i
   }
   const int wtop = top ? wave_extreme<true>(*top) : 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
   if (lane == 0) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -535,6 +535,14 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   }
 
   // -------------------------------------------------------------------- P2G
+  // The node box is cleared whole here (the gather's box, which shared it,
+  // is done with it): the barrier of the block box below then also orders
+  // the clearing before the scatter's atomics.
+  {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    i4* z = reinterpret_cast<i4*>(s_raw);
+    for (int p = (int)tid; p < S::ACC_BYTES / 16; p += FBK) z[p] = i4{0, 0, 0, 0};
+  }
   if (sp.guard) {
     // The stable step of the state this step starts from.
     bool slow = false;
@@ -639,7 +647,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   }
   int top = __float_as_int(bnd);  // non-negative floats order as their bits
   int lo[3], hi[3];
-  block_box<S::WAVES>(b, valid, lo, hi, s_box, &top);
+  block_box<S::WAVES>(b, valid, lo, hi, s_box + 7 * S::WAVES, &top);  // the gather's box words stay apart
   const float bmax = __int_as_float(top);
   static_assert(FBK <= 256, "the fixed-point bound counts at most 256 particles per node");
   const int qexp = min(126, 24 - frexp_exponent(bmax));
@@ -677,14 +685,6 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const int box_first = per_wave ? (int)lane : (int)tid, box_stride = per_wave ? 64 : FBK;
   int* s_q = reinterpret_cast<int*>(s_raw) + box0;                                  // momentum x, y, z planes
   uint32_t* s_mq = reinterpret_cast<uint32_t*>(s_raw + 12 * S::PATCH_ACC) + box0;  // fixed-point mass
-  if (staged) {
-    for (int p = box_first; p < cnt; p += box_stride) {
-      s_mq[p] = 0u;
-      s_q[p] = 0;
-      s_q[S::PATCH_ACC + p] = 0;
-      s_q[2 * S::PATCH_ACC + p] = 0;
-    }
-  }
   const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
   const int gbase = (int)key;  // linear index of the base node
 
@@ -844,7 +844,6 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   for (int p = 0; p < NPASS; ++p) {
     constexpr int R_LAST = 27 - (NPASS - 1) * R_PASS;
     const int rows = p == NPASS - 1 ? R_LAST : R_PASS;
-    if (p == 0) __syncthreads();  // node box cleared before any atomic
 #pragma unroll
     for (int r = 0; r < R_PASS; ++r) {
       const int g = p * R_PASS + r;
@@ -925,7 +924,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(SUR ? 3 : 4,
     uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts,
     uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB, SUR>::LDS_BYTES];
-  __shared__ int s_box[7 * Shape<FB, SUR>::WAVES];
+  __shared__ int s_box[2 * 7 * Shape<FB, SUR>::WAVES];
   fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
                                       acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
                                       delta_slot);
@@ -942,7 +941,7 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
     uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FBK, false>::LDS_BYTES];
-  __shared__ int s_box[7 * Shape<FBK, false>::WAVES];
+  __shared__ int s_box[2 * 7 * Shape<FBK, false>::WAVES];
   const uint32_t t = blockIdx.x;
   if (t >= b.nchunks) return;
   // The chunk and its scene's parameters are the same for every lane, but
