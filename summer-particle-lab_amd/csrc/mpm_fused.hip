@@ -909,6 +909,17 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   }
 }
 
+// Workgroups go to the chip's 8 XCDs in turn (workgroup b to XCD b mod 8),
+// and each XCD has its own L2. The chunk of workgroup b is chosen so that
+// every XCD takes one contiguous range of chunks: neighbouring chunks of the
+// cell order, whose node boxes overlap, then share an L2.
+constexpr uint32_t XCDS = 8;
+__device__ __forceinline__ uint32_t xcd_chunk() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t x = b % XCDS, k = b / XCDS, q = nb / XCDS, r = nb % XCDS;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Single scene: one chunk of FB particles per workgroup. Four workgroups per
 // CU fit the LDS (4 x 36.3 KB of 160 KB) and the register budget is held to
 // the same four waves per SIMD (the compiler then fits 116 VGPRs without
@@ -925,7 +936,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(SUR ? 3 : 4,
     uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB, SUR>::LDS_BYTES];
   __shared__ int s_box[2 * 7 * Shape<FB, SUR>::WAVES];
-  fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, blockIdx.x * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
+  fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, xcd_chunk() * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
                                       acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
                                       delta_slot);
 }
@@ -942,7 +953,7 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FBK, false>::LDS_BYTES];
   __shared__ int s_box[2 * 7 * Shape<FBK, false>::WAVES];
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_chunk();
   if (t >= b.nchunks) return;
   // The chunk and its scene's parameters are the same for every lane, but
   // the kernel writes memory, so the compiler reads them with vector loads;
