@@ -214,52 +214,98 @@ __global__ __launch_bounds__(BLOCK) void k_compact_pieces(uint8_t* __restrict__ 
 }
 
 // One node of a listed piece (node index within its scene's grid).
+// Accumulated state of one node, loaded ahead of its update.
+struct NodeIn {
+  unsigned long long M;
+  float px, py, pz;
+  bool pin;
+};
+
+__device__ __forceinline__ NodeIn load_node(size_t node, const unsigned long long* __restrict__ acc_m,
+                                            const float* __restrict__ acc_p, size_t pstride,
+                                            const uint8_t* __restrict__ fixed) {
+  NodeIn a;
+  a.M = acc_m[node];
+  a.pin = fixed[node] != 0;
+  a.px = acc_p[node];
+  a.py = acc_p[pstride + node];
+  a.pz = acc_p[2 * pstride + node];
+  return a;
+}
+
+// n / d and n % d for d >= 1 and n / d < 2^22: the float quotient (three
+// roundings, relative error below 3 2^-24) is within one of the true one,
+// and one correction step makes it exact.
+__device__ __forceinline__ uint32_t div_small(uint32_t n, uint32_t d, float inv_d, uint32_t& r) {
+  int q = (int)((float)n * inv_d);
+  int rem = (int)(n - (uint32_t)q * d);
+  if (rem < 0) {
+    --q;
+    rem += (int)d;
+  } else if (rem >= (int)d) {
+    ++q;
+    rem -= (int)d;
+  }
+  r = (uint32_t)rem;
+  return (uint32_t)q;
+}
+
+__device__ __forceinline__ void update_node(size_t node, const NodeIn& a, unsigned long long* __restrict__ acc_m,
+                                            float* __restrict__ acc_p, size_t pstride, uint8_t* __restrict__ fixed,
+                                            float4* __restrict__ vel, const GridDims& g, const StepParams& sp,
+                                            const int lo_idx[3], const int hi_idx[3], long long& Mq, double& p0,
+                                            double& p1, double& p2) {
+  Mq += (long long)a.M;
+  p0 += a.px;
+  p1 += a.py;
+  p2 += a.pz;
+  const float m = (float)a.M * sp.inv_mass_scale;
+  float v[3] = {a.px, a.py, a.pz};
+  if (m > g.mass_eps) {
+    const float im = 1.f / m;
+    const float gd = g.gravity_on_particles ? 0.f : sp.dt;
+    v[0] = a.px * im + g.gravity[0] * gd;
+    v[1] = a.py * im + g.gravity[1] * gd;
+    v[2] = a.pz * im + g.gravity[2] * gd;
+    // a scene's N^3 nodes fit 32 bits, and node / N < N^2 < 2^22 (N < 2048)
+    const uint32_t N = (uint32_t)g.N, nd = (uint32_t)node;
+    const float invN = 1.f / (float)N;
+    uint32_t ix, iy, iz;
+    const uint32_t row = div_small(nd, N, invN, ix);
+    iz = div_small(row, N, invN, iy);
+    const int idx[3] = {(int)ix, (int)iy, (int)iz};
+    bool sticky = a.pin;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if ((g.wall[2 * k] == MPM_WALL_STICKY && idx[k] <= lo_idx[k]) ||
+          (g.wall[2 * k + 1] == MPM_WALL_STICKY && idx[k] >= hi_idx[k]))
+        sticky = true;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const bool lo_hit = idx[k] <= lo_idx[k] && (g.wall[2 * k] == MPM_WALL_SLIP || v[k] < 0.f);
+      const bool hi_hit = idx[k] >= hi_idx[k] && (g.wall[2 * k + 1] == MPM_WALL_SLIP || v[k] > 0.f);
+      if (sticky || lo_hit || hi_hit) v[k] = 0.f;
+    }
+  }
+  // w carries the node mass, negative at a node a pinned particle holds (the
+  // transfers read only xyz; grid views read the flag back).
+  vel[node] = make_float4(v[0], v[1], v[2], a.pin ? -m : m);
+  acc_m[node] = 0ull;
+  acc_p[node] = 0.f;
+  acc_p[pstride + node] = 0.f;
+  acc_p[2 * pstride + node] = 0.f;
+  fixed[node] = 0;
+}
+
 __device__ __forceinline__ void grid_node(size_t node, size_t Nn, unsigned long long* __restrict__ acc_m,
                                           float* __restrict__ acc_p, size_t pstride, uint8_t* __restrict__ fixed,
                                           float4* __restrict__ vel, const GridDims& g, const StepParams& sp,
                                           const int lo_idx[3], const int hi_idx[3], long long& Mq, double& p0,
                                           double& p1, double& p2) {
   if (node >= Nn) return;
-  const unsigned long long M = acc_m[node];
-  const bool pin = fixed[node] != 0;
-  Mq += (long long)M;
-  const float px = acc_p[node], py = acc_p[pstride + node], pz = acc_p[2 * pstride + node];
-  p0 += px;
-  p1 += py;
-  p2 += pz;
-  const float m = (float)M * sp.inv_mass_scale;
-  float v[3] = {px, py, pz};
-  if (m > g.mass_eps) {
-    const float im = 1.f / m;
-    const float gd = g.gravity_on_particles ? 0.f : sp.dt;
-    v[0] = px * im + g.gravity[0] * gd;
-    v[1] = py * im + g.gravity[1] * gd;
-    v[2] = pz * im + g.gravity[2] * gd;
-    const uint32_t N = (uint32_t)g.N, nd = (uint32_t)node;  // a scene's N^3 nodes fit 32 bits
-    const int ix = (int)(nd % N), iy = (int)((nd / N) % N), iz = (int)(nd / (N * N));
-    const int idx[3] = {ix, iy, iz};
-    bool sticky = pin;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if ((g.wall[2 * a] == MPM_WALL_STICKY && idx[a] <= lo_idx[a]) ||
-          (g.wall[2 * a + 1] == MPM_WALL_STICKY && idx[a] >= hi_idx[a]))
-        sticky = true;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const bool lo_hit = idx[a] <= lo_idx[a] && (g.wall[2 * a] == MPM_WALL_SLIP || v[a] < 0.f);
-      const bool hi_hit = idx[a] >= hi_idx[a] && (g.wall[2 * a + 1] == MPM_WALL_SLIP || v[a] > 0.f);
-      if (sticky || lo_hit || hi_hit) v[a] = 0.f;
-    }
-  }
-  // w carries the node mass, negative at a node a pinned particle holds (the
-  // transfers read only xyz; grid views read the flag back).
-  vel[node] = make_float4(v[0], v[1], v[2], pin ? -m : m);
-  acc_m[node] = 0ull;
-  acc_p[node] = 0.f;
-  acc_p[pstride + node] = 0.f;
-  acc_p[2 * pstride + node] = 0.f;
-  fixed[node] = 0;
+  const NodeIn a = load_node(node, acc_m, acc_p, pstride, fixed);
+  update_node(node, a, acc_m, acc_p, pstride, fixed, vel, g, sp, lo_idx, hi_idx, Mq, p0, p1, p2);
 }
 
 // Dense view of this step's grid (after k_grid_list): mass, velocity and the
@@ -343,9 +389,16 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
   uint32_t cur = 0xffffffffu;
   long long Mq = 0;
   double p0 = 0, p1 = 0, p2 = 0;
-  for (uint32_t e = e0 + wave; e < e1; e += WAVES) {
-    const uint32_t gp = list[(start + e) % npieces];  // piece index in the (concatenated) grid
-    if constexpr (BATCH) {
+  // List slot of position start + e: the running position modulo the list
+  // capacity, reduced once (e < count <= npieces).
+  const uint32_t s0 = (uint32_t)(start % npieces);
+  auto slot = [&](uint32_t e) {
+    const uint32_t k = s0 + e;
+    return k >= npieces ? k - npieces : k;
+  };
+  if constexpr (BATCH) {
+    for (uint32_t e = e0 + wave; e < e1; e += WAVES) {
+      const uint32_t gp = list[slot(e)];  // piece index in the (concatenated) grid
       const uint32_t scene = block_scene[gp >> 2];
       if (scene != cur) {
         if (cur != 0xffffffffu) {
@@ -362,12 +415,13 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
       const size_t Nn = (size_t)sd->g.N * sd->g.N * sd->g.N;
       grid_node((size_t)(gp - (node0 >> 6)) * 64 + lane, Nn, acc_m + node0, acc_p + node0, node_stride,
                 fixed + node0, vel + node0, sd->g, sd->sp, sd->lo_idx, sd->hi_idx, Mq, p0, p1, p2);
-    } else {
-      cur = 0;
-      const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
-      const size_t Nn = (size_t)g1.N * g1.N * g1.N;
-      grid_node((size_t)gp * 64 + lane, Nn, acc_m, acc_p, Nn, fixed, vel, g1, sp1, lo, hi, Mq, p0, p1, p2);
     }
+  } else {
+    cur = 0;
+    const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    const size_t Nn = (size_t)g1.N * g1.N * g1.N;
+    for (uint32_t e = e0 + wave; e < e1; e += WAVES)
+      grid_node((size_t)list[slot(e)] * 64 + lane, Nn, acc_m, acc_p, Nn, fixed, vel, g1, sp1, lo, hi, Mq, p0, p1, p2);
   }
   Mq = wave_sum(Mq);
   p0 = wave_sum(p0);
