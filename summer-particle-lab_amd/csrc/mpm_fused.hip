@@ -36,7 +36,7 @@ namespace mpm {
 namespace {
 
 constexpr int FB = 256;            // particles per chunk of the single-scene launch
-constexpr int TR_ROWS = 5;         // per-wave transpose: one row per node of a pass, {mass, momentum} per lane
+constexpr int TR_ROWS = 6;         // per-wave transpose: one row per node of a pass, {mass, momentum} per lane
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 entry: rows start in different banks
 constexpr int CHUNK_MAX = 16;      // lanes per scatter chunk (run pieces summed by one lane), at most
 constexpr int TR_TAIL = 16;        // zero entries after the last row
@@ -62,9 +62,12 @@ static_assert(TR_ROWS + TR_TAIL + CHUNK_MAX <= 64,
 // Dense cells (long runs) take 16-lane chunks; cells of about 8 particles
 // take 8-lane chunks, whose task reads carry half the dead entries (measured
 // at 8 particles per cell: 1214 against 1314 us of the fused kernel; at 64,
-// 16 lanes 132.7 against 136.9 us). The number of rows per pass, 4 to 9,
-// did not change the time; 5 rows keep a workgroup's LDS small enough for
-// four workgroups per CU.
+// 16 lanes 132.7 against 136.9 us). Six rows per pass are the most that
+// keep a workgroup's LDS (39.6 KB) small enough for four workgroups per CU:
+// 6 rows x 9 chunks (a wave of 8 particles per cell) is one round of the
+// wave's 64 lanes per pass, five passes (measured at four waves per SIMD:
+// 1008 against 1034 us with 5 rows on the large scene, 121 against 125 us on
+// the thermal boxes).
 template <int CH>
 struct Scatter {
   static constexpr int CHUNK = CH;
@@ -921,7 +924,7 @@ __device__ __forceinline__ uint32_t xcd_chunk() {
 }
 
 // Single scene: one chunk of FB particles per workgroup. Four workgroups per
-// CU fit the LDS (4 x 36.3 KB of 160 KB) and the register budget is held to
+// CU fit the LDS (4 x 39.6 KB of 160 KB) and the register budget is held to
 // the same four waves per SIMD (the compiler then fits 116 VGPRs without
 // spills; left at three waves it took about 148, and four waves ran the large
 // scene's fused step 13% faster). The surrogate variant's wave buffer holds
