@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "mpm_internal.h"
 #include "mpm_mlp_device.h"
@@ -36,61 +37,81 @@ namespace mpm {
 namespace {
 
 constexpr int FB = 256;            // particles per chunk of the single-scene launch
-constexpr int TR_ROWS = 6;         // per-wave transpose: one row per node of a pass, {mass, momentum} per lane
 constexpr int TR_STRIDE = 65;      // 64 lanes + 1 entry: rows start in different banks
 constexpr int CHUNK_MAX = 16;      // lanes per scatter chunk (run pieces summed by one lane), at most
 constexpr int TR_TAIL = 16;        // zero entries after the last row
 constexpr int SORT_PASSES = 8;     // cells of a wave grouped by minimum passes before P2G
-constexpr int TR_ENTRIES = TR_ROWS * TR_STRIDE + TR_TAIL;
-constexpr int TR_BYTES = TR_ENTRIES * 16 + 64 * 16;  // transpose rows, zero tail, chunk table
-static_assert(TR_BYTES % 16 == 0, "wave buffers must stay 16-byte aligned");
-
-// Bytes of one wave's LDS buffer: the transpose, or in the surrogate variant
-// at least the network's activations, which reuse it.
-template <bool SUR>
-constexpr int wave_bytes() {
-  return SUR && MLP_WAVE_LDS > TR_BYTES ? (MLP_WAVE_LDS + 15) / 16 * 16 : TR_BYTES;
-}
-static_assert((TR_ROWS - 1) * TR_STRIDE + 63 + CHUNK_MAX <= TR_ENTRIES,
-              "the run-sum reads past a row's last lane stay inside the rows and the zero tail");
-static_assert(TR_ROWS + TR_TAIL <= 64, "one store per lane zeroes the row pads and the tail");
 static_assert(CHUNK_MAX <= TR_TAIL, "the zero tail covers a chunk read past the last row");
-static_assert(TR_ROWS + TR_TAIL + CHUNK_MAX <= 64,
-              "one store per lane zeroes the pads, the tail and the row past the passes");
 
 // Shape of the scatter: chunks of CH lanes and R nodes transposed per pass.
 // Dense cells (long runs) take 16-lane chunks; cells of about 8 particles
 // take 8-lane chunks, whose task reads carry half the dead entries (measured
 // at 8 particles per cell: 1214 against 1314 us of the fused kernel; at 64,
-// 16 lanes 132.7 against 136.9 us). Six rows per pass are the most that
-// keep a workgroup's LDS (39.6 KB) small enough for four workgroups per CU:
-// 6 rows x 9 chunks (a wave of 8 particles per cell) is one round of the
-// wave's 64 lanes per pass, five passes (measured at four waves per SIMD:
-// 1008 against 1034 us with 5 rows on the large scene, 121 against 125 us on
-// the thermal boxes).
+// 16 lanes 132.7 against 136.9 us). A pass's tasks are R rows x chunks: at
+// 8 particles per cell a wave holds about 9 chunks, so 6 rows (five passes,
+// 54 tasks) keep each pass within one round of the wave's 64 lanes; at 64
+// per cell it holds about 4, and 7 rows (four passes) do. The per-wave
+// buffer holds R transpose rows, the zero tail and the chunk table
+// (int4 {first lane, last lane, box base, grid base} or, packed, int2
+// {first lane | (lanes - 1) << 6 | box base << 12, grid base}); the node box
+// takes 2.5 (sparse) or 2 (dense) nodes per particle, so that four
+// workgroups fit a CU's LDS. The dense shape packs its chunk table into 8
+// bytes per chunk to fit; the sparse one keeps 16 bytes, whose fields need
+// no unpacking (packing cost the large scene 1023 against 1008 us). Measured at four waves per SIMD: large scene
+// (8-lane chunks) 1008 us with 6 rows against 1034 with 5 and 1024 with 7
+// rows and a box of 2 nodes per particle; thermal boxes (16-lane chunks)
+// 116 us with 7 rows against 121 with 6 and 125 with 5.
 template <int CH>
 struct Scatter {
   static constexpr int CHUNK = CH;
-  static constexpr int R_PASS = TR_ROWS;
+  static constexpr int R_PASS = CH == 16 ? 7 : 6;
   static constexpr int NPASS = (27 + R_PASS - 1) / R_PASS;
-  static_assert(CH % 8 == 0 && CH <= CHUNK_MAX && R_PASS <= TR_ROWS, "scatter pass shape");
+  static constexpr int ROWS = R_PASS;  // transpose rows in the wave buffer
+  static constexpr int ENTRIES = ROWS * TR_STRIDE + TR_TAIL;
+  static constexpr bool PACKED = CH == 16;  // chunk table entries of 8 bytes
+  static constexpr int WAVE_BYTES = ENTRIES * 16 + 64 * (PACKED ? 8 : 16);  // rows, zero tail, chunk table
+  static constexpr int ACC_PER_PARTICLE_X2 = CH == 16 ? 4 : 5;  // node box nodes per particle, times two
+  static_assert(CH % 8 == 0 && CH <= CHUNK_MAX, "scatter pass shape");
+  static_assert(WAVE_BYTES % 16 == 0, "wave buffers must stay 16-byte aligned");
+  static_assert((ROWS - 1) * TR_STRIDE + 63 + CHUNK_MAX <= ENTRIES,
+                "the run-sum reads past a row's last lane stay inside the rows and the zero tail");
+  static_assert(ROWS + TR_TAIL + CHUNK_MAX <= 64,
+                "one store per lane zeroes the pads, the tail and the row past the passes");
 };
+
+// Chunk table entries, unpacked (int4) or packed (int2).
+__device__ __forceinline__ int4 make_chunk(int4*, int first, int last, int box, int grid) {
+  return make_int4(first, last, box, grid);
+}
+__device__ __forceinline__ int2 make_chunk(int2*, int first, int last, int box, int grid) {
+  return make_int2(first | ((last - first) << 6) | (box << 12), grid);
+}
+__device__ __forceinline__ int chunk_first(int4 c) { return c.x; }
+__device__ __forceinline__ int chunk_first(int2 c) { return c.x & 63; }
+__device__ __forceinline__ int chunk_span(int4 c) { return c.y - c.x; }  // lanes - 1
+__device__ __forceinline__ int chunk_span(int2 c) { return (c.x >> 6) & 63; }
+__device__ __forceinline__ int chunk_box(int4 c) { return c.z; }
+__device__ __forceinline__ int chunk_box(int2 c) { return c.x >> 12; }
+__device__ __forceinline__ int chunk_grid(int4 c) { return c.w; }
+__device__ __forceinline__ int chunk_grid(int2 c) { return c.y; }
 
 // Node g of the scatter order: the 26 nodes other than the centre (n = 13,
 // offset (1, 1, 1)) in index order, then the centre, which takes the
 // remainder of the fixed-point mass. Node index n = ii + 3 j + 9 k.
 __host__ __device__ constexpr int scatter_node(int g) { return g == 26 ? 13 : g + (g >= 13 ? 1 : 0); }
 
-// LDS plan of a workgroup of FBK particles (one per lane): a node box for the
-// scatter (32-bit fixed-point mass and momentum x, y, z planes, 16 B per
-// node, 2.5 nodes per particle) and one buffer per wave; the gather box
-// (16 B per node) reuses the whole buffer.
-template <int FBK, bool SUR>
+// LDS plan of a workgroup of FBK particles (one per lane) for scatter chunks
+// of CH lanes: a node box for the scatter (32-bit fixed-point mass and
+// momentum x, y, z planes, 16 B per node) and one buffer per wave (in the
+// surrogate variant at least the network's activations, which reuse it); the
+// gather box (16 B per node) reuses the whole buffer.
+template <int FBK, bool SUR, int CH>
 struct Shape {
   static constexpr int WAVES = FBK / 64;
-  static constexpr int PATCH_ACC = 5 * FBK / 2;
+  static constexpr int PATCH_ACC = Scatter<CH>::ACC_PER_PARTICLE_X2 * FBK / 2;
   static constexpr int ACC_BYTES = 16 * PATCH_ACC;
-  static constexpr int WAVE_BYTES = wave_bytes<SUR>();
+  static constexpr int WAVE_BYTES = SUR && MLP_WAVE_LDS > Scatter<CH>::WAVE_BYTES ? (MLP_WAVE_LDS + 15) / 16 * 16
+                                                                                   : Scatter<CH>::WAVE_BYTES;
   static constexpr int LDS_BYTES = ACC_BYTES + WAVES * WAVE_BYTES;
   static constexpr int PATCH_VEL = LDS_BYTES / 16;
   static_assert(ACC_BYTES % 16 == 0, "wave buffers must stay 16-byte aligned");
@@ -269,7 +290,9 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                                             int* s_box, uint32_t* __restrict__ status,
                                             const float4* __restrict__ deltas = nullptr,
                                             const int32_t* __restrict__ delta_slot = nullptr) {
-  using S = Shape<FBK, SUR>;
+  using S = Shape<FBK, SUR, CH>;
+  constexpr int TR_ROWS = Scatter<CH>::ROWS;
+  constexpr int TR_ENTRIES = Scatter<CH>::ENTRIES;
   constexpr int CHUNK = Scatter<CH>::CHUNK;
   constexpr int R_PASS = Scatter<CH>::R_PASS;
   constexpr int NPASS = Scatter<CH>::NPASS;
@@ -691,7 +714,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const int lbase = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
   const int gbase = (int)key;  // linear index of the base node
 
-  // Per-wave LDS: 9 transpose rows (the nodes of a slab) of 64 particles,
+  // Per-wave LDS: TR_ROWS transpose rows (the nodes of a pass) of 64 particles,
   // each entry {fixed-point mass part (as bits), momentum x, y, z}, a zero
   // tail, and the chunk table {first lane, last lane, box base, grid base
   // (-1 when the chunk holds no particle)}. The run sums read whole chunks
@@ -702,7 +725,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // in every slab.
   unsigned char* wbuf = s_raw + S::ACC_BYTES + (tid >> 6) * S::WAVE_BYTES;
   float4* s_tr = reinterpret_cast<float4*>(wbuf);
-  int4* s_chunk = reinterpret_cast<int4*>(wbuf + TR_ENTRIES * 16);
+  using ChunkT = typename std::conditional<Scatter<CH>::PACKED, int2, int4>::type;
+  ChunkT* s_chunk = reinterpret_cast<ChunkT*>(wbuf + TR_ENTRIES * 16);
 
   // The cell order as seen from each position: position p reads the key and
   // box base of the lane placed at p (through the first 512 bytes of the
@@ -750,7 +774,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     const int c = __popcll(chunk_starts & (upto >> 1));  // chunks before this position
     const uint64_t later = chunk_starts & ~upto;
     const int next = later ? __ffsll((long long)later) - 1 : 64;
-    s_chunk[c] = make_int4((int)lane, next - 1, at.y, svalid ? (int)skey : -1);
+    s_chunk[c] = make_chunk(s_chunk, (int)lane, next - 1, at.y, svalid ? (int)skey : -1);  // grid base -1: no particle
   }
 
   // Sum of one (row, chunk) task of a pass, for node n, added to the node
@@ -760,10 +784,10 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // next row, or the zero tail, all finite) are masked out by the bits of
   // `in`. Mass parts are below 2^23 in magnitude, so the 24-bit multiply-add
   // sums them exactly.
-  auto run_task = [&](int row, int n, const int4 ch) {
+  auto run_task = [&](int row, int n, const ChunkT ch) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const f4* src = reinterpret_cast<const f4*>(s_tr + row * TR_STRIDE + ch.x);
-    const uint32_t in = (2u << (ch.y - ch.x)) - 1u;
+    const f4* src = reinterpret_cast<const f4*>(s_tr + row * TR_STRIDE + chunk_first(ch));
+    const uint32_t in = (2u << chunk_span(ch)) - 1u;
     int sq = 0;
     float px = 0.f, py = 0.f, pz = 0.f;
 #pragma unroll
@@ -789,17 +813,17 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         pz = fmaf(vals[q].w, mf, pz);
       }
     }
-    if (ch.w < 0) return;
+    if (chunk_grid(ch) < 0) return;
     const int k = n / 9, j = (n / 3) % 3, ii = n % 3;
     const unsigned long long mq = (unsigned long long)(long long)sq;
     if (staged) {
-      const int ln = ch.z + (k * ey + j) * ex + ii;
+      const int ln = chunk_box(ch) + (k * ey + j) * ex + ii;
       atomicAdd(&s_mq[ln], (uint32_t)sq);
       atomicAdd(&s_q[ln], __float2int_rn(px * qs));
       atomicAdd(&s_q[S::PATCH_ACC + ln], __float2int_rn(py * qs));
       atomicAdd(&s_q[2 * S::PATCH_ACC + ln], __float2int_rn(pz * qs));
     } else {
-      const size_t node = (size_t)(uint32_t)ch.w + ((size_t)k * N + j) * N + ii;
+      const size_t node = (size_t)(uint32_t)chunk_grid(ch) + ((size_t)k * N + j) * N + ii;
       atomicAdd(&acc_m[node], mq);
       atomicAdd(&acc_p[node], px);
       atomicAdd(&acc_p[pstride + node], py);
@@ -812,7 +836,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int4 ch0 = s_chunk[min((int)lane / R_PASS, 63)];
+  const ChunkT ch0 = s_chunk[min((int)lane / R_PASS, 63)];
 
   // Nodes a pinned particle touches with weight above 1e-8 are pinned
   // (src/three/mpm.js:83).
@@ -882,7 +906,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int ntask = rows * nchunks;
     for (int t = (int)lane; t < ntask; t += 64) {
-      int4 ch = ch0;
+      ChunkT ch = ch0;
       if (t >= 64 || rows != R_PASS) ch = s_chunk[t / rows];
       const int row = t % rows;
       run_task(row, scatter_node(p * R_PASS + row), ch);
@@ -937,8 +961,8 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(SUR ? 3 : 4,
     unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
     uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts,
     uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
-  __shared__ __align__(16) unsigned char s_raw[Shape<FB, SUR>::LDS_BYTES];
-  __shared__ int s_box[2 * 7 * Shape<FB, SUR>::WAVES];
+  __shared__ __align__(16) unsigned char s_raw[Shape<FB, SUR, CH>::LDS_BYTES];
+  __shared__ int s_box[2 * 7 * Shape<FB, SUR, CH>::WAVES];
   fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, xcd_chunk() * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
                                       acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
                                       delta_slot);
@@ -954,8 +978,8 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
     uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status) {
-  __shared__ __align__(16) unsigned char s_raw[Shape<FBK, false>::LDS_BYTES];
-  __shared__ int s_box[2 * 7 * Shape<FBK, false>::WAVES];
+  __shared__ __align__(16) unsigned char s_raw[Shape<FBK, false, CH>::LDS_BYTES];
+  __shared__ int s_box[2 * 7 * Shape<FBK, false, CH>::WAVES];
   const uint32_t t = xcd_chunk();
   if (t >= b.nchunks) return;
   // The chunk and its scene's parameters are the same for every lane, but
