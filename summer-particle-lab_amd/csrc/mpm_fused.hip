@@ -57,10 +57,10 @@ static_assert(CHUNK_MAX <= TR_TAIL, "the zero tail covers a chunk read past the 
 // takes 2.5 (sparse) or 2 (dense) nodes per particle, so that four
 // workgroups fit a CU's LDS. The dense shape packs its chunk table into 8
 // bytes per chunk to fit; the sparse one keeps 16 bytes, whose fields need
-// no unpacking (packing cost the large scene 1023 against 1008 us). Measured at four waves per SIMD: large scene
-// (8-lane chunks) 1008 us with 6 rows against 1034 with 5 and 1024 with 7
-// rows and a box of 2 nodes per particle; thermal boxes (16-lane chunks)
-// 116 us with 7 rows against 121 with 6 and 125 with 5.
+// no unpacking. Measured at four waves per SIMD, each pair on one GPU:
+// large scene (8-lane chunks) 1021 us with 6 rows against 1032 with 5;
+// thermal boxes (16-lane chunks) 115.7 us with 7 rows against 119.9 with 6
+// and 125.2 with 5.
 template <int CH>
 struct Scatter {
   static constexpr int CHUNK = CH;
