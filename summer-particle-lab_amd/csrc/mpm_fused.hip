@@ -255,9 +255,13 @@ __device__ __forceinline__ bool cfl_exceeded(const M3& F, const float C[9], cons
   }
   const float hard = model == 2 ? snow_hardening(jp) : 1.f;
   float comp = 1.f;
-  if (model == 1 || model == 8 || model == 9) comp = 1.f / fmaxf(.05f, det3(F) * (model == 1 ? 1.f : jp));
-  const float c = sqrtf((mat.lambda + 2.f * mat.mu) * hard * vol / m);
-  const float wave = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + c * fmaxf(fmaxf(1.f, row * col), comp);
+  if (model == 1 || model == 8 || model == 9)
+    comp = __builtin_amdgcn_rcpf(fmaxf(.05f, det3(F) * (model == 1 ? 1.f : jp)));
+  // Hardware reciprocal and square root (about 1 ulp): the limits carry the
+  // reference's 0.99999 margin, far wider than their error.
+  const float c = __builtin_amdgcn_sqrtf((mat.lambda + 2.f * mat.mu) * hard * vol * __builtin_amdgcn_rcpf(m));
+  const float wave =
+      __builtin_amdgcn_sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + c * fmaxf(fmaxf(1.f, row * col), comp);
   return !(wave <= sp.wave_limit) || !(gsum <= sp.grad_limit);
 }
 
@@ -398,70 +402,82 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       }
     }
     if (gather) {
-      float V[3] = {0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0}, Z[3] = {0, 0, 0};
-      // One x-row of three nodes at (j, k): the weighted sums of v and of
-      // its first moments.
+      // The weighted sums of the node velocities and of their first moments,
+      // V = sum w v, X, Y, Z = sum w v (offset along x, y, z), contracted
+      // axis by axis: each row of three nodes along x, then the rows along
+      // y for each k, then along z (the first term of each sum is a product,
+      // not an add to zero).
       typedef float f4 __attribute__((ext_vector_type(4)));
-      auto add_row = [&](f4 n0, f4 n1, f4 n2, int j, int k) {
-        const float wjk = w[1][j] * w[2][k];
-        const float r0x = w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x;
-        const float r0y = w[0][0] * n0.y + w[0][1] * n1.y + w[0][2] * n2.y;
-        const float r0z = w[0][0] * n0.z + w[0][1] * n1.z + w[0][2] * n2.z;
-        const float r1x = w[0][1] * n1.x + 2.f * w[0][2] * n2.x;
-        const float r1y = w[0][1] * n1.y + 2.f * w[0][2] * n2.y;
-        const float r1z = w[0][1] * n1.z + 2.f * w[0][2] * n2.z;
-        const float ax = wjk * r0x, ay = wjk * r0y, az = wjk * r0z;
-        V[0] += ax;
-        V[1] += ay;
-        V[2] += az;
-        X[0] += wjk * r1x;
-        X[1] += wjk * r1y;
-        X[2] += wjk * r1z;
-        if (j) {
-          Y[0] += (float)j * ax;
-          Y[1] += (float)j * ay;
-          Y[2] += (float)j * az;
-        }
-        if (k) {
-          Z[0] += (float)k * ax;
-          Z[1] += (float)k * ay;
-          Z[2] += (float)k * az;
+      const float wx2 = 2.f * w[0][2], wy2 = 2.f * w[1][2], wz2 = 2.f * w[2][2];
+      auto contract = [&](auto node, float Vo[3], float Xo[3], float Yo[3], float Zo[3]) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float a0[3], a1[3], ay[3];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const f4 n0 = node(k, j, 0), n1 = node(k, j, 1), n2 = node(k, j, 2);
+            const float r0[3] = {w[0][0] * n0.x + w[0][1] * n1.x + w[0][2] * n2.x,
+                                 w[0][0] * n0.y + w[0][1] * n1.y + w[0][2] * n2.y,
+                                 w[0][0] * n0.z + w[0][1] * n1.z + w[0][2] * n2.z};
+            const float r1[3] = {w[0][1] * n1.x + wx2 * n2.x, w[0][1] * n1.y + wx2 * n2.y,
+                                 w[0][1] * n1.z + wx2 * n2.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              if (j == 0) {
+                a0[c] = w[1][0] * r0[c];
+                a1[c] = w[1][0] * r1[c];
+              } else {
+                a0[c] += w[1][j] * r0[c];
+                a1[c] += w[1][j] * r1[c];
+              }
+              if (j == 1) ay[c] = w[1][1] * r0[c];
+              if (j == 2) ay[c] += wy2 * r0[c];
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if (k == 0) {
+              Vo[c] = w[2][0] * a0[c];
+              Xo[c] = w[2][0] * a1[c];
+              Yo[c] = w[2][0] * ay[c];
+            } else {
+              Vo[c] += w[2][k] * a0[c];
+              Xo[c] += w[2][k] * a1[c];
+              Yo[c] += w[2][k] * ay[c];
+            }
+            if (k == 1) Zo[c] = w[2][1] * a0[c];
+            if (k == 2) Zo[c] += wz2 * a0[c];
+          }
         }
       };
+      float V[3], X[3], Y[3], Z[3];
       if (staged) {
         // Through an LDS-qualified pointer, so the reads are ds_read (a
         // pointer shared with the global path would make them flat loads).
         typedef const __attribute__((address_space(3))) f4 lds_f4;
         lds_f4* sv = (lds_f4*)s_vel;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const int r = ((b[2] - lo[2] + k) * ey + (b[1] - lo[1] + j)) * ex + (b[0] - lo[0]);
-            add_row(sv[r], sv[r + 1], sv[r + 2], j, k);
-          }
+        const int r00 = ((b[2] - lo[2]) * ey + (b[1] - lo[1])) * ex + (b[0] - lo[0]);
+        contract([&](int k, int j, int i) { return sv[r00 + (k * ey + j) * ex + i]; }, V, X, Y, Z);
       } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const size_t r = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
-            const f4* gv = reinterpret_cast<const f4*>(vel);
-            add_row(gv[r], gv[r + 1], gv[r + 2], j, k);
-          }
+        const f4* gv = reinterpret_cast<const f4*>(vel);
+        contract([&](int k, int j, int i) { return gv[((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0] + i]; }, V,
+                 X, Y, Z);
       }
       if constexpr (MODE == 2) {
         // The gather is linear in the node velocity: the part's grid deltas
         // add their own weighted sums.
         if (dl) {
           const f4* gd = reinterpret_cast<const f4*>(dl);
+          float dV[3], dX[3], dY[3], dZ[3];
+          contract([&](int k, int j, int i) { return gd[((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0] + i]; },
+                   dV, dX, dY, dZ);
 #pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              const size_t r = ((size_t)(b[2] + k) * N + (b[1] + j)) * N + b[0];
-              add_row(gd[r], gd[r + 1], gd[r + 2], j, k);
-            }
+          for (int c = 0; c < 3; ++c) {
+            V[c] += dV[c];
+            X[c] += dX[c];
+            Y[c] += dY[c];
+            Z[c] += dZ[c];
+          }
         }
       }
       const float s4 = 4.f * invH;
