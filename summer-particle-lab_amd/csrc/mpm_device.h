@@ -376,7 +376,8 @@ __device__ __forceinline__ float pressure_return(float e[3], int model, float& j
 // X = F F^T - I = G + G^T + G G^T (no digits cancel), log(I + X) is the
 // series X - X^2/2 + X^3/3 - X^4/4 + X^5/5, by Horner steps whose factors
 // are polynomials in X (so every product is symmetric). For |X|_F <= 1/32
-// the truncation error is below (1/32)^6 / 6 < 2e-10. eps has F's left
+// the truncation error is below (1/32)^6 / 6 < 2e-10 (shorter series for
+// smaller strains, below). eps has F's left
 // singular vectors and the log stretches as eigenvalues, so its trace and
 // deviatoric Frobenius norm are those of the principal log strains. Returns
 // false (eps unset) when |X|_F is above 1/32 or not finite. Symmetric
@@ -392,9 +393,15 @@ __device__ __forceinline__ bool hencky_small(const M3& F, float eps[6]) {
   X[4] = (g2 + g6) + (g0 * g6 + g1 * g7 + g2 * g8);
   X[5] = (g5 + g7) + (g3 * g6 + g4 * g7 + g5 * g8);
   const float r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + 2.f * (X[3] * X[3] + X[4] * X[4] + X[5] * X[5]);
-  if (!(r2 <= 1.f / 1024.f)) return false;
-  // P = X/5 - I/4, then P <- X P + c I for c = 1/3, -1/2, 1; log = X P.
-  float P[6] = {0.2f * X[0] - 0.25f, 0.2f * X[1] - 0.25f, 0.2f * X[2] - 0.25f, 0.2f * X[3], 0.2f * X[4], 0.2f * X[5]};
+  const bool small = r2 <= 1.f / 1024.f;
+  // Fewer terms where every active lane's strain is small enough that the
+  // dropped ones are below FP32 rounding relative to X: |X|_F <= 2^-12 keeps
+  // X - X^2/2 (the next term is |X|^2/3 <= 2^-24/3 of X), |X|_F <= 2^-7 up
+  // to X^3/3 (|X|^3/4 <= 2^-23 of X). Decided per wave, so the lanes take one
+  // path.
+  const bool over12 = __ballot(small && r2 > 5.9604645e-8f) != 0ull;  // |X| > 2^-12
+  const bool over7 = __ballot(small && r2 > 6.1035156e-5f) != 0ull;   // |X| > 2^-7
+  if (!small) return false;
   auto mul = [&](const float A[6], float c, float R[6]) {
     R[0] = X[0] * A[0] + X[3] * A[3] + X[4] * A[4] + c;
     R[1] = X[3] * A[3] + X[1] * A[1] + X[5] * A[5] + c;
@@ -403,10 +410,24 @@ __device__ __forceinline__ bool hencky_small(const M3& F, float eps[6]) {
     R[4] = X[0] * A[4] + X[3] * A[5] + X[4] * A[2];
     R[5] = X[3] * A[4] + X[1] * A[5] + X[5] * A[2];
   };
-  float Q[6];
-  mul(P, 1.f / 3.f, Q);
-  mul(Q, -0.5f, P);
-  mul(P, 1.f, Q);
+  float P[6], Q[6];
+  if (over7) {
+    // P = X/5 - I/4, then P <- X P + c I for c = 1/3, -1/2, 1; log = X P.
+    const float P0[6] = {0.2f * X[0] - 0.25f, 0.2f * X[1] - 0.25f, 0.2f * X[2] - 0.25f,
+                         0.2f * X[3], 0.2f * X[4], 0.2f * X[5]};
+    mul(P0, 1.f / 3.f, Q);
+    mul(Q, -0.5f, P);
+    mul(P, 1.f, Q);
+  } else if (over12) {
+    // Q = X/3 - I/2, then X Q + I; log = X (X Q + I).
+    const float Q0[6] = {X[0] * (1.f / 3.f) - 0.5f, X[1] * (1.f / 3.f) - 0.5f, X[2] * (1.f / 3.f) - 0.5f,
+                         X[3] * (1.f / 3.f), X[4] * (1.f / 3.f), X[5] * (1.f / 3.f)};
+    mul(Q0, 1.f, Q);
+  } else {
+    // log = X (I - X/2).
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Q[k] = -0.5f * X[k] + (k < 3 ? 1.f : 0.f);
+  }
   mul(Q, 0.f, P);
 #pragma unroll
   for (int k = 0; k < 6; ++k) eps[k] = 0.5f * P[k];
