@@ -798,8 +798,9 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   // read unconditionally, so the loads issue back to back; entries past the
   // chunk's last lane (at most CHUNK - 1 past the row: the row's pad, the
   // next row, or the zero tail, all finite) are masked out by the bits of
-  // `in`. Mass parts are below 2^23 in magnitude, so the 24-bit multiply-add
-  // sums them exactly.
+  // `in`. Mass parts are below 2^23 and sit in an entry's low 24 bits (above
+  // them an entry may hold the float bits of the 2^23 of its rounding), so
+  // the 24-bit multiply-add reads and sums them exactly.
   auto run_task = [&](int row, int n, const ChunkT ch) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const f4* src = reinterpret_cast<const f4*>(s_tr + row * TR_STRIDE + chunk_first(ch));
@@ -813,7 +814,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       for (int q = 0; q < 8; ++q) vals[q] = src[h + q];
       int qfirst = 0;
       if (h == 0) {  // a chunk holds at least its first entry
-        sq = __float_as_int(vals[0].x);
+        sq = __float_as_int(vals[0].x) & 0xffffff;
         px = vals[0].y;
         py = vals[0].z;
         pz = vals[0].w;
@@ -882,7 +883,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     c1x2[c] = 2.f * c1[c];
     c2x2[c] = 2.f * c2[c];
   }
-  int qsum = 0;
+  uint32_t qsum = 0u;  // the other parts' entries, each 2^23 (as float bits) plus the part
 #pragma unroll
   for (int p = 0; p < NPASS; ++p) {
     constexpr int R_LAST = 27 - (NPASS - 1) * R_PASS;
@@ -896,12 +897,14 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         const float wt = w[0][ii] * wyz[j][k];
         int qm;
         if (g == 26) {
-          qm = Q - qsum;
+          qm = (int)((uint32_t)Q + 26u * 0x4b000000u - qsum);
         } else {
           // rint(wt Qf) for 0 <= wt Qf < 2^23: the sum with 2^23 rounds to
-          // the nearest integer and its low mantissa bits hold it.
-          qm = __float_as_int(fmaf(wt, Qf, 8388608.f)) - 0x4b000000;
-          qsum += qm;
+          // the nearest integer and its low mantissa bits hold it. The entry
+          // keeps the float's bits: the run sums read the part from the low
+          // 24 bits.
+          qm = __float_as_int(fmaf(wt, Qf, 8388608.f));
+          qsum += (uint32_t)qm;
         }
         // q0 + ii c0 + j c1 + k c2 without multiplying by zero offsets (the
         // row part is shared by the three nodes of a row).
