@@ -215,7 +215,7 @@ void particles(mpm_batch* b, int mode) {
 }
 
 void grid(mpm_batch* b) {
-  launch_compact(b->stream, b->d, b->total_nodes / 64, (uint32_t)b->S);
+  launch_compact(b->stream, b->d, pieces_of_nodes(b->total_nodes), (uint32_t)b->S);
   launch_grid_batch(b->stream, b->d, b->d_scenes, b->d_block_scene, b->total_nodes, b->grid_blocks);
 }
 
@@ -364,7 +364,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   alloc((void**)&d.acc_p, sizeof(float) * 3 * Nt);
   alloc((void**)&d.vel, sizeof(float4) * Nt);
   alloc((void**)&d.fixed, Nt);
-  alloc((void**)&d.pieces, piece_flag_bytes(Nt / 64));
+  alloc((void**)&d.pieces, piece_flag_bytes(pieces_of_nodes(Nt)));
   alloc((void**)&d.keys, sizeof(uint32_t) * cap);
   alloc((void**)&d.keys_alt, sizeof(uint32_t) * cap);
   alloc((void**)&d.vals, sizeof(uint32_t) * cap);
@@ -373,7 +373,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   alloc((void**)&d.hist, sizeof(uint32_t) * hist_words);
   alloc((void**)&d.partials, sizeof(uint32_t) * part_words);
   alloc((void**)&d.red, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes);
-  alloc((void**)&d.piece_list, sizeof(uint32_t) * (Nt / 64));
+  alloc((void**)&d.piece_list, sizeof(uint32_t) * pieces_of_nodes(Nt));
   alloc((void**)&d.piece_counters, sizeof(unsigned long long) * 3);
   alloc((void**)&d.materials, sizeof(DevMaterial) * 64);
   alloc((void**)&b->d_scenes, sizeof(SceneDesc) * nscenes);
@@ -388,12 +388,12 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nt, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nt, b->stream);
-  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, piece_flag_bytes(Nt / 64), b->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, piece_flag_bytes(pieces_of_nodes(Nt)), b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes, b->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.piece_counters, 0, sizeof(unsigned long long) * 3, b->stream);
   // Eight grid update workgroups per CU: a batch touches pieces of every
   // scene each step.
-  b->grid_blocks = (uint32_t)std::max(1, std::min<int>(8 * cus, (int)((Nt / 64 + 3) / 4)));
+  b->grid_blocks = (uint32_t)std::max(1, std::min<int>(8 * cus, (int)((pieces_of_nodes(Nt) + 15) / 16)));
   if (st == hipSuccess) st = hipMemsetAsync(d.planes, 0, sizeof(float) * NPLANES * (size_t)cap, b->stream);
   std::vector<uint32_t> block_scene(b->total_blocks);
   for (int32_t s = 0; s < nscenes; ++s) {

@@ -379,7 +379,7 @@ int mpm_create(const mpm_config* c, mpm_engine** out) {
   alloc((void**)&d.acc_p, sizeof(float) * 3 * Nn);
   alloc((void**)&d.vel, sizeof(float4) * Nn);
   alloc((void**)&d.fixed, Nn);
-  alloc((void**)&d.pieces, piece_flag_bytes((Nn + 63) / 64));
+  alloc((void**)&d.pieces, piece_flag_bytes(pieces_of_nodes(Nn)));
   alloc((void**)&d.keys, sizeof(uint32_t) * cap);
   alloc((void**)&d.keys_alt, sizeof(uint32_t) * cap);
   alloc((void**)&d.vals, sizeof(uint32_t) * cap);
@@ -400,7 +400,7 @@ int mpm_create(const mpm_config* c, mpm_engine** out) {
   if (st == hipSuccess) st = hipMemsetAsync(d.acc_p, 0, sizeof(float) * 3 * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.vel, 0, sizeof(float4) * Nn, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.fixed, 0, Nn, e->stream);
-  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, piece_flag_bytes((Nn + 63) / 64), e->stream);
+  if (st == hipSuccess) st = hipMemsetAsync(d.pieces, 0, piece_flag_bytes(pieces_of_nodes(Nn)), e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.red, 0, sizeof(double) * 4 * SUM_SLOTS, e->stream);
   if (st == hipSuccess) st = hipMemsetAsync(d.piece_counters, 0, sizeof(unsigned long long) * 3, e->stream);
   int cus = 256;
@@ -409,8 +409,8 @@ int mpm_create(const mpm_config* c, mpm_engine** out) {
   // nodes), rising to four per CU from 2^18 pieces (a 256^3-cell grid has
   // 265k). Workgroups past the step's list exit at once but each still costs
   // a dispatch, which a small grid's short list does not repay.
-  const int per_cu = std::max(1, std::min(4, (int)(npieces >> 16)));
-  e->grid_blocks = (uint32_t)std::max(1, std::min<int>(per_cu * cus, (int)((npieces + 3) / 4)));
+  const int per_cu = std::max(1, std::min(4, (int)(((size_t)npieces * PIECE_NODES) >> 22)));
+  e->grid_blocks = (uint32_t)std::max(1, std::min<int>(per_cu * cus, (int)((npieces + 15) / 16)));
   if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
   if (st != hipSuccess) {
     mpm_destroy(e);
@@ -1025,7 +1025,7 @@ int mpm_debug_p2g(mpm_engine* e, double dt, double* grid_mass, double* grid_mome
   HIP_OK(e, hipMemsetAsync(e->d.acc_m, 0, 8 * Nn, e->stream));
   HIP_OK(e, hipMemsetAsync(e->d.acc_p, 0, 12 * Nn, e->stream));
   HIP_OK(e, hipMemsetAsync(e->d.fixed, 0, Nn, e->stream));
-  HIP_OK(e, hipMemsetAsync(e->d.pieces, 0, piece_flag_bytes((Nn + 63) / 64), e->stream));
+  HIP_OK(e, hipMemsetAsync(e->d.pieces, 0, piece_flag_bytes(pieces_of_nodes(Nn)), e->stream));
   HIP_OK(e, hipGetLastError());
   HIP_OK(e, hipStreamSynchronize(e->stream));
   for (size_t i = 0; i < Nn; ++i) {

@@ -845,7 +845,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
       atomicAdd(&acc_p[node], px);
       atomicAdd(&acc_p[pstride + node], py);
       atomicAdd(&acc_p[2 * pstride + node], pz);
-      pieces[node >> 6] = 1;
+      pieces[node >> PIECE_SHIFT] = 1;
     }
   };
   // The chunk of this lane's first task is the same in every full pass:
@@ -949,7 +949,7 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         atomicAdd(&acc_p[node], px);
         atomicAdd(&acc_p[pstride + node], py);
         atomicAdd(&acc_p[2 * pstride + node], pz);
-        pieces[node >> 6] = 1;
+        pieces[node >> PIECE_SHIFT] = 1;
       }
     }
   }
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const uint32_t node0 = __builtin_amdgcn_readfirstlane(sd->node0);
   const uint32_t end = __builtin_amdgcn_readfirstlane(sd->p0 + sd->n);
   fused_chunk<MODE, false, FBK, EXT, CH>(P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0,
-                                         acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> 6), nullptr,
+                                         acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> PIECE_SHIFT), nullptr,
                                          nullptr, s_raw, s_box, status + ch.x);
 }
 

@@ -187,6 +187,11 @@ uint32_t grid_pieces(const GridDims& g);
 // Bytes of the piece flag array for `npieces` pieces: padded with zeros to a
 // multiple of 16, since the compaction reads 16 flags per 16-byte load.
 inline size_t piece_flag_bytes(size_t npieces) { return (npieces + 15) & ~(size_t)15; }
+// The grid update works on pieces of 16 consecutive nodes (a quarter of a
+// wave each): node n is in piece n >> PIECE_SHIFT.
+constexpr int PIECE_SHIFT = 4;
+constexpr uint32_t PIECE_NODES = 1u << PIECE_SHIFT;
+inline uint32_t pieces_of_nodes(size_t nodes) { return (uint32_t)((nodes + PIECE_NODES - 1) >> PIECE_SHIFT); }
 // Batched launchers (mpm_kernels.hip, mpm_fused.hip): all scenes of a batch
 // in one launch; the particle launch runs one workgroup per chunk.
 void launch_bin_keys_batch(hipStream_t s, const DevState& d, const BatchArgs& b);

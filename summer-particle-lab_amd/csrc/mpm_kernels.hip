@@ -141,8 +141,8 @@ __device__ __forceinline__ void step_stress(const M3& F, int model, const DevMat
   }
 }
 
-// Grid update over the active pieces only. P2G flags every 64-node piece it
-// adds mass to; k_compact_pieces appends the flagged pieces to a circular
+// Grid update over the active pieces only. P2G flags every piece of 16
+// consecutive nodes it adds mass to; k_compact_pieces appends the flagged pieces to a circular
 // list (and clears the flags), and k_grid_list updates the pieces appended
 // since the previous update: v = p/m, gravity, wall rules
 // (src/three/mpm.js:89-102), clear of the accumulators. Grid sums
@@ -155,60 +155,79 @@ __device__ __forceinline__ void step_stress(const M3& F, int model, const DevMat
 
 // Compaction of the piece flags, G flags per lane. G = 1: a ballot per wave.
 // G = 16, for large grids: one 16-byte load per lane (flags are 0 or 1, so a
-// word's popcount is its number of set flags) and a wave scan, so the single
-// append counter takes 16 times fewer atomics; the flag array is zero-padded
-// to whole 16-byte groups. Either way one counter atomic per wave reserves
-// the wave's list positions. Block 0 also starts the step's list at the
-// previous update's end and clears this step's grid sums.
+// word's popcount is its number of set flags) and a wave scan; the flag array
+// is zero-padded to whole 16-byte groups. Either way a workgroup of 16 waves
+// sums its waves' counts in LDS and takes its list positions with one counter
+// atomic, so the single append counter sees one atomic per 1024 (G = 1) or
+// 16384 (G = 16) flags. Block 0 also starts the step's list at the previous
+// update's end and clears this step's grid sums.
+constexpr uint32_t COMPACT_BLOCK = 1024;
+
 template <int G>
-__global__ __launch_bounds__(BLOCK) void k_compact_pieces(uint8_t* __restrict__ pieces, uint32_t npieces,
-                                                          uint32_t* __restrict__ list,
-                                                          unsigned long long* __restrict__ counters,
-                                                          double* __restrict__ sums, uint32_t nsums) {
+__global__ __launch_bounds__(COMPACT_BLOCK) void k_compact_pieces(uint8_t* __restrict__ pieces, uint32_t npieces,
+                                                                  uint32_t* __restrict__ list,
+                                                                  unsigned long long* __restrict__ counters,
+                                                                  double* __restrict__ sums, uint32_t nsums) {
   static_assert(G == 1 || G == 16, "1 or 16 flags per lane");
+  __shared__ uint32_t s_off[COMPACT_BLOCK / 64];
+  __shared__ unsigned long long s_base;
   if (blockIdx.x == 0) {
-    for (uint32_t k = threadIdx.x; k < nsums; k += BLOCK) sums[k] = 0.0;
+    for (uint32_t k = threadIdx.x; k < nsums; k += COMPACT_BLOCK) sums[k] = 0.0;
     if (threadIdx.x == 0) counters[1] = counters[2];
   }
-  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t i = blockIdx.x * COMPACT_BLOCK + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  // This lane's flag count, the count of the lanes before it in the wave and
+  // the wave's total.
+  uint32_t cnt, before, total;
+  uint4 f = make_uint4(0u, 0u, 0u, 0u);
   if constexpr (G == 1) {
-    bool act = false;
-    if (i < npieces) {
-      act = pieces[i] != 0;
-      if (act) pieces[i] = 0;
+    cnt = 0;
+    if (i < npieces && pieces[i] != 0) {
+      cnt = 1;
+      pieces[i] = 0;
     }
-    const uint64_t m = __ballot(act);
-    unsigned long long base = 0;
-    if (lane == 0 && m) base = atomicAdd(&counters[0], (unsigned long long)__popcll(m));
-    base = __shfl(base, 0, 64);
-    if (act) list[(base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))) % npieces] = i;
+    const uint64_t m = __ballot(cnt != 0);
+    before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    total = (uint32_t)__popcll(m);
   } else {
     uint4* groups = reinterpret_cast<uint4*>(pieces);  // group i holds flags 16i .. 16i+15
-    uint4 f = make_uint4(0u, 0u, 0u, 0u);
     if (i < (npieces + 15u) / 16u) f = groups[i];
-    const uint32_t cnt = __popc(f.x) + __popc(f.y) + __popc(f.z) + __popc(f.w);
+    cnt = __popc(f.x) + __popc(f.y) + __popc(f.z) + __popc(f.w);
     uint32_t inc = cnt;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const uint32_t t = __shfl_up(inc, d, 64);
       if ((int)lane >= d) inc += t;
     }
-    const uint32_t total = __shfl(inc, 63, 64);
-    unsigned long long base = 0;
-    if (lane == 0 && total) base = atomicAdd(&counters[0], (unsigned long long)total);
-    base = __shfl(base, 0, 64);
-    if (cnt) {
-      uint32_t slot = (uint32_t)((base + (inc - cnt)) % npieces);
-      const uint32_t w[4] = {f.x, f.y, f.z, f.w};
+    before = inc - cnt;
+    total = __shfl(inc, 63, 64);
+    if (cnt) groups[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  if (lane == 0) s_off[wave] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < COMPACT_BLOCK / 64; ++w) {
+      const uint32_t x = s_off[w];
+      s_off[w] = t;
+      t += x;
+    }
+    s_base = t ? atomicAdd(&counters[0], (unsigned long long)t) : 0ull;
+  }
+  __syncthreads();
+  if (!cnt) return;
+  uint32_t slot = (uint32_t)((s_base + s_off[wave] + before) % npieces);
+  if constexpr (G == 1) {
+    list[slot] = i;
+  } else {
+    const uint32_t w[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) {
-          list[slot] = i * 16u + (uint32_t)k;
-          if (++slot == npieces) slot = 0;
-        }
+    for (int k = 0; k < 16; ++k) {
+      if ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) {
+        list[slot] = i * 16u + (uint32_t)k;
+        if (++slot == npieces) slot = 0;
       }
-      groups[i] = make_uint4(0u, 0u, 0u, 0u);
     }
   }
 }
@@ -318,9 +337,9 @@ __global__ __launch_bounds__(BLOCK) void k_grid_export(const float4* __restrict_
                                                        float* __restrict__ mass, float* __restrict__ v,
                                                        uint8_t* __restrict__ fixed) {
   const unsigned long long start = counters[1], end = counters[2];
-  const uint32_t lane = threadIdx.x & 63u, W = gridDim.x * (BLOCK / 64);
-  for (uint64_t e = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); e < end - start; e += W) {
-    const size_t node = (size_t)list[(start + e) % npieces] * 64 + lane;
+  const uint32_t W = gridDim.x * (BLOCK / PIECE_NODES);  // one piece per quarter-wave
+  for (uint64_t e = blockIdx.x * (BLOCK / PIECE_NODES) + (threadIdx.x >> PIECE_SHIFT); e < end - start; e += W) {
+    const size_t node = (size_t)list[(start + e) % npieces] * PIECE_NODES + (threadIdx.x & (PIECE_NODES - 1));
     if (node >= Nn) continue;
     const float4 u = vel[node];
     mass[node] = fabsf(u.w);
@@ -338,9 +357,9 @@ __global__ __launch_bounds__(BLOCK) void k_grid_import(float4* __restrict__ vel,
                                                        const unsigned long long* __restrict__ counters, size_t Nn,
                                                        const float* __restrict__ v) {
   const unsigned long long start = counters[1], end = counters[2];
-  const uint32_t lane = threadIdx.x & 63u, W = gridDim.x * (BLOCK / 64);
-  for (uint64_t e = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); e < end - start; e += W) {
-    const size_t node = (size_t)list[(start + e) % npieces] * 64 + lane;
+  const uint32_t W = gridDim.x * (BLOCK / PIECE_NODES);  // one piece per quarter-wave
+  for (uint64_t e = blockIdx.x * (BLOCK / PIECE_NODES) + (threadIdx.x >> PIECE_SHIFT); e < end - start; e += W) {
+    const size_t node = (size_t)list[(start + e) % npieces] * PIECE_NODES + (threadIdx.x & (PIECE_NODES - 1));
     if (node >= Nn) continue;
     float4 u = vel[node];
     u.x = v[3 * node];
@@ -360,12 +379,28 @@ __device__ __forceinline__ void flush_sums(double* __restrict__ sums, uint32_t s
   atomicAdd(s + 3, p2);
 }
 
+// Sums of the lanes in `need` (a wave-uniform mask) flushed to their scenes'
+// slots, one flush per distinct scene among them.
+__device__ __forceinline__ void flush_lanes(double* __restrict__ sums, uint32_t slot, uint64_t need, uint32_t cur,
+                                            long long Mq, double p0, double p1, double p2, uint32_t lane) {
+  while (need) {
+    const uint32_t sc = __shfl(cur, (int)__builtin_ctzll(need), 64);
+    const bool in = ((need >> lane) & 1ull) && cur == sc;
+    const long long m = wave_sum(in ? Mq : 0ll);
+    const double a = wave_sum(in ? p0 : 0.0), b = wave_sum(in ? p1 : 0.0), c = wave_sum(in ? p2 : 0.0);
+    if (lane == 0) flush_sums(sums, sc, slot, m, a, b, c);
+    need &= ~__ballot(in);
+  }
+}
+
 // Workgroup b updates a contiguous range of the step's list entries, at
-// least one per wave, its waves taking every (BLOCK / 64)-th entry of it;
-// workgroups past the end exit at once. Neighbouring entries mostly belong to
-// one scene, so a wave keeps running sums for its current scene and flushes
-// them only when the scene changes; at the end the waves of a workgroup
-// combine sums of equal scenes before the atomics.
+// least four per wave: each quarter-wave updates one piece of 16 nodes, the
+// waves taking every fourth group of four entries. Workgroups past the end
+// exit at once. Neighbouring entries mostly belong to one scene, so each lane
+// keeps running sums for its current scene; when a lane's scene changes the
+// wave flushes the old sums of the changing lanes, grouped by scene. At the
+// end the lanes of a wave combine sums of equal scenes, and the waves of a
+// workgroup combine the last group of each before the atomics.
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
                                                      uint8_t* __restrict__ fixed, float4* __restrict__ vel,
@@ -381,8 +416,8 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
   const unsigned long long start = counters[1], end = counters[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[2] = end;
   const uint32_t count = (uint32_t)(end - start);
-  constexpr uint32_t WAVES = BLOCK / 64;
-  const uint32_t per = max(WAVES, (count + gridDim.x - 1) / gridDim.x);
+  constexpr uint32_t WAVES = BLOCK / 64, QUARTERS = 64 / PIECE_NODES;
+  const uint32_t per = max(QUARTERS * WAVES, (count + gridDim.x - 1) / gridDim.x);
   const uint32_t e0 = blockIdx.x * per;
   if (e0 >= count) return;  // the same for the whole workgroup
   const uint32_t e1 = min(count, e0 + per);
@@ -396,43 +431,72 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
     const uint32_t k = s0 + e;
     return k >= npieces ? k - npieces : k;
   };
+  const uint32_t q = lane / PIECE_NODES, sub = lane & (PIECE_NODES - 1);
   if constexpr (BATCH) {
-    for (uint32_t e = e0 + wave; e < e1; e += WAVES) {
-      const uint32_t gp = list[slot(e)];  // piece index in the (concatenated) grid
-      const uint32_t scene = block_scene[gp >> 2];
-      if (scene != cur) {
-        if (cur != 0xffffffffu) {
-          const long long m = wave_sum(Mq);
-          const double a = wave_sum(p0), b = wave_sum(p1), c = wave_sum(p2);
-          if (lane == 0) flush_sums(sums, cur, blockIdx.x % SUM_SLOTS, m, a, b, c);
-        }
-        cur = scene;
-        Mq = 0;
-        p0 = p1 = p2 = 0;
+    for (uint32_t e4 = e0 + QUARTERS * wave; e4 < e1; e4 += QUARTERS * WAVES) {
+      const uint32_t e = e4 + q;
+      const bool valid = e < e1;
+      uint32_t gp = 0, scene = cur;
+      if (valid) {
+        gp = list[slot(e)];  // piece index in the (concatenated) grid
+        scene = block_scene[gp >> (8 - PIECE_SHIFT)];  // 256-node grid blocks
       }
-      const SceneDesc* sd = scenes + scene;
-      const uint32_t node0 = sd->node0;
-      const size_t Nn = (size_t)sd->g.N * sd->g.N * sd->g.N;
-      grid_node((size_t)(gp - (node0 >> 6)) * 64 + lane, Nn, acc_m + node0, acc_p + node0, node_stride,
-                fixed + node0, vel + node0, sd->g, sd->sp, sd->lo_idx, sd->hi_idx, Mq, p0, p1, p2);
+      const uint64_t change = __ballot(scene != cur);
+      if (change) {
+        flush_lanes(sums, blockIdx.x % SUM_SLOTS, change & __ballot(cur != 0xffffffffu), cur, Mq, p0, p1, p2, lane);
+        if (scene != cur) {
+          cur = scene;
+          Mq = 0;
+          p0 = p1 = p2 = 0;
+        }
+      }
+      if (valid) {
+        const SceneDesc* sd = scenes + scene;
+        const uint32_t node0 = sd->node0;
+        const size_t Nn = (size_t)sd->g.N * sd->g.N * sd->g.N;
+        grid_node((size_t)(gp - (node0 >> PIECE_SHIFT)) * PIECE_NODES + sub, Nn, acc_m + node0, acc_p + node0,
+                  node_stride, fixed + node0, vel + node0, sd->g, sd->sp, sd->lo_idx, sd->hi_idx, Mq, p0, p1, p2);
+      }
     }
   } else {
     cur = 0;
     const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
     const size_t Nn = (size_t)g1.N * g1.N * g1.N;
-    for (uint32_t e = e0 + wave; e < e1; e += WAVES)
-      grid_node((size_t)list[slot(e)] * 64 + lane, Nn, acc_m, acc_p, Nn, fixed, vel, g1, sp1, lo, hi, Mq, p0, p1, p2);
+    for (uint32_t e4 = e0 + QUARTERS * wave; e4 < e1; e4 += QUARTERS * WAVES) {
+      const uint32_t e = e4 + q;
+      if (e < e1)
+        grid_node((size_t)list[slot(e)] * PIECE_NODES + sub, Nn, acc_m, acc_p, Nn, fixed, vel, g1, sp1, lo, hi, Mq,
+                  p0, p1, p2);
+    }
   }
-  Mq = wave_sum(Mq);
-  p0 = wave_sum(p0);
-  p1 = wave_sum(p1);
-  p2 = wave_sum(p2);
+  // Lanes group by scene; every group but the last flushes at once, the last
+  // goes to the workgroup combine below.
+  uint32_t wscene = 0xffffffffu;
+  long long wm = 0;
+  double wa = 0, wb = 0, wc = 0;
+  uint64_t need = __ballot(cur != 0xffffffffu);
+  while (need) {
+    const uint32_t sc = __shfl(cur, (int)__builtin_ctzll(need), 64);
+    const bool in = cur == sc;
+    const long long m = wave_sum(in ? Mq : 0ll);
+    const double a = wave_sum(in ? p0 : 0.0), b = wave_sum(in ? p1 : 0.0), c = wave_sum(in ? p2 : 0.0);
+    need &= ~__ballot(in);
+    if (need) {
+      if (lane == 0) flush_sums(sums, sc, blockIdx.x % SUM_SLOTS, m, a, b, c);
+    } else {
+      wscene = sc;
+      wm = m;
+      wa = a;
+      wb = b;
+      wc = c;
+    }
+  }
   if (lane == 0) {
-    s_scene[wave] = cur;
-    s_sum[wave][0] = __longlong_as_double(Mq);
-    s_sum[wave][1] = p0;
-    s_sum[wave][2] = p1;
-    s_sum[wave][3] = p2;
+    s_scene[wave] = wscene;
+    s_sum[wave][0] = __longlong_as_double(wm);
+    s_sum[wave][1] = wa;
+    s_sum[wave][2] = wb;
+    s_sum[wave][3] = wc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -927,7 +991,7 @@ inline unsigned blocks(size_t n, int b = BLOCK) { return (unsigned)((n + b - 1) 
 }  // namespace
 
 int particle_reduce_k() { return RED_K; }
-uint32_t grid_pieces(const GridDims& g) { return (uint32_t)(((size_t)g.N * g.N * g.N + 63) / 64); }
+uint32_t grid_pieces(const GridDims& g) { return pieces_of_nodes((size_t)g.N * g.N * g.N); }
 int particle_reduce_blocks() { return RED_BLOCKS; }
 
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g) {
@@ -954,17 +1018,17 @@ void launch_permute_state(hipStream_t s, DevState& d, uint32_t n, const uint32_t
 }
 
 void launch_compact(hipStream_t s, const DevState& d, uint32_t npieces, uint32_t nscenes, LaunchEvents ev) {
-  // A ballot per 64 flags up to 2^17 pieces (at most 2048 counter atomics);
-  // 16 flags per lane above.
+  // A ballot per 64 flags up to 2^17 pieces (at most 128 counter atomics);
+  // 16 flags per lane above (a 128^3-node grid has 2^17 pieces).
+  constexpr uint32_t CB = COMPACT_BLOCK;
   if (npieces <= (1u << 17)) {
-    hipExtLaunchKernelGGL(k_compact_pieces<1>, dim3(std::max<uint32_t>(1, (npieces + BLOCK - 1) / BLOCK)), dim3(BLOCK),
-                          0, s, ev.start, ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
+    hipExtLaunchKernelGGL(k_compact_pieces<1>, dim3(std::max<uint32_t>(1, (npieces + CB - 1) / CB)), dim3(CB), 0, s,
+                          ev.start, ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
                           nscenes * SUM_SLOTS * 4);
   } else {
     const uint32_t groups = (npieces + 15) / 16;
-    hipExtLaunchKernelGGL(k_compact_pieces<16>, dim3((groups + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, ev.start,
-                          ev.stop, 0, d.pieces, npieces, d.piece_list, d.piece_counters, d.red,
-                          nscenes * SUM_SLOTS * 4);
+    hipExtLaunchKernelGGL(k_compact_pieces<16>, dim3((groups + CB - 1) / CB), dim3(CB), 0, s, ev.start, ev.stop, 0,
+                          d.pieces, npieces, d.piece_list, d.piece_counters, d.red, nscenes * SUM_SLOTS * 4);
   }
 }
 
@@ -987,7 +1051,7 @@ void launch_grid_batch(hipStream_t s, const DevState& d, const SceneDesc* scenes
                        uint32_t node_stride, uint32_t blocks, LaunchEvents ev) {
   hipExtLaunchKernelGGL(k_grid_list<true>, dim3(blocks), dim3(BLOCK), 0, s, ev.start, ev.stop, 0, d.acc_m, d.acc_p,
                         d.fixed, d.vel, GridDims{}, StepParams{}, 0, 0, 0, 0, 0, 0, scenes, block_scene, node_stride,
-                        d.piece_list, node_stride / 64, d.piece_counters, d.red);
+                        d.piece_list, pieces_of_nodes(node_stride), d.piece_counters, d.red);
 }
 
 void launch_frame_pre(hipStream_t s, const DevState& d, uint32_t n, float* ox, const FrameDev& f, float dt) {

@@ -1,10 +1,11 @@
 """GPU parity of the grid update on grids large enough for its wide paths,
 through the C ABI.
 
-The piece compaction reads one flag per lane up to 2^17 pieces of 64 nodes
+The piece compaction reads one flag per lane up to 2^17 pieces of 16 nodes
 and 16 flags per lane (a 16-byte load and a wave scan) above; the grid update
 then runs several workgroups per CU, each over a contiguous range of the
-step's piece list. A piece listed twice or missed would break the exact
+step's piece list, a quarter-wave per piece. In a batch the four pieces of a
+wave can belong to different scenes, whose sums go to their own slots. A piece listed twice or missed would break the exact
 equality of fixed-point grid mass and particle mass, and move particles.
 
 Bars: positions against the FP64 oracle atol .003 rtol .001 (the reference's
@@ -25,7 +26,7 @@ E_REF = 1.1 / 64 * 16000
 def pieces(cells):
     # a domain of `cells` cells per axis, its lower corner on a cell boundary,
     # has cells + 5 nodes per axis (two below, three above)
-    return ((cells + 5) ** 3 + 63) // 64
+    return ((cells + 5) ** 3 + 15) // 16
 
 
 def test_large_grid_matches_oracle(gpu):
@@ -56,11 +57,11 @@ def rel_l2(a, b):
 
 
 def test_batch_over_large_total_grid(gpu):
-    # Three 192^3-cell scenes: 358k pieces in the batch, 119k in each scene
+    # Three 96^3-cell scenes: 193k pieces in the batch, 64k in each scene
     # alone, so the batch takes the 16-flag compaction and the single-scene
     # engines the one-flag path.
-    bs = scenes.batched_scenes(nscenes=3, counts=(2_000, 4_000), grids=(192,), seed=7)
-    assert 3 * pieces(192) > 1 << 17 > pieces(192)
+    bs = scenes.batched_scenes(nscenes=3, counts=(2_000, 4_000), grids=(96,), seed=7)
+    assert 3 * pieces(96) > 1 << 17 > pieces(96)
     steps = 20
     b = bs.batch(sort_every=8)
     x0 = [b.download(i)["positions"] for i in range(len(bs.scenes))]
