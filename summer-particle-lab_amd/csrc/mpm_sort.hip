@@ -1,12 +1,16 @@
 // Stable LSD radix sort of (key, value) pairs for particle binning.
 //
-// 8-bit digits, one pass per digit: a per-tile digit histogram, a device-wide
-// exclusive scan of the digit-major histogram table, then a scatter in which
-// every key's rank among equal digits of its tile is computed from wave
-// ballots (lanes with the same digit are matched bit by bit), so the order of
-// equal keys is preserved and the result equals std::stable_sort on the keys.
-// Tiles are 2048 keys (256 threads x 8 items), processed item by item in
-// ascending key position.
+// Digits of up to 9 bits: a key of b bits takes ceil(b / 9) passes of equal
+// width (25-bit cell keys: three 9-bit passes). Each pass: a per-tile digit
+// histogram (a tile-major table, one contiguous row of digit counts per
+// tile), a scan of each digit's column over the tiles plus the digits'
+// bases (three small kernels over blocks of 64 tile rows), then a scatter. Tiles are 4096 keys; in the scatter each wave ranks
+// its 1024 consecutive keys item by item (lanes with the same digit are
+// matched with one ballot per digit bit, and a per-wave running count per
+// digit in LDS gives earlier items' share), the tile's keys are placed in
+// LDS in digit order, and the tile is written out from LDS in that order, so
+// consecutive lanes write consecutive positions of a digit's run. Equal keys
+// keep their order: the result equals std::stable_sort on the keys.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -14,91 +18,166 @@
 #include <mutex>
 
 #include "mpm_internal.h"
+#include "mpm_wave.h"
 
 namespace mpm {
 namespace {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
-constexpr int RS_RADIX = 256;
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ITEMS = 16;                          // keys per lane
+constexpr int RS_WAVE_KEYS = 64 * RS_ITEMS;           // consecutive keys of one wave
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;        // 4096 keys per tile
+constexpr int RS_MAX_BITS = 9;
+constexpr int RS_RADIX = 1 << RS_MAX_BITS;            // histogram table width per tile
+constexpr int CS_TILES = 64;                          // tile rows per column-scan block
 
-// Lanes of this wave whose digit equals `d`, among lanes with `valid` set.
+// Lanes of this wave whose digit equals `d`, among lanes with `valid` set
+// (digits below 2^RS_MAX_BITS). Per digit bit: the lanes with the bit set
+// (a ballot), and each lane keeps the lanes that agree with it, one 32-bit
+// half at a time: peers & ~(ballot ^ own bit), one three-input bit operation
+// per half (truth table 0x90 over operands 0xf0, 0xcc, 0xaa).
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
-  uint64_t peers = __ballot(valid);
+  const uint64_t v = __ballot(valid);
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t m = __ballot(bit);
-    peers &= bit ? m : ~m;
+  for (int b = 0; b < RS_MAX_BITS; ++b) {
+    const uint32_t own = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // all ones where this lane's bit is set
+    const uint64_t m = __ballot(own != 0u);
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)m, own, 0x90);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(m >> 32), own, 0x90);
   }
-  return peers;
+  return ((uint64_t)hi << 32) | lo;
 }
 
+// Digit counts of one tile: hist[tile * radix + d] for d < radix. Counting
+// needs no ranks: each run of equal digits in neighbouring lanes adds its
+// length with one LDS atomic from its first lane (lanes of one atomic that
+// share a digit are summed by the LDS).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
-                                                        uint32_t ntiles, uint32_t* __restrict__ hist) {
+                                                        uint32_t radix, uint32_t ntiles, uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[RS_RADIX];
   const uint32_t tid = threadIdx.x, tile = blockIdx.x;
   const uint32_t lane = tid & 63u;
-  cnt[tid] = 0;
-  __syncthreads();
-  for (int item = 0; item < RS_ITEMS; ++item) {
-    const uint32_t idx = tile * RS_TILE + item * RS_THREADS + tid;
-    const bool valid = idx < n;
-    const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
-    const uint64_t peers = match_digit(d, valid);
-    // The highest lane of each digit group adds the group's size.
-    if (valid && (63u - (uint32_t)__clzll(peers)) == lane) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+  for (uint32_t d = tid; d < RS_RADIX; d += RS_THREADS) cnt[d] = 0;
+  uint32_t k[RS_ITEMS];
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const uint32_t idx = tile * RS_TILE + it * RS_THREADS + tid;
+    k[it] = idx < n ? keys[idx] : 0u;
   }
   __syncthreads();
-  hist[(size_t)tid * ntiles + tile] = cnt[tid];
+  const uint64_t above = ~1ull << lane;  // lanes after this one
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const bool valid = tile * RS_TILE + it * RS_THREADS + tid < n;  // a prefix of the wave's lanes
+    const uint32_t d = valid ? (k[it] >> shift) & (radix - 1u) : 0xffffffffu;
+    // lane 0 reads a value that is no digit, so it starts a run
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xfffffffeu, (int)d, DPP_WAVE_SHR1, 0xf, 0xf,
+                                                                false);
+    const bool head = valid && d != prev;
+    const uint64_t ends = __ballot(head) | ~__ballot(valid);  // a run ends before the next head or invalid lane
+    if (head) {
+      const uint64_t next = ends & above;
+      const uint32_t len = (next ? (uint32_t)__builtin_ctzll(next) : 64u) - lane;
+      atomicAdd(&cnt[d], len);
+    }
+  }
+  __syncthreads();
+  for (uint32_t d = tid; d < radix; d += RS_THREADS) hist[(size_t)tile * radix + d] = cnt[d];
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __restrict__ keys,
                                                            const uint32_t* __restrict__ vals,
                                                            uint32_t* __restrict__ keys_out,
                                                            uint32_t* __restrict__ vals_out, uint32_t n,
-                                                           uint32_t shift, uint32_t ntiles,
+                                                           uint32_t shift, uint32_t radix, uint32_t ntiles,
                                                            const uint32_t* __restrict__ offsets) {
-  __shared__ uint32_t s_base[RS_RADIX];
-  __shared__ uint32_t s_cnt[4][RS_RADIX];
+  __shared__ uint32_t s_k[RS_TILE];
+  __shared__ uint32_t s_v[RS_TILE];
+  __shared__ uint32_t s_cnt[RS_WAVES][RS_RADIX];  // running counts per wave, then the wave's offset per digit
+  __shared__ uint32_t s_start[RS_RADIX];          // first tile position of each digit
+  __shared__ uint32_t s_glob[RS_RADIX];           // output position of tile position 0 of each digit's run
+  __shared__ uint32_t s_w[RS_WAVES];
   const uint32_t tid = threadIdx.x, tile = blockIdx.x;
   const uint32_t lane = tid & 63u, wave = tid >> 6;
-  s_base[tid] = offsets[(size_t)tid * ntiles + tile];
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-  for (int item = 0; item < RS_ITEMS; ++item) {
-    __syncthreads();
+  const uint32_t mask = radix - 1u;
+  for (uint32_t d = tid; d < RS_RADIX; d += RS_THREADS) {
 #pragma unroll
-    for (int w = 0; w < 4; ++w) s_cnt[w][tid] = 0;
-    const uint32_t idx = tile * RS_TILE + item * RS_THREADS + tid;
-    const bool valid = idx < n;
-    const uint32_t k = valid ? keys[idx] : 0u;
-    const uint32_t v = valid ? vals[idx] : 0u;
-    const uint32_t d = (k >> shift) & 255u;
+    for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][d] = 0;
+    s_glob[d] = d < radix ? offsets[(size_t)tile * radix + d] : 0u;
+  }
+  const uint32_t base = tile * RS_TILE + wave * RS_WAVE_KEYS;
+  uint32_t k[RS_ITEMS], v[RS_ITEMS], r[RS_ITEMS];
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const uint32_t idx = base + it * 64 + lane;
+    k[it] = idx < n ? keys[idx] : 0u;
+    v[it] = idx < n ? vals[idx] : 0u;
+  }
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+  uint32_t* cnt = s_cnt[wave];
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const bool valid = base + it * 64 + lane < n;
+    const uint32_t d = (k[it] >> shift) & mask;
     const uint64_t peers = match_digit(d, valid);
-    const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    const uint32_t size = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid && rank == size - 1u) s_cnt[wave][d] = size;
-    __syncthreads();
-    {
-      const uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
-      const uint32_t b = s_base[tid];
-      s_cnt[0][tid] = b;
-      s_cnt[1][tid] = b + c0;
-      s_cnt[2][tid] = b + c0 + c1;
-      s_cnt[3][tid] = b + c0 + c1 + c2;
-      s_base[tid] = b + c0 + c1 + c2 + c3;
+    const uint32_t before = (uint32_t)__popcll(peers & lt);
+    const uint32_t run = valid ? cnt[d] : 0u;
+    r[it] = run + before;
+    // Only this wave touches its counts, and its LDS operations complete in
+    // order: the next item's reads see this update.
+    if (valid && before + 1u == (uint32_t)__popcll(peers)) cnt[d] = run + before + 1u;
+  }
+  __syncthreads();
+  // Per digit: the waves' offsets within the digit's run, and the tile's
+  // count; thread t holds digits 2t and 2t + 1.
+  uint32_t tot[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t d = 2u * tid + h;
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      const uint32_t c = s_cnt[w][d];
+      s_cnt[w][d] = run;
+      run += c;
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t pos = s_cnt[wave][d] + rank;
-      keys_out[pos] = k;
-      vals_out[pos] = v;
+    tot[h] = run;
+  }
+  uint32_t inc = tot[0] + tot[1];
+  const uint32_t pair = inc;
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const uint32_t t = __shfl_up(inc, dd, 64);
+    if ((int)lane >= dd) inc += t;
+  }
+  if (lane == 63) s_w[wave] = inc;
+  __syncthreads();
+  uint32_t ex = inc - pair;
+  for (uint32_t w = 0; w < wave; ++w) ex += s_w[w];
+  s_start[2 * tid] = ex;
+  s_start[2 * tid + 1] = ex + tot[0];
+  s_glob[2 * tid] -= ex;
+  s_glob[2 * tid + 1] -= ex + tot[0];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    if (base + it * 64 + lane < n) {
+      const uint32_t d = (k[it] >> shift) & mask;
+      const uint32_t pos = s_start[d] + cnt[d] + r[it];
+      s_k[pos] = k[it];
+      s_v[pos] = v[it];
     }
+  }
+  __syncthreads();
+  const uint32_t tile_n = min((uint32_t)RS_TILE, n - tile * RS_TILE);
+  for (uint32_t j = tid; j < tile_n; j += RS_THREADS) {
+    const uint32_t kk = s_k[j];
+    const uint32_t pos = s_glob[(kk >> shift) & mask] + j;
+    keys_out[pos] = kk;
+    vals_out[pos] = s_v[j];
   }
 }
 
@@ -113,77 +192,58 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
   return v;
 }
 
-// Exclusive scan across a block of SCAN_THREADS threads; returns the block
-// total in `total`.
-__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* s_w, uint32_t& total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t inc = wave_scan(v);
+// Column scan of the tile-major count table, thread d for digit d: the sum
+// of each block of CS_TILES rows ...
+__global__ __launch_bounds__(RS_RADIX) void k_col_reduce(const uint32_t* __restrict__ counts, uint32_t ntiles,
+                                                         uint32_t radix, uint32_t* __restrict__ sums) {
+  const uint32_t d = threadIdx.x, b = blockIdx.x;
+  if (d >= radix) return;
+  const uint32_t t0 = b * CS_TILES, t1 = min(ntiles, t0 + CS_TILES);
+  uint32_t s = 0;
+#pragma unroll 8
+  for (uint32_t t = t0; t < t1; ++t) s += counts[(size_t)t * radix + d];
+  sums[(size_t)b * radix + d] = s;
+}
+
+// ... each block's offset within its digit's column and each digit's total
+// (one wave per digit, 64 blocks per round) ...
+__global__ __launch_bounds__(RS_THREADS) void k_col_scan(uint32_t* __restrict__ sums, uint32_t nblk, uint32_t radix,
+                                                         uint32_t* __restrict__ totals) {
+  const uint32_t d = blockIdx.x * RS_WAVES + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (d >= radix) return;  // the same for the whole wave
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nblk; b0 += 64) {
+    const uint32_t b = b0 + lane;
+    const uint32_t x = b < nblk ? sums[(size_t)b * radix + d] : 0u;
+    const uint32_t inc = wave_scan(x);
+    if (b < nblk) sums[(size_t)b * radix + d] = carry + inc - x;
+    carry += __shfl(inc, 63, 64);
+  }
+  if (lane == 0) totals[d] = carry;
+}
+
+// ... and each tile's start, in place of its counts: the totals of the
+// smaller digits, plus the block's offset in the digit's column, plus the
+// tile rows before it in the block.
+__global__ __launch_bounds__(RS_RADIX) void k_col_down(uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t radix,
+                                                       const uint32_t* __restrict__ sums,
+                                                       const uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_w[RS_RADIX / 64];
+  const uint32_t d = threadIdx.x, b = blockIdx.x, lane = d & 63u, wave = d >> 6;
+  const uint32_t tot = d < radix ? totals[d] : 0u;
+  const uint32_t inc = wave_scan(tot);
   if (lane == 63) s_w[wave] = inc;
   __syncthreads();
-  uint32_t before = 0, sum = 0;
-  for (int w = 0; w < SCAN_THREADS / 64; ++w) {
-    if (w < wave) before += s_w[w];
-    sum += s_w[w];
-  }
-  __syncthreads();
-  total = sum;
-  return before + inc - v;
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t total,
-                                                              uint32_t* __restrict__ partials) {
-  __shared__ uint32_t s_w[SCAN_THREADS / 64];
-  const uint32_t base = blockIdx.x * SCAN_BLOCK;
-  uint32_t sum = 0;
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    const uint32_t idx = base + i * SCAN_THREADS + threadIdx.x;
-    if (idx < total) sum += in[idx];
-  }
-  uint32_t t;
-  block_exclusive(sum, s_w, t);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-}
-
-// Single-block exclusive scan of the per-block sums.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(uint32_t* __restrict__ partials, uint32_t np) {
-  __shared__ uint32_t s_w[SCAN_THREADS / 64];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < np; base += SCAN_THREADS) {
-    const uint32_t idx = base + threadIdx.x;
-    const uint32_t v = idx < np ? partials[idx] : 0u;
-    uint32_t t;
-    const uint32_t ex = block_exclusive(v, s_w, t);
-    if (idx < np) partials[idx] = carry + ex;
-    carry += t;
-  }
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(uint32_t* __restrict__ data, uint32_t total,
-                                                            const uint32_t* __restrict__ partials) {
-  __shared__ uint32_t s_data[SCAN_BLOCK];
-  __shared__ uint32_t s_w[SCAN_THREADS / 64];
-  const uint32_t base = blockIdx.x * SCAN_BLOCK;
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    const uint32_t idx = base + i * SCAN_THREADS + threadIdx.x;
-    s_data[i * SCAN_THREADS + threadIdx.x] = idx < total ? data[idx] : 0u;
-  }
-  __syncthreads();
-  uint32_t local[SCAN_ITEMS];
-  uint32_t sum = 0;
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    local[i] = s_data[threadIdx.x * SCAN_ITEMS + i];
-    sum += local[i];
-  }
-  uint32_t t;
-  uint32_t run = block_exclusive(sum, s_w, t) + partials[blockIdx.x];
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    s_data[threadIdx.x * SCAN_ITEMS + i] = run;
-    run += local[i];
-  }
-  __syncthreads();
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    const uint32_t idx = base + i * SCAN_THREADS + threadIdx.x;
-    if (idx < total) data[idx] = s_data[i * SCAN_THREADS + threadIdx.x];
+  uint32_t base = inc - tot;
+  for (uint32_t w = 0; w < wave; ++w) base += s_w[w];
+  if (d >= radix) return;
+  const uint32_t t0 = b * CS_TILES, t1 = min(ntiles, t0 + CS_TILES);
+  uint32_t run = base + sums[(size_t)b * radix + d];
+#pragma unroll 8
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t x = counts[(size_t)t * radix + d];
+    counts[(size_t)t * radix + d] = run;
+    run += x;
   }
 }
 
@@ -195,28 +255,30 @@ size_t radix_hist_words(uint32_t n) {
 }
 
 size_t radix_partial_words(uint32_t n) {
-  return (radix_hist_words(n) + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
+  const size_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  return ((ntiles + CS_TILES - 1) / CS_TILES + 1) * RS_RADIX;
 }
 
 void radix_sort_pairs(hipStream_t s, uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt,
                       uint32_t* hist, uint32_t* partials, uint32_t n, uint32_t key_bits) {
-  if (n == 0) return;
+  if (n == 0 || key_bits == 0) return;
+  const uint32_t passes = (key_bits + RS_MAX_BITS - 1) / RS_MAX_BITS;
+  const uint32_t width = (key_bits + passes - 1) / passes;
+  const uint32_t radix = 1u << width;
   const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-  const uint32_t total = ntiles * RS_RADIX;
-  const uint32_t nblocks = (total + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, keys, n, shift, ntiles, hist);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nblocks), dim3(SCAN_THREADS), 0, s, hist, total, partials);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_THREADS), 0, s, partials, nblocks);
-    hipLaunchKernelGGL(k_scan_down, dim3(nblocks), dim3(SCAN_THREADS), 0, s, hist, total, partials);
+  const uint32_t nblk = (ntiles + CS_TILES - 1) / CS_TILES;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t shift = p * width;
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, keys, n, shift, radix, ntiles, hist);
+    hipLaunchKernelGGL(k_col_reduce, dim3(nblk), dim3(RS_RADIX), 0, s, hist, ntiles, radix, partials);
+    hipLaunchKernelGGL(k_col_scan, dim3((radix + RS_WAVES - 1) / RS_WAVES), dim3(RS_THREADS), 0, s, partials, nblk,
+                       radix, partials + (size_t)nblk * radix);
+    hipLaunchKernelGGL(k_col_down, dim3(nblk), dim3(RS_RADIX), 0, s, hist, ntiles, radix, partials,
+                       partials + (size_t)nblk * radix);
     hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, keys, vals, keys_alt, vals_alt, n, shift,
-                       ntiles, hist);
-    uint32_t* t = keys;
-    keys = keys_alt;
-    keys_alt = t;
-    t = vals;
-    vals = vals_alt;
-    vals_alt = t;
+                       radix, ntiles, hist);
+    std::swap(keys, keys_alt);
+    std::swap(vals, vals_alt);
   }
 }
 
@@ -237,7 +299,7 @@ using namespace mpm;
 constexpr size_t NA_MAX = MPM_SORT_NOALLOC_MAX;
 constexpr size_t NA_TILES = (NA_MAX + RS_TILE - 1) / RS_TILE;
 constexpr size_t NA_HIST = NA_TILES * RS_RADIX;
-constexpr size_t NA_PARTIALS = (NA_HIST + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
+constexpr size_t NA_PARTIALS = ((NA_TILES + CS_TILES - 1) / CS_TILES + 1) * RS_RADIX;
 constexpr size_t NA_WORDS = 2 * NA_MAX + NA_HIST + NA_PARTIALS;
 
 __device__ __align__(256) uint32_t scratch[NA_WORDS];
