@@ -188,30 +188,33 @@ BatchArgs bucket_args(const mpm_batch* b, int k) {
   return BatchArgs{b->d_scenes, bk.chunks, bk.nchunks, b->total_nodes};
 }
 
-// Bin, sort and permute every scene's particles into cell order. With
-// `state_only` set the re-bin sits between a grid update and the fused step
-// that follows it, whose G2P rebuilds v, C and the stress: only the planes
-// that carry over a step move.
-void rebin(mpm_batch* b, bool state_only = false) {
+// Bin and sort every scene's particles into cell order: d.vals then holds,
+// for each new slot, the slot the particle lives in now. A re-bin at the
+// start of a call also permutes every plane; a re-bin between two fused
+// steps sorts only, and the fused step after it re-gathers the particles.
+void sort_keys(mpm_batch* b) {
   launch_bin_keys_batch(b->stream, b->d, bin_args(b));
   radix_sort_pairs(b->stream, b->d.keys, b->d.vals, b->d.keys_alt, b->d.vals_alt, b->d.hist, b->d.partials,
                    b->total_n, b->key_bits);
-  if (state_only)
-    launch_permute_state(b->stream, b->d, b->total_n, b->d.vals);
-  else
-    launch_permute(b->stream, b->d, b->total_n, b->d.vals);
   b->steps_since_sort = 0;
 }
 
+void rebin(mpm_batch* b) {
+  sort_keys(b);
+  launch_permute(b->stream, b->d, b->total_n, b->d.vals);
+}
+
+// Mode 3 (the fused step re-gathering after a sort) writes the alternate
+// buffers in every launch and swaps them after the last.
 void particles(mpm_batch* b, int mode) {
   if (b->n_all_chunks) {
     launch_fused_batch(b->stream, b->d, BatchArgs{b->d_scenes, b->d_all_chunks, b->n_all_chunks, b->total_nodes}, mode,
                        b->all_fb);
-    return;
+  } else {
+    for (int k = 0; k < NBUCKETS; ++k)
+      if (b->bucket[k].nchunks) launch_fused_batch(b->stream, b->d, bucket_args(b, k), mode, b->bucket[k].fb);
   }
-  for (int k = 0; k < NBUCKETS; ++k)
-    if (b->bucket[k].nchunks)
-      launch_fused_batch(b->stream, b->d, bucket_args(b, k), mode, b->bucket[k].fb);
+  if (mode == 3) swap_particle_buffers(b->d);
 }
 
 void grid(mpm_batch* b) {
@@ -231,8 +234,12 @@ void begin_steps(mpm_batch* b) {
 
 void fused_step(mpm_batch* b) {
   grid(b);
-  if (++b->steps_since_sort >= b->sort_every) rebin(b, true);
-  particles(b, 1);
+  if (++b->steps_since_sort >= b->sort_every) {
+    sort_keys(b);
+    particles(b, 3);
+  } else {
+    particles(b, 1);
+  }
 }
 
 void end_steps(mpm_batch* b) {
@@ -246,8 +253,8 @@ void end_steps(mpm_batch* b) {
 // exit, and the next re-bin is due P steps later.
 void period(mpm_batch* b, int P) {
   grid(b);
-  rebin(b, true);
-  particles(b, 1);
+  sort_keys(b);
+  particles(b, 3);
   for (int s = 1; s < P; ++s) {
     grid(b);
     particles(b, 1);

@@ -267,12 +267,9 @@ namespace {
 
 bool supported_model(int m) { return m >= 0 && m <= 11; }
 
-// Bin, sort and permute the resident particles into cell order. With
-// `state_only` set the re-bin sits between a grid update and the fused step
-// that follows it, whose G2P rebuilds v, C and the stress: only the planes
-// that carry over a step move.
-int rebin(mpm_engine* e, bool state_only = false) {
-  if (e->n == 0) return MPM_OK;
+// Bin and sort the resident particles into cell order: d.vals then holds,
+// for each new slot, the slot the particle lives in now.
+int sort_keys(mpm_engine* e) {
   {
     Timed t(e, "bin_keys");
     launch_bin_keys(e->stream, e->d, e->n, e->g);
@@ -282,15 +279,23 @@ int rebin(mpm_engine* e, bool state_only = false) {
     radix_sort_pairs(e->stream, e->d.keys, e->d.vals, e->d.keys_alt, e->d.vals_alt, e->d.hist, e->d.partials, e->n,
                      key_bits(e->g.N));
   }
-  {
-    Timed t(e, "permute");
-    if (state_only)
-      launch_permute_state(e->stream, e->d, e->n, e->d.vals);
-    else
-      launch_permute(e->stream, e->d, e->n, e->d.vals);
-  }
   HIP_OK(e, hipGetLastError());
   e->steps_since_sort = 0;
+  return MPM_OK;
+}
+
+// Bin, sort and permute the resident particles into cell order (a re-bin at
+// the start of a call: every plane moves). A re-bin between two fused steps
+// sorts only, and the fused step after it re-gathers the particles.
+int rebin(mpm_engine* e) {
+  if (e->n == 0) return MPM_OK;
+  int rc = sort_keys(e);
+  if (rc) return rc;
+  {
+    Timed t(e, "permute");
+    launch_permute(e->stream, e->d, e->n, e->d.vals);
+  }
+  HIP_OK(e, hipGetLastError());
   return MPM_OK;
 }
 
@@ -561,13 +566,18 @@ int mpm_step(mpm_engine* e, double dt, int32_t nsteps) {
       launch_fused(e->stream, e->d, e->n, e->g, sp, 2, t.ev());
     } else {
       // A re-bin due here sorts the positions this grid was built from,
-      // which the next G2P gathers at.
+      // which the next G2P gathers at; that fused step reads the particles
+      // through the new order and writes them in it.
       if (e->steps_since_sort >= e->cfg.sort_every) {
-        int rc = rebin(e, true);
+        int rc = sort_keys(e);
         if (rc) return rc;
+        Timed t(e, "g2p2g_rebin", true);
+        launch_fused(e->stream, e->d, e->n, e->g, sp, 3, t.ev());
+        swap_particle_buffers(e->d);
+      } else {
+        Timed t(e, "g2p2g", true);
+        launch_fused(e->stream, e->d, e->n, e->g, sp, 1, t.ev());
       }
-      Timed t(e, "g2p2g", true);
-      launch_fused(e->stream, e->d, e->n, e->g, sp, 1, t.ev());
     }
   }
   HIP_OK(e, hipGetLastError());

@@ -283,7 +283,7 @@ __device__ __forceinline__ bool state_invalid(const M3& F, float ep, const float
 // The particle work of one chunk of FBK cell-sorted particles in slots
 // [first, min(first + FBK, end)). Grid pointers are those of the chunk's
 // scene; momentum planes are `pstride` floats apart.
-template <int MODE, bool SUR, int FBK, bool EXT, int CH>
+template <int MODE, bool SUR, int FBK, bool EXT, int CH, bool RG = false>
 __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap,
                                             uint32_t first, uint32_t end, const DevMaterial* __restrict__ mats,
                                             const GridDims& g, const StepParams& sp, size_t pstride,
@@ -293,7 +293,9 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
                                             unsigned long long* __restrict__ sur_counts, unsigned char* s_raw,
                                             int* s_box, uint32_t* __restrict__ status,
                                             const float4* __restrict__ deltas = nullptr,
-                                            const int32_t* __restrict__ delta_slot = nullptr) {
+                                            const int32_t* __restrict__ delta_slot = nullptr,
+                                            const Regather& rg = Regather{}) {
+  static_assert(!RG || MODE == 1, "a re-gather is a fused step");
   using S = Shape<FBK, SUR, CH>;
   constexpr int TR_ROWS = Scatter<CH>::ROWS;
   constexpr int TR_ENTRIES = Scatter<CH>::ENTRIES;
@@ -304,7 +306,11 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
   const uint32_t lane = tid & 63u;
   const uint32_t i = first + tid;
   const bool valid = i < end;
-  const uint32_t ic = valid ? i : first;
+  // The slot this lane reads (after a re-bin sort, through the new order)
+  // and the buffers it writes.
+  const uint32_t ic = RG ? rg.order[valid ? i : first] : (valid ? i : first);
+  float* const Po = RG ? rg.planes_out : P;
+  uint32_t* const meta_o = RG ? rg.meta_out : meta;
   const int N = g.N;
   const float H = g.H, invH = g.invH, dt = sp.dt;
 
@@ -547,16 +553,16 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     }
 
     if (valid && !pinned) {
-      st(P, cap, P_X, i, x[0]);
-      st(P, cap, P_Y, i, x[1]);
-      st(P, cap, P_Z, i, x[2]);
+      st(Po, cap, P_X, i, x[0]);
+      st(Po, cap, P_Y, i, x[1]);
+      st(Po, cap, P_Z, i, x[2]);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) st(P, cap, P_F0 + k, i, F[k]);
-      if (model >= 2) st(P, cap, P_EP, i, ep);
-      if (model == 2 || model == 8 || model == 9 || failed) st(P, cap, P_JP, i, jp);
-      if (model == 3) st(P, cap, P_DIL, i, dil);
-      if (EXT && mat.fr_on) st(P, cap, P_DMG, i, dmg);
-      if (new_flags != flags) meta[i] = (mt & 255u) | (new_flags << 8);
+      for (int k = 0; k < 9; ++k) st(Po, cap, P_F0 + k, i, F[k]);
+      if (model >= 2) st(Po, cap, P_EP, i, ep);
+      if (model == 2 || model == 8 || model == 9 || failed) st(Po, cap, P_JP, i, jp);
+      if (model == 3) st(Po, cap, P_DIL, i, dil);
+      if (EXT && mat.fr_on) st(Po, cap, P_DMG, i, dmg);
+      if (RG || new_flags != flags) meta_o[i] = (mt & 255u) | (new_flags << 8);
       if (MODE == 2) {
         st(P, cap, P_VX, i, v[0]);
         st(P, cap, P_VY, i, v[1]);
@@ -566,6 +572,29 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
         if (model >= 2) {
 #pragma unroll
           for (int k = 0; k < 6; ++k) st(P, cap, P_T0 + k, i, tau[k]);
+        }
+      }
+    }
+    if constexpr (RG) {
+      // The rest of what carries over a step, which this step does not
+      // write: mass, volume, temperature, the plastic strain of the elastic
+      // models, the history planes not updated here; a pinned particle
+      // keeps its stored state and moves every plane.
+      if (valid) {
+        rg.ids_out[i] = rg.ids[ic];
+        if (pinned) {
+          for (int k = 0; k < NPLANES; ++k) st(Po, cap, k, i, ld(P, cap, k, ic));
+          meta_o[i] = mt;
+        } else {
+          st(Po, cap, P_MASS, i, m);
+          st(Po, cap, P_VOL, i, vol);
+          st(Po, cap, P_TEMP, i, hist ? temp0 : ld(P, cap, P_TEMP, ic));
+          if (!(model >= 2)) st(Po, cap, P_EP, i, hist ? ep0 : ld(P, cap, P_EP, ic));
+          if (rg.history) {
+            if (!(model == 2 || model == 8 || model == 9 || failed)) st(Po, cap, P_JP, i, ld(P, cap, P_JP, ic));
+            if (model != 3) st(Po, cap, P_DIL, i, ld(P, cap, P_DIL, ic));
+            if (!(EXT && mat.fr_on)) st(Po, cap, P_DMG, i, ld(P, cap, P_DMG, ic));
+          }
         }
       }
     }
@@ -589,7 +618,8 @@ __device__ __forceinline__ void fused_chunk(float* __restrict__ P, uint32_t* __r
     // The stable step of the state this step starts from.
     bool slow = false;
     if (valid) {
-      const float jpc = (model == 2 || model == 8 || model == 9) ? ld(P, cap, P_JP, ic) : 1.f;
+      // Jp as this step left it (written above for these models)
+      const float jpc = (model == 2 || model == 8 || model == 9) ? ld(Po, cap, P_JP, RG ? i : ic) : 1.f;
       slow = cfl_exceeded(F, Cm, v, m, vol, mat, model, jpc, sp);
     }
     if (__any(slow) && lane == 0u) atomicOr(status, STATUS_CFL);
@@ -973,18 +1003,21 @@ __device__ __forceinline__ uint32_t xcd_chunk() {
 // scene's fused step 13% faster). The surrogate variant's wave buffer holds
 // the network's activations (47.2 KB per workgroup) and its MFMA operands
 // need about 160 VGPRs: three waves.
+// MODE 3 is the fused step of mode 1 re-gathering the particles after a
+// re-bin sort (its own kernel, so that profiles keep it apart).
 template <int MODE, bool SUR, bool EXT, int CH>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(SUR ? 3 : 4, SUR ? 3 : 4))) void k_fused(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, uint32_t n,
     const DevMaterial* __restrict__ mats, GridDims g, StepParams sp, const float4* __restrict__ vel,
     unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p, uint8_t* __restrict__ fixed,
     uint8_t* __restrict__ pieces, const MlpDev* __restrict__ mlp, unsigned long long* __restrict__ sur_counts,
-    uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot) {
+    uint32_t* __restrict__ status, const float4* __restrict__ deltas, const int32_t* __restrict__ delta_slot,
+    Regather rg) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FB, SUR, CH>::LDS_BYTES];
   __shared__ int s_box[2 * 7 * Shape<FB, SUR, CH>::WAVES];
-  fused_chunk<MODE, SUR, FB, EXT, CH>(P, meta, cap, xcd_chunk() * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel,
-                                      acc_m, acc_p, fixed, pieces, mlp, sur_counts, s_raw, s_box, status, deltas,
-                                      delta_slot);
+  fused_chunk<MODE == 3 ? 1 : MODE, SUR, FB, EXT, CH, MODE == 3>(
+      P, meta, cap, xcd_chunk() * FB, n, mats, g, sp, (size_t)g.N * g.N * g.N, vel, acc_m, acc_p, fixed, pieces, mlp,
+      sur_counts, s_raw, s_box, status, deltas, delta_slot, rg);
 }
 
 // Many scenes in one launch: one workgroup per chunk of the bucket's chunk
@@ -996,7 +1029,7 @@ template <int MODE, int FBK, bool EXT, int CH>
 __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_fused_batch(
     float* __restrict__ P, uint32_t* __restrict__ meta, uint32_t cap, const DevMaterial* __restrict__ mats,
     BatchArgs b, const float4* __restrict__ vel, unsigned long long* __restrict__ acc_m, float* __restrict__ acc_p,
-    uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status) {
+    uint8_t* __restrict__ fixed, uint8_t* __restrict__ pieces, uint32_t* __restrict__ status, Regather rg) {
   __shared__ __align__(16) unsigned char s_raw[Shape<FBK, false, CH>::LDS_BYTES];
   __shared__ int s_box[2 * 7 * Shape<FBK, false, CH>::WAVES];
   const uint32_t t = xcd_chunk();
@@ -1011,14 +1044,33 @@ __global__ __launch_bounds__(FBK) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const StepParams sp = to_scalar(sd->sp);
   const uint32_t node0 = __builtin_amdgcn_readfirstlane(sd->node0);
   const uint32_t end = __builtin_amdgcn_readfirstlane(sd->p0 + sd->n);
-  fused_chunk<MODE, false, FBK, EXT, CH>(P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0,
-                                         acc_m + node0, acc_p + node0, fixed + node0, pieces + (node0 >> PIECE_SHIFT), nullptr,
-                                         nullptr, s_raw, s_box, status + ch.x);
+  fused_chunk<MODE == 3 ? 1 : MODE, false, FBK, EXT, CH, MODE == 3>(
+      P, meta, cap, ch.y, end, mats, g, sp, b.node_stride, vel + node0, acc_m + node0, acc_p + node0, fixed + node0,
+      pieces + (node0 >> PIECE_SHIFT), nullptr, nullptr, s_raw, s_box, status + ch.x, nullptr, nullptr, rg);
 }
 
 inline unsigned fblocks(uint32_t n) { return (n + FB - 1) / FB; }
 
+Regather regather_of(const DevState& d, int mode) {
+  Regather r;
+  if (mode == 3) {
+    r.order = d.vals;
+    r.planes_out = d.planes_alt;
+    r.meta_out = d.meta_alt;
+    r.ids = d.ids;
+    r.ids_out = d.ids_alt;
+    r.history = d.history ? 1u : 0u;
+  }
+  return r;
+}
+
 }  // namespace
+
+void swap_particle_buffers(DevState& d) {
+  std::swap(d.planes, d.planes_alt);
+  std::swap(d.ids, d.ids_alt);
+  std::swap(d.meta, d.meta_alt);
+}
 
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode,
                   LaunchEvents ev) {
@@ -1027,7 +1079,7 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
 #define MPM_FUSED(M, S, X, C)                                                                                      \
   hipExtLaunchKernelGGL((k_fused<M, S, X, C>), grid, block, 0, s, ev.start, ev.stop, 0, d.planes, d.meta, d.cap, n, \
                         d.materials, g, p, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.mlp, d.sur_counts,         \
-                        d.status, mode == 2 ? d.deltas : nullptr, d.delta_slot)
+                        d.status, mode == 2 ? d.deltas : nullptr, d.delta_slot, rg)
 #define MPM_FUSED_C(M, S, X)  \
   if (d.narrow)               \
     MPM_FUSED(M, S, X, 8);    \
@@ -1040,12 +1092,17 @@ void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& 
     MPM_FUSED_C(M, S, false);    \
   }
   const bool sur = d.mlp != nullptr;
+  const Regather rg = regather_of(d, mode);
   if (mode == 0) {
     MPM_FUSED_X(0, false);
   } else if (mode == 1 && sur) {
     MPM_FUSED_X(1, true);
   } else if (mode == 1) {
     MPM_FUSED_X(1, false);
+  } else if (mode == 3 && sur) {
+    MPM_FUSED_X(3, true);
+  } else if (mode == 3) {
+    MPM_FUSED_X(3, false);
   } else if (sur) {
     MPM_FUSED_X(2, true);
   } else {
@@ -1061,7 +1118,7 @@ void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, in
   const dim3 grid(b.nchunks);
 #define MPM_FUSED_B(M, K, X, C)                                                                                     \
   hipExtLaunchKernelGGL((k_fused_batch<M, K, X, C>), grid, dim3(K), 0, s, ev.start, ev.stop, 0, d.planes, d.meta,   \
-                        d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.status)
+                        d.cap, d.materials, b, d.vel, d.acc_m, d.acc_p, d.fixed, d.pieces, d.status, rg)
 #define MPM_FUSED_K(M, X, C)     \
   if (fb == 64)                  \
     MPM_FUSED_B(M, 64, X, C);    \
@@ -1081,10 +1138,13 @@ void launch_fused_batch(hipStream_t s, const DevState& d, const BatchArgs& b, in
   } else {                       \
     MPM_FUSED_C(M, false);       \
   }
+  const Regather rg = regather_of(d, mode);
   if (mode == 0) {
     MPM_FUSED_M(0);
   } else if (mode == 1) {
     MPM_FUSED_M(1);
+  } else if (mode == 3) {
+    MPM_FUSED_M(3);
   } else {
     MPM_FUSED_M(2);
   }

@@ -160,6 +160,18 @@ struct FrameDev {
   float vol_unit;  // particle volume unit of the spacing rule (1/64 for the reference's V)
 };
 
+// Source of a fused step that re-gathers the particles after a re-bin sort:
+// slot i of the step is slot order[i] of the current buffers, and the step
+// writes slot i of the alternate buffers.
+struct Regather {
+  const uint32_t* order = nullptr;
+  float* planes_out = nullptr;
+  uint32_t* meta_out = nullptr;
+  const uint32_t* ids = nullptr;
+  uint32_t* ids_out = nullptr;
+  uint32_t history = 0;  // Jp, dilation and damage carry over a step
+};
+
 // Optional events that a launch stamps with its own start and end (profiling
 // without barrier packets between launches).
 struct LaunchEvents {
@@ -169,9 +181,6 @@ struct LaunchEvents {
 // Kernel launchers (mpm_kernels.hip)
 void launch_bin_keys(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g);
 void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
-// The planes that carry over a step only (every plane of a pinned particle):
-// for a re-bin between two fused steps, whose G2P rebuilds v, C and stress.
-void launch_permute_state(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order);
 // Grid update: launch_compact lists the pieces P2G flagged (and clears the
 // grid sums), launch_grid / launch_grid_batch update the listed pieces with
 // `blocks` workgroups, each taking a contiguous range of the list.
@@ -179,10 +188,16 @@ void launch_compact(hipStream_t s, const DevState& d, uint32_t npieces, uint32_t
 void launch_grid(hipStream_t s, const DevState& d, const GridDims& g, const StepParams& p, const int lo_idx[3],
                  const int hi_idx[3], uint32_t blocks, LaunchEvents ev = {});
 // Fused particle kernel (mpm_fused.hip): mode 0 P2G from the stored state,
-// 1 G2P of this step + P2G of the next, 2 G2P writing the full state. With
-// d.mlp set, the G2P modes take the surrogate variant.
+// 1 G2P of this step + P2G of the next, 2 G2P writing the full state, 3 the
+// fused step of mode 1 right after a re-bin sort: it reads slot d.vals[i] of
+// the current buffers and writes slot i of the alternate ones (every plane
+// that carries over a step, all planes of a pinned particle, ids and meta),
+// in place of a separate permute; swap_particle_buffers then makes the
+// alternate buffers current. With d.mlp set, the G2P modes take the
+// surrogate variant.
 void launch_fused(hipStream_t s, const DevState& d, uint32_t n, const GridDims& g, const StepParams& p, int mode,
                   LaunchEvents ev = {});
+void swap_particle_buffers(DevState& d);
 uint32_t grid_pieces(const GridDims& g);
 // Bytes of the piece flag array for `npieces` pieces: padded with zeros to a
 // multiple of 16, since the compaction reads 16 flags per 16-byte load.

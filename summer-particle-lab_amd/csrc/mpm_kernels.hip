@@ -93,42 +93,6 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const float* __restrict__ P, 
   meta_out[i] = meta[src];
 }
 
-// Re-bin between two fused steps: the G2P that follows recomputes v and C
-// from the grid and the step's stress from F, so only the planes that carry
-// over a step move -- x, F, m, V, T, ep and, when some material keeps them,
-// Jp, dilation and damage. A pinned particle skips G2P and keeps its stored
-// v, C and stress, so it moves every plane.
-__global__ __launch_bounds__(BLOCK) void k_permute_state(const float* __restrict__ P, float* __restrict__ Q,
-                                                         const uint32_t* __restrict__ ids,
-                                                         uint32_t* __restrict__ ids_out,
-                                                         const uint32_t* __restrict__ meta,
-                                                         uint32_t* __restrict__ meta_out, uint32_t cap, uint32_t n,
-                                                         const uint32_t* __restrict__ order, int history) {
-  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t src = order[i];
-  const uint32_t mt = meta[src];
-  auto move = [&](int k) { Q[(size_t)k * cap + i] = P[(size_t)k * cap + src]; };
-#pragma unroll
-  for (int k = P_X; k <= P_Z; ++k) move(k);
-#pragma unroll
-  for (int k = P_F0; k < P_F0 + 9; ++k) move(k);
-#pragma unroll
-  for (int k = P_MASS; k <= P_EP; ++k) move(k);
-  if (history) {
-#pragma unroll
-    for (int k = P_JP; k <= P_DMG; ++k) move(k);
-  }
-  if ((mt >> 8) & FLAG_PINNED) {
-    for (int k = P_VX; k < P_F0; ++k) move(k);
-    for (int k = P_T0; k < P_T0 + 6; ++k) move(k);
-    if (!history)
-      for (int k = P_JP; k <= P_DMG; ++k) move(k);
-  }
-  ids_out[i] = ids[src];
-  meta_out[i] = mt;
-}
-
 // Kirchhoff stress of this step before the damage factor (6-vector), for
 // the particle's effective model.
 __device__ __forceinline__ void step_stress(const M3& F, int model, const DevMaterial& m, uint32_t flags,
@@ -1003,15 +967,6 @@ void launch_permute(hipStream_t s, DevState& d, uint32_t n, const uint32_t* orde
   if (!n) return;
   hipLaunchKernelGGL(k_permute, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.planes_alt, d.ids, d.ids_alt, d.meta,
                      d.meta_alt, d.cap, n, order);
-  std::swap(d.planes, d.planes_alt);
-  std::swap(d.ids, d.ids_alt);
-  std::swap(d.meta, d.meta_alt);
-}
-
-void launch_permute_state(hipStream_t s, DevState& d, uint32_t n, const uint32_t* order) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_permute_state, dim3(blocks(n)), dim3(BLOCK), 0, s, d.planes, d.planes_alt, d.ids, d.ids_alt,
-                     d.meta, d.meta_alt, d.cap, n, order, d.history ? 1 : 0);
   std::swap(d.planes, d.planes_alt);
   std::swap(d.ids, d.ids_alt);
   std::swap(d.meta, d.meta_alt);
