@@ -23,6 +23,7 @@ struct DevMaterial {
   int32_t model;
   float E, mu, lambda, G, lr;
   float th_Tref, th_Thot, th_hotRatio, th_A, th_H, th_rate, th_Q, th_n, th_stress, th_kelvin;
+  float th_inv_span;  // 1 / (Thot - Tref)
   float pl_A, pl_B, pl_n, pl_C, pl_rate0, pl_Tref, pl_Tm, pl_m;
   float sn_ys;     // snow stretch yield strain
   float sd_alpha;  // sand: sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
@@ -201,8 +202,22 @@ __device__ __forceinline__ void compose(const float U[9], const float s[3], cons
       F[i * 3 + j] = U[i * 3] * s[0] * V[j * 3] + U[i * 3 + 1] * s[1] * V[j * 3 + 1] + U[i * 3 + 2] * s[2] * V[j * 3 + 2];
 }
 
+// theta = (T - Tref) / (Thot - Tref) as the IEEE quotient rounds it, from
+// the correctly rounded reciprocal of the span (set on the host): the
+// product corrected once by its exact residual (Markstein) -- a multiply and
+// two fused multiply-adds in place of a division. Checked bit for bit against
+// the division for every float T in [-300, 3000] over six spans and 2e8
+// random normal pairs. A span that is not positive (no reciprocal) divides.
 __device__ __forceinline__ float thermal_strength(float ep, float T, const DevMaterial& m) {
-  const float theta = fminf(1.f, fmaxf(0.f, (T - m.th_Tref) / (m.th_Thot - m.th_Tref)));
+  const float a = T - m.th_Tref, span = m.th_Thot - m.th_Tref;
+  float q;
+  if (m.th_inv_span != 0.f) {
+    const float q0 = a * m.th_inv_span;
+    q = fmaf(fmaf(-q0, span, a), m.th_inv_span, q0);
+  } else {
+    q = a / span;
+  }
+  const float theta = fminf(1.f, fmaxf(0.f, q));
   return (m.th_A + m.th_H * ep) * (1.f - (1.f - m.th_hotRatio) * theta);
 }
 
@@ -457,11 +472,14 @@ __device__ __forceinline__ bool elastic_return(const M3& F, int model, float ep,
   const float tr = eps[0] + eps[1] + eps[2];
   const float mean = tr * (1.f / 3.f);
   const float d0 = eps[0] - mean, d1 = eps[1] - mean, d2 = eps[2] - mean;
-  const float length = sqrtf(d0 * d0 + d1 * d1 + d2 * d2 + 2.f * (eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5]));
-  const float J = fmaxf(1e-6f, expf(tr));
-  const float q = 2.4494897427831781f * m.G * length / J;
+  // Only the yield test reads length and J, so the hardware square root and
+  // exponential (a few ulp) serve, and q = sqrt(6) G length / J <= y is
+  // tested as sqrt(6) G length <= y J (J > 0). A NaN fails either way.
+  const float length =
+      __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2 + 2.f * (eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5]));
+  const float J = fmaxf(1e-6f, __expf(tr));
   const float y = model == 10 ? thermal_strength(ep, T, m) : flow_stress(model, ep, 0.f, T, m);
-  if (!(q <= y)) return false;
+  if (!(2.4494897427831781f * m.G * length <= y * J)) return false;
   hencky_stress(eps, m, tau);
   return true;
 }

@@ -155,6 +155,9 @@ DevMaterial to_dev_material(const mpm_material& m) {
   d.th_n = m.th_n;
   d.th_stress = m.th_stress;
   d.th_kelvin = m.th_kelvin_offset;
+  // the correctly rounded reciprocal of the span (0 where there is none)
+  const float span = m.th_Thot - m.th_Tref, inv_span = 1.f / span;
+  d.th_inv_span = (span > 0.f && std::isfinite(inv_span) && inv_span >= 1e-30f) ? inv_span : 0.f;
   d.pl_A = m.pl_A;
   d.pl_B = m.pl_B;
   d.pl_n = m.pl_n;

@@ -212,7 +212,8 @@ __device__ __forceinline__ T to_scalar(const T& v) {
 // Box coordinates (lx, ly, lz) of entry p of an ex x ey x ez box, without
 // integer division. For 0 <= p < 2^21 and any divisor d >= 1,
 // (p + 1/2) / d lies at least 0.5 / d inside [q, q + 1). The float product
-// (p + 1/2) * (1/d), with the reciprocal within 1 ulp and one rounding, is
+// (p + 1/2) * (1/d), with the reciprocal within 1 ulp (the hardware
+// reciprocal) and one rounding, is
 // off by a relative 2^-22 at most, that is by less than
 // (p + 1/2) / d * 2^-22 < 0.5 / d, so its truncation is the exact quotient q.
 // Boxes hold at most a few thousand entries.
@@ -220,8 +221,8 @@ struct BoxIndex {
   int ex, exy;
   float inv_ex, inv_exy;
   __device__ __forceinline__ BoxIndex(int ex_, int ey_) : ex(ex_), exy(ex_ * ey_) {
-    inv_ex = 1.f / (float)ex;
-    inv_exy = 1.f / (float)exy;
+    inv_ex = __builtin_amdgcn_rcpf((float)ex);
+    inv_exy = __builtin_amdgcn_rcpf((float)exy);
   }
   __device__ __forceinline__ void operator()(int p, int& lx, int& ly, int& lz) const {
     lz = (int)(((float)p + 0.5f) * inv_exy);
