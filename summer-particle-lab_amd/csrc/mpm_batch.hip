@@ -5,9 +5,9 @@
 // Layout: one particle pool (slots of scene s are [p0_s, p0_s + n_s), p0 the
 // prefix sum of the counts) and one concatenated grid (scene s's dense N_s^3
 // block starts at node0_s, a multiple of one 256-node grid block). One radix
-// sort per re-bin orders the whole pool by key0_s + blocked cell key, so
-// every scene stays in its own slot range, cell-sorted in 4x4x4 blocks, ties
-// by original id.
+// sort per re-bin, segmented by scene (its tiles never straddle two scenes),
+// orders each scene's slot range by the scene's blocked cell key: cells in
+// 4x4x4 blocks, ties by original id.
 //
 // Per step: one particle launch per size bucket (scenes grouped by grid
 // edge; each bucket has its own particles-per-workgroup, chosen by the
@@ -75,6 +75,11 @@ struct mpm_batch {
   uint32_t n_all_chunks = 0;
   int all_fb = 0;
   uint32_t* d_block_scene = nullptr;
+  // The re-bin sort runs per scene (each scene's slots hold its particles):
+  // sort tiles that never straddle two scenes, and the scenes' tile ranges.
+  uint2* d_sort_tiles = nullptr;
+  uint2* d_sort_segs = nullptr;
+  SortSegments sort_seg{};
   uint32_t total_n = 0, total_nodes = 0, total_blocks = 0, key_bits = 0;
   uint32_t grid_blocks = 1;
   int sort_every = 16;
@@ -195,7 +200,7 @@ BatchArgs bucket_args(const mpm_batch* b, int k) {
 void sort_keys(mpm_batch* b) {
   launch_bin_keys_batch(b->stream, b->d, bin_args(b));
   radix_sort_pairs(b->stream, b->d.keys, b->d.vals, b->d.keys_alt, b->d.vals_alt, b->d.hist, b->d.partials,
-                   b->total_n, b->key_bits);
+                   b->total_n, b->key_bits, &b->sort_seg);
   b->steps_since_sort = 0;
 }
 
@@ -315,7 +320,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   b->mass_q.assign(nscenes, 0);
   b->occupied.assign(nscenes, 0);
   b->uploaded.assign(nscenes, 0);
-  uint64_t p0 = 0, node0 = 0, block0 = 0, key0 = 0;
+  uint64_t p0 = 0, node0 = 0, block0 = 0, keys_max = 1;
   for (int32_t s = 0; s < nscenes; ++s) {
     SceneDesc& sd = b->sc[s];
     const int rc = grid_geometry(&scenes[s], sd.g, sd.lo_idx, sd.hi_idx);
@@ -328,8 +333,7 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
     sd.n = counts[s];
     sd.node0 = (uint32_t)node0;
     sd.block0 = (uint32_t)block0;
-    sd.key0 = (uint32_t)key0;
-    key0 += blocked_key_count(sd.g.N);
+    keys_max = std::max<uint64_t>(keys_max, blocked_key_count(sd.g.N));
     const uint64_t Nn = (uint64_t)sd.g.N * sd.g.N * sd.g.N;
     const uint64_t blocks = (Nn + GRID_BLOCK_NODES - 1) / GRID_BLOCK_NODES;
     p0 += counts[s];
@@ -337,15 +341,17 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
     block0 += blocks;
     b->cell[s] = scenes[s].cell_size;
   }
-  if (p0 > (1ull << 31) || node0 > (1ull << 31) || key0 > (1ull << 32)) {
+  if (p0 > (1ull << 31) || node0 > (1ull << 31)) {
     delete b;
     return MPM_ERR_CAPACITY;
   }
   b->total_n = (uint32_t)p0;
   b->total_nodes = (uint32_t)node0;
   b->total_blocks = (uint32_t)block0;
-  b->key_bits = 8;
-  while (b->key_bits < 32 && (1ull << b->key_bits) < key0) b->key_bits += 8;
+  // Keys are scene-local blocked cell keys, sorted per scene: the bits of
+  // the largest scene's key range.
+  b->key_bits = 1;
+  while (b->key_bits < 32 && (1ull << b->key_bits) < keys_max) ++b->key_bits;
   if (opt) {
     if (opt->sort_every >= 1) b->sort_every = opt->sort_every;
     b->use_graph = opt->use_graph != 0;
@@ -376,7 +382,25 @@ int mpm_batch_create(const mpm_config* scenes, const uint32_t* counts, int32_t n
   alloc((void**)&d.keys_alt, sizeof(uint32_t) * cap);
   alloc((void**)&d.vals, sizeof(uint32_t) * cap);
   alloc((void**)&d.vals_alt, sizeof(uint32_t) * cap);
-  const size_t hist_words = radix_hist_words(cap), part_words = radix_partial_words(cap);
+  // the sort's tiles: at most one partial tile per scene beyond cap / tile
+  const size_t hist_words = radix_hist_words((uint32_t)std::min<uint64_t>(
+                                (uint64_t)cap + (uint64_t)nscenes * RADIX_TILE, 0xffffffffull)),
+               part_words = radix_partial_words(cap);
+  std::vector<uint2> sort_tiles, sort_segs;
+  {
+    std::vector<uint32_t> sizes(nscenes);
+    for (int32_t s = 0; s < nscenes; ++s) sizes[s] = b->sc[s].n;
+    radix_segment_tables(sizes.data(), (uint32_t)nscenes, sort_tiles, sort_segs);
+  }
+  alloc((void**)&b->d_sort_tiles, sizeof(uint2) * sort_tiles.size());
+  alloc((void**)&b->d_sort_segs, sizeof(uint2) * sort_segs.size());
+  if (st == hipSuccess && !sort_tiles.empty())
+    st = hipMemcpyAsync(b->d_sort_tiles, sort_tiles.data(), sizeof(uint2) * sort_tiles.size(), hipMemcpyHostToDevice,
+                        b->stream);
+  if (st == hipSuccess && !sort_segs.empty())
+    st = hipMemcpyAsync(b->d_sort_segs, sort_segs.data(), sizeof(uint2) * sort_segs.size(), hipMemcpyHostToDevice,
+                        b->stream);
+  b->sort_seg = SortSegments{b->d_sort_tiles, b->d_sort_segs, (uint32_t)sort_tiles.size(), (uint32_t)sort_segs.size()};
   alloc((void**)&d.hist, sizeof(uint32_t) * hist_words);
   alloc((void**)&d.partials, sizeof(uint32_t) * part_words);
   alloc((void**)&d.red, sizeof(double) * 4 * SUM_SLOTS * (size_t)nscenes);
@@ -435,7 +459,7 @@ int mpm_batch_destroy(mpm_batch* b) {
   void* ptrs[] = {d.planes, d.planes_alt, d.ids,   d.ids_alt,  d.meta,     d.meta_alt, d.acc_m, d.acc_p,
                   d.vel,    d.fixed,      d.pieces, d.keys,    d.keys_alt, d.vals,     d.vals_alt, d.hist,
                   d.partials, d.red,      d.materials, b->d_scenes, b->d_bin_chunks, b->d_block_scene,
-                  d.piece_list, d.piece_counters, d.status};
+                  d.piece_list, d.piece_counters, d.status, b->d_sort_tiles, b->d_sort_segs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& bk : b->bucket)

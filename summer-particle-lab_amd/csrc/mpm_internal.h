@@ -128,17 +128,14 @@ constexpr int SUR_SLOTS = 64;
 constexpr int SUM_SLOTS = 16;  // slots per scene of the grid sums (spreads the atomics)
 
 // One scene of a batch: its grid and step parameters, its particle slot
-// range [p0, p0 + n), the first node / grid block of its grid in the
-// concatenated grid arrays (node0 is a multiple of 256, one grid block) and
-// the first of its sort keys (key0 + blocked_cell_key; scenes take
-// consecutive ranges of blocked_key_count(N) keys).
+// range [p0, p0 + n) and the first node / grid block of its grid in the
+// concatenated grid arrays (node0 is a multiple of 256, one grid block).
 struct SceneDesc {
   GridDims g;
   StepParams sp;
   uint32_t p0, n;
   uint32_t node0, block0;
   int lo_idx[3], hi_idx[3];
-  uint32_t key0;
 };
 
 // Particle-kernel arguments of a batched launch: the scene table, the chunk
@@ -261,7 +258,20 @@ inline bool returns_in_g2p(const mpm_material& m) { return m.model >= 2 || m.fra
 // the sorted pairs are in (keys, vals) (the buffers are swapped as needed).
 size_t radix_hist_words(uint32_t n);
 size_t radix_partial_words(uint32_t n);
+// A sort of consecutive segments, each stably sorted on its own (the pairs of
+// a segment stay in its range): tiles of at most RADIX_TILE keys that never
+// straddle two segments, {first key, key count} each, and per segment
+// {first tile, tile count} (segments with tiles only).
+constexpr uint32_t RADIX_TILE = 4096;
+struct SortSegments {
+  const uint2* tiles = nullptr;
+  const uint2* segs = nullptr;
+  uint32_t ntiles = 0, nsegs = 0;
+};
+// The tiles and segments of segments of the given sizes, in order.
+void radix_segment_tables(const uint32_t* sizes, uint32_t count, std::vector<uint2>& tiles, std::vector<uint2>& segs);
 void radix_sort_pairs(hipStream_t s, uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt,
-                      uint32_t* hist, uint32_t* partials, uint32_t n, uint32_t key_bits);
+                      uint32_t* hist, uint32_t* partials, uint32_t n, uint32_t key_bits,
+                      const SortSegments* seg = nullptr);
 
 }  // namespace mpm

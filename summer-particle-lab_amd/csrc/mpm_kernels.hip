@@ -55,9 +55,10 @@ __global__ __launch_bounds__(BLOCK) void k_bin_keys(const float* __restrict__ P,
   vals[id] = i;
 }
 
-// Batched keys: chunk -> scene; the key is the scene's first key plus the
-// base cell's blocked key, so one sort orders scenes by slot range and the
-// cells of each scene in 4x4x4 blocks, as the single-scene binning does.
+// Batched keys: chunk -> scene; the key is the base cell's blocked key in
+// its scene. The sort runs per scene (a scene's slots and ids are its own
+// range), so each scene's cells come in 4x4x4 blocks, as the single-scene
+// binning does.
 __global__ __launch_bounds__(BLOCK) void k_bin_keys_batch(const float* __restrict__ P,
                                                           const uint32_t* __restrict__ ids, uint32_t cap,
                                                           BatchArgs b, uint32_t chunk_size,
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_keys_batch(const float* __restric
   const int by = base_of(plane(P, cap, P_Y, i), g.O[1], g.invH, g.N, t);
   const int bz = base_of(plane(P, cap, P_Z, i), g.O[2], g.invH, g.N, t);
   const uint32_t id = ids[i];
-  keys[id] = sd->key0 + blocked_cell_key(bx, by, bz, g.N);
+  keys[id] = blocked_cell_key(bx, by, bz, g.N);  // sorted per scene
   vals[id] = i;
 }
 

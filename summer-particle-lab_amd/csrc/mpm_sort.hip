@@ -28,6 +28,7 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_ITEMS = 16;                          // keys per lane
 constexpr int RS_WAVE_KEYS = 64 * RS_ITEMS;           // consecutive keys of one wave
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;        // 4096 keys per tile
+static_assert(RS_TILE == (int)RADIX_TILE, "tile size");
 constexpr int RS_MAX_BITS = 9;
 constexpr int RS_RADIX = 1 << RS_MAX_BITS;            // histogram table width per tile
 constexpr int CS_TILES = 64;                          // tile rows per column-scan block
@@ -54,23 +55,32 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 // needs no ranks: each run of equal digits in neighbouring lanes adds its
 // length with one LDS atomic from its first lane (lanes of one atomic that
 // share a digit are summed by the LDS).
+// Keys [first, first + count) of tile t: consecutive tiles of RS_TILE keys,
+// or the tile table of a segmented sort.
+__device__ __forceinline__ uint2 tile_range(const uint2* __restrict__ tiles, uint32_t t, uint32_t n) {
+  if (tiles) return make_uint2(__builtin_amdgcn_readfirstlane(tiles[t].x), __builtin_amdgcn_readfirstlane(tiles[t].y));
+  return make_uint2(t * RS_TILE, min((uint32_t)RS_TILE, n - t * RS_TILE));
+}
+
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
-                                                        uint32_t radix, uint32_t ntiles, uint32_t* __restrict__ hist) {
+                                                        uint32_t radix, const uint2* __restrict__ tiles,
+                                                        uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[RS_RADIX];
   const uint32_t tid = threadIdx.x, tile = blockIdx.x;
   const uint32_t lane = tid & 63u;
+  const uint2 tr = tile_range(tiles, tile, n);
   for (uint32_t d = tid; d < RS_RADIX; d += RS_THREADS) cnt[d] = 0;
   uint32_t k[RS_ITEMS];
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    const uint32_t idx = tile * RS_TILE + it * RS_THREADS + tid;
-    k[it] = idx < n ? keys[idx] : 0u;
+    const uint32_t j = it * RS_THREADS + tid;
+    k[it] = j < tr.y ? keys[tr.x + j] : 0u;
   }
   __syncthreads();
   const uint64_t above = ~1ull << lane;  // lanes after this one
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    const bool valid = tile * RS_TILE + it * RS_THREADS + tid < n;  // a prefix of the wave's lanes
+    const bool valid = it * RS_THREADS + tid < tr.y;  // a prefix of the wave's lanes
     const uint32_t d = valid ? (k[it] >> shift) & (radix - 1u) : 0xffffffffu;
     // lane 0 reads a value that is no digit, so it starts a run
     const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xfffffffeu, (int)d, DPP_WAVE_SHR1, 0xf, 0xf,
@@ -91,7 +101,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
                                                            const uint32_t* __restrict__ vals,
                                                            uint32_t* __restrict__ keys_out,
                                                            uint32_t* __restrict__ vals_out, uint32_t n,
-                                                           uint32_t shift, uint32_t radix, uint32_t ntiles,
+                                                           uint32_t shift, uint32_t radix,
+                                                           const uint2* __restrict__ tiles,
                                                            const uint32_t* __restrict__ offsets) {
   __shared__ uint32_t s_k[RS_TILE];
   __shared__ uint32_t s_v[RS_TILE];
@@ -102,25 +113,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
   const uint32_t tid = threadIdx.x, tile = blockIdx.x;
   const uint32_t lane = tid & 63u, wave = tid >> 6;
   const uint32_t mask = radix - 1u;
+  const uint2 tr = tile_range(tiles, tile, n);
   for (uint32_t d = tid; d < RS_RADIX; d += RS_THREADS) {
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][d] = 0;
     s_glob[d] = d < radix ? offsets[(size_t)tile * radix + d] : 0u;
   }
-  const uint32_t base = tile * RS_TILE + wave * RS_WAVE_KEYS;
+  // this wave's keys: tile positions [wave * RS_WAVE_KEYS, ...) below tr.y
+  const uint32_t base = wave * RS_WAVE_KEYS;
   uint32_t k[RS_ITEMS], v[RS_ITEMS], r[RS_ITEMS];
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    const uint32_t idx = base + it * 64 + lane;
-    k[it] = idx < n ? keys[idx] : 0u;
-    v[it] = idx < n ? vals[idx] : 0u;
+    const uint32_t j = base + it * 64 + lane;
+    k[it] = j < tr.y ? keys[tr.x + j] : 0u;
+    v[it] = j < tr.y ? vals[tr.x + j] : 0u;
   }
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   uint32_t* cnt = s_cnt[wave];
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    const bool valid = base + it * 64 + lane < n;
+    const bool valid = base + it * 64 + lane < tr.y;
     const uint32_t d = (k[it] >> shift) & mask;
     const uint64_t peers = match_digit(d, valid);
     const uint32_t before = (uint32_t)__popcll(peers & lt);
@@ -164,7 +177,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    if (base + it * 64 + lane < n) {
+    if (base + it * 64 + lane < tr.y) {
       const uint32_t d = (k[it] >> shift) & mask;
       const uint32_t pos = s_start[d] + cnt[d] + r[it];
       s_k[pos] = k[it];
@@ -172,8 +185,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
     }
   }
   __syncthreads();
-  const uint32_t tile_n = min((uint32_t)RS_TILE, n - tile * RS_TILE);
-  for (uint32_t j = tid; j < tile_n; j += RS_THREADS) {
+  for (uint32_t j = tid; j < tr.y; j += RS_THREADS) {
     const uint32_t kk = s_k[j];
     const uint32_t pos = s_glob[(kk >> shift) & mask] + j;
     keys_out[pos] = kk;
@@ -247,7 +259,48 @@ __global__ __launch_bounds__(RS_RADIX) void k_col_down(uint32_t* __restrict__ co
   }
 }
 
+// Segmented sort: each tile's start per digit within its segment, in place
+// of its counts (one workgroup per segment, thread d for digit d): the
+// segment's first key position, plus the segment's counts of the smaller
+// digits, plus the segment's tiles before this one.
+__global__ __launch_bounds__(RS_RADIX) void k_seg_scan(uint32_t* __restrict__ counts, uint32_t radix,
+                                                       const uint2* __restrict__ segs,
+                                                       const uint2* __restrict__ tiles) {
+  __shared__ uint32_t s_w[RS_RADIX / 64];
+  const uint32_t d = threadIdx.x, lane = d & 63u, wave = d >> 6;
+  const uint2 sg = segs[blockIdx.x];  // {first tile, tile count}
+  uint32_t tot = 0;
+  if (d < radix) {
+    for (uint32_t t = sg.x; t < sg.x + sg.y; ++t) tot += counts[(size_t)t * radix + d];
+  }
+  const uint32_t inc = wave_scan(tot);
+  if (lane == 63) s_w[wave] = inc;
+  __syncthreads();
+  uint32_t run = inc - tot + tiles[sg.x].x;
+  for (uint32_t w = 0; w < wave; ++w) run += s_w[w];
+  if (d >= radix) return;
+  for (uint32_t t = sg.x; t < sg.x + sg.y; ++t) {
+    const uint32_t x = counts[(size_t)t * radix + d];
+    counts[(size_t)t * radix + d] = run;
+    run += x;
+  }
+}
+
 }  // namespace
+
+void radix_segment_tables(const uint32_t* sizes, uint32_t count, std::vector<uint2>& tiles, std::vector<uint2>& segs) {
+  tiles.clear();
+  segs.clear();
+  uint32_t first = 0;
+  for (uint32_t s = 0; s < count; ++s) {
+    if (sizes[s]) {
+      const uint32_t t0 = (uint32_t)tiles.size();
+      for (uint32_t k = 0; k < sizes[s]; k += RS_TILE) tiles.push_back(make_uint2(first + k, std::min<uint32_t>(RS_TILE, sizes[s] - k)));
+      segs.push_back(make_uint2(t0, (uint32_t)tiles.size() - t0));
+    }
+    first += sizes[s];
+  }
+}
 
 size_t radix_hist_words(uint32_t n) {
   const size_t ntiles = (n + RS_TILE - 1) / RS_TILE;
@@ -260,23 +313,29 @@ size_t radix_partial_words(uint32_t n) {
 }
 
 void radix_sort_pairs(hipStream_t s, uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt,
-                      uint32_t* hist, uint32_t* partials, uint32_t n, uint32_t key_bits) {
+                      uint32_t* hist, uint32_t* partials, uint32_t n, uint32_t key_bits, const SortSegments* seg) {
   if (n == 0 || key_bits == 0) return;
+  if (seg && !seg->ntiles) return;
   const uint32_t passes = (key_bits + RS_MAX_BITS - 1) / RS_MAX_BITS;
   const uint32_t width = (key_bits + passes - 1) / passes;
   const uint32_t radix = 1u << width;
-  const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  const uint32_t ntiles = seg ? seg->ntiles : (n + RS_TILE - 1) / RS_TILE;
+  const uint2* tiles = seg ? seg->tiles : nullptr;
   const uint32_t nblk = (ntiles + CS_TILES - 1) / CS_TILES;
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t shift = p * width;
-    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, keys, n, shift, radix, ntiles, hist);
-    hipLaunchKernelGGL(k_col_reduce, dim3(nblk), dim3(RS_RADIX), 0, s, hist, ntiles, radix, partials);
-    hipLaunchKernelGGL(k_col_scan, dim3((radix + RS_WAVES - 1) / RS_WAVES), dim3(RS_THREADS), 0, s, partials, nblk,
-                       radix, partials + (size_t)nblk * radix);
-    hipLaunchKernelGGL(k_col_down, dim3(nblk), dim3(RS_RADIX), 0, s, hist, ntiles, radix, partials,
-                       partials + (size_t)nblk * radix);
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, keys, n, shift, radix, tiles, hist);
+    if (seg) {
+      hipLaunchKernelGGL(k_seg_scan, dim3(seg->nsegs), dim3(RS_RADIX), 0, s, hist, radix, seg->segs, seg->tiles);
+    } else {
+      hipLaunchKernelGGL(k_col_reduce, dim3(nblk), dim3(RS_RADIX), 0, s, hist, ntiles, radix, partials);
+      hipLaunchKernelGGL(k_col_scan, dim3((radix + RS_WAVES - 1) / RS_WAVES), dim3(RS_THREADS), 0, s, partials, nblk,
+                         radix, partials + (size_t)nblk * radix);
+      hipLaunchKernelGGL(k_col_down, dim3(nblk), dim3(RS_RADIX), 0, s, hist, ntiles, radix, partials,
+                         partials + (size_t)nblk * radix);
+    }
     hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, keys, vals, keys_alt, vals_alt, n, shift,
-                       radix, ntiles, hist);
+                       radix, tiles, hist);
     std::swap(keys, keys_alt);
     std::swap(vals, vals_alt);
   }
