@@ -387,12 +387,13 @@ int mpm_profile(mpm_engine* e, int32_t enable, int32_t reset);
 /* Copy up to max kernel timing records; *n receives the record count. */
 int mpm_kernel_times(mpm_engine* e, mpm_kernel_time* out, int32_t max, int32_t* n);
 
-/* Bin the resident particles and return, in device slot order, the cell key
- * of every slot before sorting (keys_in), the sorted keys (keys_out) and the
- * sorted permutation of original particle ids (perm). Any pointer may be
- * NULL. Keys are (bz*N + by)*N + bx of the base cell
- * b = clamp(floor((x - O)/h - 1/2), 0, N - 3) on FP32 positions
- * (src/three/mpm.js:28). */
+/* Bin the resident particles as a re-bin does and return the cell key of
+ * every particle indexed by its original id (keys_in), the sorted keys
+ * (keys_out) and the original ids in sorted order (perm); nothing moves.
+ * Any pointer may be NULL. A key is the blocked index of the base cell
+ * b = clamp(floor((x - O)/h - 1/2), 0, N - 3) of the FP32 position
+ * (src/three/mpm.js:28): cells by 4x4x4 block (blocks in z, y, x order),
+ * x fastest inside a block; the stable sort breaks ties by id. */
 int mpm_debug_bin(mpm_engine* e, uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm);
 
 /* Stable LSD radix sort of arbitrary 32-bit keys on the device; values are
@@ -601,6 +602,12 @@ int mpm_batch_step(mpm_batch* b, int32_t nsteps);
 int mpm_batch_synchronize(mpm_batch* b);
 /* FP64 conservation report of one scene (as mpm_conservation_report). */
 int mpm_batch_conservation(mpm_batch* b, int32_t scene, mpm_conservation* out);
+/* The batch's binning, as mpm_debug_bin for the whole pool (total_n
+ * entries, ids and slots of scene s in [p0_s, p0_s + n_s)): keys are the
+ * scene-local blocked cell keys of each scene's grid, and the sort runs per
+ * scene, so each scene's ids stay in its range. A test hook with no
+ * counterpart in the reference. */
+int mpm_batch_debug_bin(mpm_batch* b, uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm);
 int mpm_batch_get_tuning(const mpm_batch* b, mpm_batch_tuning* out);
 int mpm_batch_set_tuning(mpm_batch* b, const mpm_batch_tuning* t);
 /* Autotuner: times particles per workgroup {64, 128, 256} per bucket on

@@ -627,6 +627,26 @@ int mpm_batch_step(mpm_batch* b, int32_t nsteps) {
   return MPM_OK;
 }
 
+int mpm_batch_debug_bin(mpm_batch* b, uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm) {
+  if (!b) return MPM_ERR_INVALID_ARGUMENT;
+  const uint32_t n = b->total_n;
+  if (!n) return MPM_OK;
+  launch_bin_keys_batch(b->stream, b->d, bin_args(b));
+  if (keys_in) BHIP(b, hipMemcpyAsync(keys_in, b->d.keys, 4ull * n, hipMemcpyDeviceToHost, b->stream));
+  std::vector<uint32_t> ids(n), order(n);
+  BHIP(b, hipMemcpyAsync(ids.data(), b->d.ids, 4ull * n, hipMemcpyDeviceToHost, b->stream));
+  // local copies: the batch keeps its buffer assignment (graphs track it)
+  uint32_t *k = b->d.keys, *v = b->d.vals, *k_alt = b->d.keys_alt, *v_alt = b->d.vals_alt;
+  radix_sort_pairs(b->stream, k, v, k_alt, v_alt, b->d.hist, b->d.partials, n, b->key_bits, &b->sort_seg);
+  if (keys_out) BHIP(b, hipMemcpyAsync(keys_out, k, 4ull * n, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipMemcpyAsync(order.data(), v, 4ull * n, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (perm)
+    for (uint32_t s = 0; s < n; ++s) perm[s] = ids[order[s]];
+  return MPM_OK;
+}
+
 int mpm_batch_synchronize(mpm_batch* b) {
   if (!b) return MPM_ERR_INVALID_ARGUMENT;
   BHIP(b, hipMemcpyAsync(b->status_host, b->d.status, sizeof(uint32_t) * (size_t)b->S, hipMemcpyDeviceToHost,

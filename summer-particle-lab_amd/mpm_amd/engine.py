@@ -433,6 +433,16 @@ class Batch:
         self.counts = cnt
         self.scenes = len(configs)
 
+    def debug_bin(self):
+        """The batch's binning (mpm_batch_debug_bin) over the whole pool:
+        keys by particle id, sorted keys, ids in sorted order."""
+        n = int(self.counts.sum())
+        kin, kout, perm = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        P = C.POINTER(C.c_uint32)
+        self._check(self._lib.mpm_batch_debug_bin(self.h, kin.ctypes.data_as(P), kout.ctypes.data_as(P),
+                                                  perm.ctypes.data_as(P)))
+        return kin, kout, perm
+
     def close(self):
         if self.h:
             self._lib.mpm_batch_destroy(self.h)

@@ -219,6 +219,7 @@ SIGNATURES = {
     "mpm_batch_step": (C.c_int, [_P, C.c_int32]),
     "mpm_batch_synchronize": (C.c_int, [_P]),
     "mpm_batch_conservation": (C.c_int, [_P, C.c_int32, C.POINTER(MpmConservation)]),
+    "mpm_batch_debug_bin": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "mpm_batch_get_tuning": (C.c_int, [_P, C.POINTER(MpmBatchTuning)]),
     "mpm_batch_set_tuning": (C.c_int, [_P, C.POINTER(MpmBatchTuning)]),
     "mpm_batch_tune": (C.c_int, [_P, C.c_int32, C.POINTER(MpmBatchTuning)]),

@@ -10,7 +10,11 @@ Bars:
   * every particles-per-workgroup choice (64, 128, 256, equal or mixed
     across size buckets) gives the same trajectories: same bar;
   * fixed-point grid mass equals the particle mass of every scene exactly;
-  * the autotuner returns a valid configuration and the batch keeps stepping.
+  * the autotuner returns a valid configuration and the batch keeps stepping;
+  * the batch's binning (sorted per scene, scene-local keys) is bit-exact: each
+    scene's keys equal the host's blocked keys of its FP32 positions, and its
+    permutation equals a stable argsort of them (ties by id), before and after
+    steps that re-bin.
 """
 import numpy as np
 import pytest
@@ -121,3 +125,37 @@ def test_batch_rejects_bad_use(gpu, small):
     with pytest.raises(L.MpmError) as ei:
         fresh.step(1)
     assert ei.value.code == L.MPM_ERR_NOT_READY
+
+
+def test_batch_binning_is_a_stable_sort_per_scene(gpu):
+    from mpm_amd import Material
+    from mpm_amd.engine import Batch, make_config
+    from test_gpu_engine import cpu_keys
+    rng = np.random.default_rng(23)
+    # (particles, cells per axis): 3, 1, 2 and 1 sort tiles of 4096 keys
+    specs = [(9_000, 32), (100, 64), (5_000, 96), (4_096, 48)]
+    cfgs, xs = [], []
+    for n, cells in specs:
+        h = 1.0 / cells
+        cfgs.append(make_config((0, 0, 0), (1, 1, 1), h, n))
+        # clustered around a few centres, so cells hold many ties
+        centres = rng.uniform(0.2, 0.8, (6, 3))
+        xs.append(np.clip(centres[rng.integers(0, 6, n)] + rng.normal(0, 2 * h, (n, 3)), 0.05, 0.95))
+    b = Batch(cfgs, [n for n, _ in specs], sort_every=2)
+    b.set_materials([Material.from_E_nu(0, 1e5, 0.3)])
+    for i, (n, _) in enumerate(specs):
+        b.upload(i, xs[i], rng.normal(0, 0.3, (n, 3)), np.full(n, 1e-3), np.full(n, 1e-6))
+    b.set_dt([1e-4] * len(specs))
+    for _ in range(2):
+        kin, kout, perm = b.debug_bin()
+        p0 = 0
+        for i, (n, cells) in enumerate(specs):
+            want = cpu_keys(b.download(i)["positions"], (0, 0, 0), (1, 1, 1), 1.0 / cells)
+            order = np.argsort(want, kind="stable")
+            assert np.array_equal(kin[p0:p0 + n], want), i
+            assert np.array_equal(perm[p0:p0 + n], p0 + order), i
+            assert np.array_equal(kout[p0:p0 + n], want[order]), i
+            p0 += n
+        b.step(7)  # re-bins inside the call move particles between slots
+        b.synchronize()
+    b.close()
