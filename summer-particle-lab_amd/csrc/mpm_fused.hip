@@ -197,18 +197,6 @@ __device__ __forceinline__ bool box_inside(const int lo[3], const int hi[3], int
 // Exponent e of a finite x with |x| < 2^e (0 for x = 0).
 __device__ __forceinline__ int frexp_exponent(float x) { return __builtin_amdgcn_frexp_expf(x); }
 
-// A wave-uniform value moved into scalar registers, four bytes at a time.
-template <typename T>
-__device__ __forceinline__ T to_scalar(const T& v) {
-  static_assert(sizeof(T) % 4 == 0, "whole words only");
-  T out;
-  const int* src = reinterpret_cast<const int*>(&v);
-  int* dst = reinterpret_cast<int*>(&out);
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
-  return out;
-}
-
 // Box coordinates (lx, ly, lz) of entry p of an ex x ey x ez box, without
 // integer division. For 0 <= p < 2^21 and any divisor d >= 1,
 // (p + 1/2) / d lies at least 0.5 / d inside [q, q + 1). The float product

@@ -415,7 +415,27 @@ __global__ __launch_bounds__(BLOCK) void k_grid_list(unsigned long long* __restr
           p0 = p1 = p2 = 0;
         }
       }
-      if (valid) {
+      // Valid lanes are a prefix of the wave, so lane 0's scene is theirs
+      // when they share one (the common case): its descriptor then comes
+      // through scalar registers instead of a vector load per lane.
+      const uint32_t s0 = __builtin_amdgcn_readfirstlane(scene);
+      if (__all(!valid || scene == s0)) {
+        if (valid) {
+          const SceneDesc* sd = scenes + s0;
+          const GridDims g = to_scalar(sd->g);
+          const StepParams sp = to_scalar(sd->sp);
+          const uint32_t node0 = __builtin_amdgcn_readfirstlane(sd->node0);
+          int lo[3], hi[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            lo[k] = __builtin_amdgcn_readfirstlane(sd->lo_idx[k]);
+            hi[k] = __builtin_amdgcn_readfirstlane(sd->hi_idx[k]);
+          }
+          const size_t Nn = (size_t)g.N * g.N * g.N;
+          grid_node((size_t)(gp - (node0 >> PIECE_SHIFT)) * PIECE_NODES + sub, Nn, acc_m + node0, acc_p + node0,
+                    node_stride, fixed + node0, vel + node0, g, sp, lo, hi, Mq, p0, p1, p2);
+        }
+      } else if (valid) {
         const SceneDesc* sd = scenes + scene;
         const uint32_t node0 = sd->node0;
         const size_t Nn = (size_t)sd->g.N * sd->g.N * sd->g.N;

@@ -60,6 +60,18 @@ __device__ __forceinline__ T wave_reduce(T x, T identity, Fold fold) {
   return lane63(x);
 }
 
+// A wave-uniform value moved into scalar registers, four bytes at a time.
+template <typename T>
+__device__ __forceinline__ T to_scalar(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "whole words only");
+  T out;
+  const int* src = reinterpret_cast<const int*>(&v);
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return out;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
   return wave_reduce(x, T(0), [](T a, T b) { return a + b; });
