@@ -83,7 +83,7 @@ struct DevState {
   float* acc_p;               // node momentum 3 planes [3*N^3]
   float4* vel;                // node velocity (xyz) and mass (w) [N^3]
   uint8_t* fixed;             // [N^3]
-  uint8_t* pieces;            // active 64-node pieces [ceil(N^3/64)]
+  uint8_t* pieces;            // flags of the active 16-node pieces [ceil(N^3/16), padded to 16 bytes]
   // sort scratch
   uint32_t *keys, *keys_alt, *vals, *vals_alt, *hist, *partials;
   // grid sums of the last grid update, {fixed-point mass (int64 bits),
